@@ -1,0 +1,259 @@
+"""Python binding of the MI355X BFS engine (libbfsx.so, C ABI in include/bfsx.h).
+
+The engine is native: every call goes through the C ABI into HIP kernels for
+gfx950.  There is no Python or CPU fallback; if libbfsx.so is missing or no
+device is visible, the calls raise.
+
+    from bfsx import Engine
+    eng = Engine()
+    eng.generate_rmat(19, 16, seed_graph=1002, seed_perm=2002)
+    roots = eng.pick_roots(64, seed=3002)
+    levels, stats = eng.run(int(roots[0]), "llb")
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbfsx.so")
+
+LLB, GLB, SLB, PULL, LEARNED = 0, 1, 2, 3, 4
+STRATEGIES = {"llb": LLB, "local": LLB, "glb": GLB, "global": GLB, "slb": SLB,
+              "local_strided": SLB, "pull": PULL, "learned": LEARNED}
+STEP_NAMES = ["llb", "glb", "slb", "pull"]
+GRAPH_SYMMETRIC = 1
+RUN_GRAPH_LOOP = 1
+RUN_TRACE = 2
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "out of memory",
+          4: "invalid state", 5: "i/o error", 6: "overflow", 7: "unsupported"}
+
+
+class BfsxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+        self.msg = msg
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("n_rows", C.c_uint32), ("n_cols", C.c_uint32), ("nnz", C.c_uint64),
+                ("col_ptr_bytes", C.c_int32), ("flags", C.c_uint32),
+                ("max_degree", C.c_uint32), ("n_isolated", C.c_uint32)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("edges_count", C.c_uint64), ("n_levels", C.c_uint32),
+                ("max_level", C.c_uint32), ("reached", C.c_uint64),
+                ("steps", C.c_uint32 * 4)]
+
+    def as_dict(self):
+        return {"ms": self.ms, "edges_count": self.edges_count, "n_levels": self.n_levels,
+                "max_level": self.max_level, "reached": self.reached,
+                "steps": {STEP_NAMES[i]: self.steps[i] for i in range(4)}}
+
+
+class LevelRecord(C.Structure):
+    _fields_ = [("level", C.c_uint32), ("choice", C.c_uint32),
+                ("frontier_vertices", C.c_uint64), ("frontier_edges", C.c_uint64),
+                ("unvisited_vertices", C.c_uint64), ("unvisited_edges", C.c_uint64),
+                ("discovered", C.c_uint64), ("us", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [("llb_wg", C.c_uint32), ("glb_wg", C.c_uint32),
+                ("glb_blocks_per_cu", C.c_uint32), ("glb_chunk", C.c_uint32),
+                ("hub_degree", C.c_uint32), ("pull_wg", C.c_uint32),
+                ("graph_levels", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+P = C.c_void_p
+_lib = None
+
+# every symbol include/bfsx.h declares, with (restype, argtypes)
+SIGNATURES = {
+    "bfsx_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+    "bfsx_destroy": (None, [P]),
+    "bfsx_last_error": (C.c_char_p, [P]),
+    "bfsx_status_string": (C.c_char_p, [C.c_int]),
+    "bfsx_device_info": (C.c_int, [P, C.c_char_p, C.c_size_t, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_uint64)]),
+    "bfsx_upload_csc": (C.c_int, [P, C.c_uint32, C.c_uint32, P, C.c_int, P, C.c_uint64,
+                                  C.c_uint32]),
+    "bfsx_generate_rmat": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_uint64, C.c_uint64, C.c_int, C.POINTER(C.c_uint64)]),
+    "bfsx_generate_grid": (C.c_int, [P, C.c_uint32, C.c_double, C.c_uint32, C.c_uint64, C.c_int,
+                                     C.POINTER(C.c_uint64)]),
+    "bfsx_graph_info_get": (C.c_int, [P, C.POINTER(GraphInfo)]),
+    "bfsx_download_csc": (C.c_int, [P, P, P]),
+    "bfsx_pick_roots": (C.c_int, [P, C.c_uint32, C.c_uint64, C.c_int, P,
+                                  C.POINTER(C.c_uint32)]),
+    "bfsx_run": (C.c_int, [P, C.c_uint32, C.c_int, C.c_uint32, P, P, C.POINTER(RunStats)]),
+    "bfsx_run_batch": (C.c_int, [P, P, C.c_uint32, C.c_int, C.c_uint32, P, P]),
+    "bfsx_last_trace": (C.c_int, [P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "bfsx_get_params": (C.c_int, [P, C.POINTER(Params)]),
+    "bfsx_set_params": (C.c_int, [P, C.POINTER(Params)]),
+}
+
+
+def lib():
+    """Loads libbfsx.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(P) if a is not None else None
+
+
+def strategy_id(s):
+    if isinstance(s, str):
+        return STRATEGIES[s.lower()]
+    return int(s)
+
+
+class Engine:
+    """One bfsx context: a device, a stream and a resident graph."""
+
+    def __init__(self, device=0):
+        self._h = P()
+        rc = lib().bfsx_create(device, C.byref(self._h))
+        if rc != 0:
+            raise BfsxError(rc, "bfsx_create failed (no usable HIP device?)")
+
+    def _check(self, rc):
+        if rc != 0:
+            raise BfsxError(rc, lib().bfsx_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().bfsx_destroy(self._h)
+            self._h = P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- device / graph -------------------------------------------------
+    def device_info(self):
+        name = C.create_string_buffer(128)
+        cus = C.c_int(0)
+        hbm = C.c_uint64(0)
+        self._check(lib().bfsx_device_info(self._h, name, 128, C.byref(cus), C.byref(hbm)))
+        return {"arch": name.value.decode(), "compute_units": cus.value, "hbm_bytes": hbm.value}
+
+    def upload_csc(self, col_ptr, row_idx, n_rows=None, symmetric=False):
+        col_ptr = np.ascontiguousarray(col_ptr)
+        if col_ptr.dtype not in (np.uint32, np.uint64):
+            raise TypeError("col_ptr must be uint32 or uint64")
+        row_idx = np.ascontiguousarray(row_idx, np.uint32)
+        n_cols = len(col_ptr) - 1
+        n_rows = n_cols if n_rows is None else n_rows
+        self._check(lib().bfsx_upload_csc(self._h, n_cols, n_rows, _ptr(col_ptr),
+                                          col_ptr.dtype.itemsize, _ptr(row_idx), len(row_idx),
+                                          GRAPH_SYMMETRIC if symmetric else 0))
+
+    def generate_rmat(self, scale, edge_factor, a=0.57, b=0.19, c=0.19, seed_graph=1,
+                      seed_perm=2, ptr_bytes=4):
+        nnz = C.c_uint64(0)
+        self._check(lib().bfsx_generate_rmat(self._h, scale, edge_factor, a, b, c, seed_graph,
+                                             seed_perm, ptr_bytes, C.byref(nnz)))
+        return nnz.value
+
+    def generate_grid(self, side, drop_prob=0.05, n_long=0, seed=1, ptr_bytes=4):
+        nnz = C.c_uint64(0)
+        self._check(lib().bfsx_generate_grid(self._h, side, drop_prob, n_long, seed, ptr_bytes,
+                                             C.byref(nnz)))
+        return nnz.value
+
+    def info(self):
+        g = GraphInfo()
+        self._check(lib().bfsx_graph_info_get(self._h, C.byref(g)))
+        return {k: getattr(g, k) for k, _ in g._fields_}
+
+    def download_csc(self):
+        g = self.info()
+        cp = np.empty(g["n_cols"] + 1, np.uint64 if g["col_ptr_bytes"] == 8 else np.uint32)
+        ri = np.empty(max(g["nnz"], 1), np.uint32)
+        self._check(lib().bfsx_download_csc(self._h, _ptr(cp), _ptr(ri)))
+        return cp, ri[:g["nnz"]]
+
+    def pick_roots(self, n, seed, mode=0):
+        out = np.zeros(n, np.uint32)
+        got = C.c_uint32(0)
+        self._check(lib().bfsx_pick_roots(self._h, n, seed, mode, _ptr(out), C.byref(got)))
+        return out[:got.value].copy()
+
+    # ---- search ---------------------------------------------------------
+    def run(self, root, strategy="llb", trace=False, parents=False, levels=True,
+            graph_loop=False):
+        g = self.info()
+        lev = np.empty(g["n_cols"], np.int32) if levels else None
+        par = np.empty(g["n_cols"], np.uint32) if parents else None
+        st = RunStats()
+        flags = (RUN_TRACE if trace else 0) | (RUN_GRAPH_LOOP if graph_loop else 0)
+        self._check(lib().bfsx_run(self._h, int(root), strategy_id(strategy), flags, _ptr(lev),
+                                   _ptr(par), C.byref(st)))
+        out = [lev, st.as_dict()]
+        if parents:
+            out.append(par)
+        return tuple(out)
+
+    def run_batch(self, roots, strategy="llb", levels=False, trace=False, graph_loop=False):
+        roots = np.ascontiguousarray(roots, np.uint32)
+        g = self.info()
+        lev = np.empty((len(roots), g["n_cols"]), np.int32) if levels else None
+        st = (RunStats * max(len(roots), 1))()
+        flags = (RUN_TRACE if trace else 0) | (RUN_GRAPH_LOOP if graph_loop else 0)
+        self._check(lib().bfsx_run_batch(self._h, _ptr(roots), len(roots), strategy_id(strategy),
+                                         flags, _ptr(lev), C.cast(st, P)))
+        return lev, [st[i].as_dict() for i in range(len(roots))]
+
+    def last_trace(self):
+        n = C.c_uint32(0)
+        self._check(lib().bfsx_last_trace(self._h, None, 0, C.byref(n)))
+        recs = (LevelRecord * max(n.value, 1))()
+        self._check(lib().bfsx_last_trace(self._h, C.cast(recs, P), n.value, C.byref(n)))
+        return [{k: getattr(recs[i], k) for k, _ in LevelRecord._fields_} for i in range(n.value)]
+
+    @property
+    def params(self):
+        p = Params()
+        self._check(lib().bfsx_get_params(self._h, C.byref(p)))
+        return p.as_dict()
+
+    def set_params(self, **kw):
+        p = Params()
+        self._check(lib().bfsx_get_params(self._h, C.byref(p)))
+        for k, v in kw.items():
+            setattr(p, k, int(v))
+        self._check(lib().bfsx_set_params(self._h, C.byref(p)))
+
+
+def harmonic_gteps(edges, ms):
+    """Harmonic mean over roots of edges/time, in GTEPS: n / sum(t_r / E_r)."""
+    edges = np.asarray(edges, np.float64)
+    ms = np.asarray(ms, np.float64)
+    return len(edges) / np.sum((ms * 1e-3) / edges) / 1e9

@@ -1,0 +1,689 @@
+// Level-synchronous BFS step kernels for gfx950 (wave64).
+//
+// Reference path (src/run_bfs_gpu.hpp): per level an explore kernel marks
+// neighbours of the frontier in byte masks (LLB/SLB :388-477, GLB :175-385)
+// and a level-generation kernel scans all V vertices to turn marks into the
+// next frontier (:481-521).  Here discovery is exact-once through a visited
+// bitmap with atomicOr, and the discovering lane writes the level and
+// appends the vertex to the next frontier in the same kernel: one ballot and
+// one atomicAdd per wave give every new vertex its slot (mbcnt prefix), so
+// there is no O(V) level-generation pass.
+//
+// Step kinds:
+//   LLB   a work-group takes WG consecutive frontier entries, scans their
+//         degrees in registers/LDS and expands all their edges cooperatively
+//         (binary search of the edge index over the LDS scan) -- the
+//         reference's local load balancing.  Hubs above `hub_degree` are
+//         taken out of the scan and expanded by a whole wave each.
+//   SLB   the same kernel with the strided entry mapping of
+//         run_bfs_gpu.hpp:416-417 (group + lane * n_groups).
+//   GLB   device-wide scan of frontier degrees, then a persistent grid pulls
+//         fixed-size edge chunks from an atomic work queue; each chunk finds
+//         its frontier entries from a precomputed chunk->entry table and
+//         stages them in LDS (no global binary search per edge).
+//   PULL  bottom-up: every unvisited vertex looks for a neighbour in the
+//         frontier bitmap and stops at the first hit; a wave owns 64
+//         consecutive vertices, so bitmap words are written without atomics.
+#include "bfsx_internal.hpp"
+
+namespace bfsx {
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+  const int l = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T y = __shfl_up(x, d, 64);
+    if (l >= d) x += y;
+  }
+  return x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+// Exclusive scan over a work-group; s_wave holds WG/64 + 1 entries.
+template <int WG, class T>
+__device__ __forceinline__ T block_excl_scan(T x, T &total, T *s_wave) {
+  constexpr int NW = WG / 64;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const T inc = wave_incl_scan(x);
+  if (NW == 1) {
+    total = __shfl(inc, 63, 64);
+    return inc - x;
+  }
+  if (l == 63) s_wave[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    const T v = (l < NW) ? s_wave[l] : T(0);
+    const T vi = wave_incl_scan(v);
+    if (l < NW) s_wave[l] = vi - v;
+    if (l == NW - 1) s_wave[NW] = vi;
+  }
+  __syncthreads();
+  total = s_wave[NW];
+  return s_wave[w] + inc - x;
+}
+
+// Exact-once discovery: a plain read filters vertices already visited (the
+// bitmap is small enough to stay in L2/MALL), the atomicOr decides the winner.
+__device__ __forceinline__ bool try_visit(uint32_t *vis, uint32_t n) {
+  const uint32_t w = n >> 5, bit = 1u << (n & 31);
+  if (vis[w] & bit) return false;
+  return !(atomicOr(&vis[w], bit) & bit);
+}
+
+// Wave-aggregated append: one atomicAdd per wave, slots by mbcnt prefix.
+// Every active lane of the wave must call it.
+__device__ __forceinline__ void wave_append(bool isnew, uint32_t n, uint32_t *q_out,
+                                            uint32_t *q_count) {
+  const unsigned long long m = __ballot(isnew);
+  if (m == 0) return;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)__lane_id() == leader) base = atomicAdd(q_count, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (isnew) q_out[base + lanes_below(m)] = n;
+}
+
+// Degree sum of the discovered vertices of a wave, one atomic per wave.
+template <typename PtrT>
+__device__ __forceinline__ void wave_track_degree(bool isnew, uint32_t n, uint32_t n_rows,
+                                                  const PtrT *col_ptr,
+                                                  unsigned long long *acc) {
+  const unsigned long long m = __ballot(isnew);
+  if (m == 0) return;
+  unsigned long long d = 0;
+  if (isnew && n < n_rows) d = (unsigned long long)(col_ptr[n + 1] - col_ptr[n]);
+  d = wave_sum(d);
+  if ((int)__lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(acc, d);
+}
+
+struct KArgs {
+  const void *col_ptr;
+  const uint32_t *row_idx;
+  uint32_t n_rows, n_cols;
+  const uint32_t *q_in;
+  uint32_t F;
+  uint32_t *q_out;
+  uint32_t *q_out_count;
+  unsigned long long *q_out_edges;  // nullable
+  uint32_t *vis;
+  int32_t *levels;
+  uint32_t *parents;  // nullable
+  int32_t next_level;
+  uint32_t hub_degree;
+};
+
+template <typename PtrT>
+__device__ __forceinline__ void discover(const KArgs &a, const PtrT *col_ptr, uint32_t n,
+                                         uint32_t from) {
+  const bool isnew = try_visit(a.vis, n);
+  if (isnew) {
+    a.levels[n] = a.next_level;
+    if (a.parents) a.parents[n] = from;
+  }
+  wave_append(isnew, n, a.q_out, a.q_out_count);
+  if (a.q_out_edges) wave_track_degree(isnew, n, a.n_rows, col_ptr, a.q_out_edges);
+}
+
+// ---------------------------------------------------------------------------
+// LLB / SLB
+// ---------------------------------------------------------------------------
+template <typename PtrT, int WG, bool STRIDED>
+__global__ __launch_bounds__(WG) void k_llb(KArgs a) {
+  constexpr int NW = WG / 64;
+  __shared__ PtrT s_start[WG];
+  __shared__ PtrT s_scan[WG];
+  __shared__ PtrT s_wave[NW + 1];
+  __shared__ uint32_t s_v[WG];
+  __shared__ uint32_t s_hub_n;
+  __shared__ uint32_t s_hub[WG];
+  const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
+  const uint32_t t = threadIdx.x;
+  const uint32_t idx = STRIDED ? blockIdx.x + t * gridDim.x : blockIdx.x * WG + t;
+  if (t == 0) s_hub_n = 0;
+  PtrT deg = 0, st = 0;
+  uint32_t v = 0;
+  if (idx < a.F) {
+    v = a.q_in[idx];
+    if (v < a.n_rows) {
+      st = col_ptr[v];
+      deg = col_ptr[v + 1] - st;
+    }
+  }
+  __syncthreads();
+  // Hubs leave the block scan and are expanded wave-per-vertex below.
+  if (a.hub_degree && deg >= (PtrT)a.hub_degree) {
+    s_hub[atomicAdd(&s_hub_n, 1u)] = t;
+  }
+  s_start[t] = st;
+  s_v[t] = v;
+  const PtrT sdeg = (a.hub_degree && deg >= (PtrT)a.hub_degree) ? PtrT(0) : deg;
+  PtrT total;
+  const PtrT ex = block_excl_scan<WG>(sdeg, total, s_wave);
+  s_scan[t] = ex;
+  __syncthreads();
+  for (PtrT i = t; i < total; i += WG) {
+    // last slot j with s_scan[j] <= i (its range holds i: empty slots tie
+    // with the next one and are skipped by taking the last)
+    uint32_t j = 0;
+#pragma unroll
+    for (uint32_t step = WG / 2; step > 0; step >>= 1)
+      if (s_scan[j + step] <= i) j += step;
+    const PtrT e = s_start[j] + (i - s_scan[j]);
+    discover<PtrT>(a, col_ptr, a.row_idx[e], s_v[j]);
+  }
+  if (a.hub_degree) {
+    const uint32_t nh = s_hub_n;
+    const uint32_t w = t >> 6, l = t & 63;
+    for (uint32_t h = w; h < nh; h += NW) {
+      const uint32_t slot = s_hub[h];
+      const PtrT b = s_start[slot];
+      const PtrT e1 = col_ptr[s_v[slot] + 1];
+      for (PtrT e = b + l; e < e1; e += 64) discover<PtrT>(a, col_ptr, a.row_idx[e], s_v[slot]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GLB: tile scan -> tile-offset scan -> global offsets + chunk table -> expand
+// ---------------------------------------------------------------------------
+constexpr int GLB_TILE = 2048;  // frontier entries per scan tile (256 x 8)
+
+template <typename PtrT>
+__global__ __launch_bounds__(256) void k_glb_tiles(const PtrT *col_ptr, uint32_t n_rows,
+                                                   const uint32_t *q_in, uint32_t F,
+                                                   PtrT *g_loc, PtrT *g_start,
+                                                   PtrT *tile_sum) {
+  __shared__ PtrT s_wave[256 / 64 + 1];
+  const uint32_t base = blockIdx.x * GLB_TILE + threadIdx.x * 8;
+  PtrT d[8];
+  PtrT sum = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t i = base + k;
+    PtrT s = 0, dg = 0;
+    if (i < F) {
+      const uint32_t v = q_in[i];
+      if (v < n_rows) {
+        s = col_ptr[v];
+        dg = col_ptr[v + 1] - s;
+      }
+      g_start[i] = s;
+    }
+    d[k] = dg;
+    sum += dg;
+  }
+  PtrT total;
+  PtrT run = block_excl_scan<256>(sum, total, s_wave);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t i = base + k;
+    if (i < F) g_loc[i] = run;
+    run += d[k];
+  }
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// One work-group turns tile sums into exclusive tile offsets, publishes the
+// frontier's edge total and chunk count, and resets the work-queue head.
+template <typename PtrT>
+__global__ __launch_bounds__(1024) void k_glb_tile_offsets(PtrT *tile_sum, uint32_t n_tiles,
+                                                           uint32_t chunk, LevelCtl *ctl,
+                                                           PtrT *g_scan_end) {
+  __shared__ PtrT s_wave[1024 / 64 + 1];
+  __shared__ PtrT s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t b = 0; b < n_tiles; b += 4096) {
+    PtrT v[4];
+    PtrT sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t i = b + threadIdx.x * 4 + k;
+      v[k] = i < n_tiles ? tile_sum[i] : PtrT(0);
+      sum += v[k];
+    }
+    PtrT total;
+    PtrT run = block_excl_scan<1024>(sum, total, s_wave) + s_carry;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t i = b + threadIdx.x * 4 + k;
+      if (i < n_tiles) tile_sum[i] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const PtrT tot = s_carry;
+    *g_scan_end = tot;
+    ctl->glb_total = (unsigned long long)tot;
+    ctl->glb_chunks = (uint32_t)(((unsigned long long)tot + chunk - 1) / chunk);
+    ctl->glb_cursor = 0;
+  }
+}
+
+// Global exclusive offsets, and chunk_lo[c] = frontier entry holding edge c*chunk.
+template <typename PtrT>
+__global__ void k_glb_offsets(const PtrT *g_loc, const PtrT *tile_off, const PtrT *g_scan_end,
+                              uint32_t F, PtrT *g_scan, uint32_t chunk, uint32_t *chunk_lo) {
+  const PtrT total = *g_scan_end;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F; i += gridDim.x * blockDim.x) {
+    const PtrT ex = g_loc[i] + tile_off[i / GLB_TILE];
+    PtrT in;
+    if (i + 1 < F)
+      in = g_loc[i + 1] + tile_off[(i + 1) / GLB_TILE];
+    else
+      in = total;
+    g_scan[i] = ex;
+    // chunks whose first edge falls in [ex, in)
+    const unsigned long long c0 = ((unsigned long long)ex + chunk - 1) / chunk;
+    const unsigned long long c1 = ((unsigned long long)in + chunk - 1) / chunk;
+    for (unsigned long long c = c0; c < c1; c++) chunk_lo[c] = i;
+  }
+}
+
+template <typename PtrT, int WG>
+__global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
+                                                   const PtrT *g_start,
+                                                   const uint32_t *chunk_lo, LevelCtl *ctl,
+                                                   uint32_t chunk) {
+  __shared__ PtrT s_scan[WG];
+  __shared__ PtrT s_start[WG];
+  __shared__ uint32_t s_v[WG];
+  __shared__ PtrT s_win_lo, s_win_hi;
+  __shared__ uint32_t s_c;
+  const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
+  const uint32_t t = threadIdx.x;
+  const uint32_t n_chunks = ctl->glb_chunks;
+  const PtrT total = (PtrT)ctl->glb_total;
+  for (;;) {
+    if (t == 0) s_c = atomicAdd(&ctl->glb_cursor, 1u);
+    __syncthreads();
+    const uint32_t c = s_c;
+    __syncthreads();
+    if (c >= n_chunks) break;
+    const PtrT e0 = (PtrT)c * chunk;
+    const PtrT e1 = (e0 + chunk < total) ? e0 + chunk : total;
+    const uint32_t ilo = chunk_lo[c];
+    const uint32_t ihi = (c + 1 < n_chunks) ? chunk_lo[c + 1] : a.F - 1;
+    for (uint32_t wb = ilo; wb <= ihi; wb += WG) {
+      const uint32_t nwin = (ihi + 1 - wb < WG) ? ihi + 1 - wb : WG;
+      if (t < nwin) {
+        s_scan[t] = g_scan[wb + t];
+        s_start[t] = g_start[wb + t];
+        if (a.parents) s_v[t] = a.q_in[wb + t];
+      } else {
+        s_scan[t] = ~PtrT(0);
+      }
+      if (t == 0) {
+        s_win_lo = g_scan[wb];
+        s_win_hi = (wb + nwin < a.F) ? g_scan[wb + nwin] : total;
+      }
+      __syncthreads();
+      const PtrT lo = e0 > s_win_lo ? e0 : s_win_lo;
+      const PtrT hi = e1 < s_win_hi ? e1 : s_win_hi;
+      for (PtrT i = lo + t; i < hi; i += WG) {
+        uint32_t j = 0;
+#pragma unroll
+        for (uint32_t step = WG / 2; step > 0; step >>= 1)
+          if (s_scan[j + step] <= i) j += step;
+        const PtrT e = s_start[j] + (i - s_scan[j]);
+        discover<PtrT>(a, col_ptr, a.row_idx[e], a.parents ? s_v[j] : 0u);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PULL (bottom-up)
+// ---------------------------------------------------------------------------
+template <typename PtrT, int WG>
+__global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
+                                             uint32_t *fbm_next, uint32_t n_groups) {
+  const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nwaves = gridDim.x * (WG / 64);
+  for (uint32_t g = blockIdx.x * (WG / 64) + (threadIdx.x >> 6); g < n_groups; g += nwaves) {
+    const uint32_t u = g * 64 + lane;
+    const uint32_t vword = a.vis[u >> 5];
+    const bool cand = u < a.n_rows && !((vword >> (u & 31)) & 1u);
+    bool found = false;
+    uint32_t par = 0;
+    unsigned long long deg = 0;
+    if (cand) {
+      const PtrT b = col_ptr[u], e = col_ptr[u + 1];
+      deg = (unsigned long long)(e - b);
+      for (PtrT k = b; k < e; k++) {
+        const uint32_t w = a.row_idx[k];
+        if ((fbm_cur[w >> 5] >> (w & 31)) & 1u) {
+          found = true;
+          par = w;
+          break;
+        }
+      }
+    }
+    const unsigned long long m = __ballot(found);
+    if (lane == 0) {
+      fbm_next[2 * g] = (uint32_t)m;
+      if ((uint32_t)m) a.vis[2 * g] = vword | (uint32_t)m;
+    } else if (lane == 32) {
+      fbm_next[2 * g + 1] = (uint32_t)(m >> 32);
+      if ((uint32_t)(m >> 32)) a.vis[2 * g + 1] = vword | (uint32_t)(m >> 32);
+    }
+    if (m) {
+      if (found) {
+        a.levels[u] = a.next_level;
+        if (a.parents) a.parents[u] = par;
+      }
+      wave_append(found, u, a.q_out, a.q_out_count);
+      if (a.q_out_edges) {
+        const unsigned long long s = wave_sum(found ? deg : 0ull);
+        if ((int)lane == __ffsll((unsigned long long)m) - 1) atomicAdd(a.q_out_edges, s);
+      }
+    }
+  }
+}
+
+// frontier bitmap of one level from the levels array (a wave owns 64 vertices)
+__global__ void k_levels_to_fbm(const int32_t *levels, uint32_t n_cols, int32_t level,
+                                uint32_t *fbm, uint32_t n_groups) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
+  for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < n_groups;
+       g += nwaves) {
+    const uint32_t u = g * 64 + lane;
+    const bool f = u < n_cols && levels[u] == level;
+    const unsigned long long m = __ballot(f);
+    if (lane == 0) fbm[2 * g] = (uint32_t)m;
+    if (lane == 32) fbm[2 * g + 1] = (uint32_t)(m >> 32);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// search set-up and per-level bookkeeping
+// ---------------------------------------------------------------------------
+template <typename PtrT>
+__global__ void k_init(const PtrT *col_ptr, uint32_t n_rows, uint32_t n_cols, uint32_t root,
+                       int32_t *levels, uint32_t *parents, uint32_t *vis,
+                       const uint32_t *iso_bm, uint32_t n_words, uint32_t *queue0,
+                       LevelCtl *ctl) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_cols; i += stride) {
+    levels[i] = (i == root) ? 0 : -1;
+    if (parents) parents[i] = (i == root) ? root : 0xFFFFFFFFu;
+  }
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n_words; w += stride) {
+    uint32_t x = iso_bm ? iso_bm[w] : 0u;
+    if (w == (root >> 5)) x |= 1u << (root & 31);
+    vis[w] = x;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    queue0[0] = root;
+    const unsigned long long d =
+        root < n_rows ? (unsigned long long)(col_ptr[root + 1] - col_ptr[root]) : 0ull;
+    LevelCtl c{};
+    c.q_count[0] = 1;
+    c.q_count[1] = 0;
+    c.level = 0;
+    c.q_edges[0] = d;
+    c.visited_vertices = 1;
+    c.visited_edges = d;
+    c.choice = BFSX_LLB;
+    *ctl = c;
+  }
+}
+
+__global__ void k_level_begin(LevelCtl *ctl) {
+  if (threadIdx.x == 0) ctl->t_begin = (unsigned long long)wall_clock64();
+}
+
+__global__ void k_level_end(LevelCtl *ctl, uint32_t consumed, uint32_t produced, int stamp) {
+  if (threadIdx.x == 0) {
+    if (stamp) ctl->t_end = (unsigned long long)wall_clock64();
+    const uint32_t n = ctl->q_count[produced];
+    ctl->discovered = n;
+    ctl->visited_vertices += n;
+    ctl->visited_edges += ctl->q_edges[produced];
+    ctl->q_count[consumed] = 0;
+    ctl->q_edges[consumed] = 0;
+    ctl->glb_cursor = 0;
+    ctl->level += 1;
+    ctl->done = n == 0;
+  }
+}
+
+// sum of out-degree over reached vertices < n_rows, reached count, max level
+template <typename PtrT>
+__global__ void k_summarize(const PtrT *col_ptr, const int32_t *levels, uint32_t n_rows,
+                            uint32_t n_cols, unsigned long long *out) {
+  unsigned long long edges = 0, reached = 0;
+  int32_t mx = -1;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_cols;
+       i += gridDim.x * blockDim.x) {
+    const int32_t l = levels[i];
+    if (l >= 0) {
+      reached++;
+      mx = l > mx ? l : mx;
+      if (i < n_rows) edges += (unsigned long long)(col_ptr[i + 1] - col_ptr[i]);
+    }
+  }
+  edges = wave_sum(edges);
+  reached = wave_sum(reached);
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const int32_t o = __shfl_xor(mx, d, 64);
+    mx = o > mx ? o : mx;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (edges) atomicAdd(&out[0], edges);
+    if (reached) atomicAdd(&out[1], reached);
+    if (mx >= 0) atomicMax(&out[2], (unsigned long long)mx);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static unsigned cap_grid(uint64_t blocks, unsigned cap) {
+  if (blocks == 0) return 1;
+  return blocks > cap ? cap : (unsigned)blocks;
+}
+
+static KArgs kargs(const StepArgs &s, uint32_t hub) {
+  KArgs a;
+  a.col_ptr = s.col_ptr;
+  a.row_idx = s.row_idx;
+  a.n_rows = s.n_rows;
+  a.n_cols = s.n_cols;
+  a.q_in = s.q_in;
+  a.F = s.F;
+  a.q_out = s.q_out;
+  a.q_out_count = &s.ctl->q_count[s.q_out_idx];
+  a.q_out_edges = s.track_degrees ? &s.ctl->q_edges[s.q_out_idx] : nullptr;
+  a.vis = s.vis;
+  a.levels = s.levels;
+  a.parents = s.parents;
+  a.next_level = s.next_level;
+  a.hub_degree = hub;
+  return a;
+}
+
+template <typename PtrT, bool STRIDED>
+static void launch_llb_t(bfsx_ctx *ctx, const KArgs &a) {
+  const uint32_t wg = ctx->params.llb_wg;
+  const unsigned grid = (unsigned)((a.F + wg - 1) / wg);
+  hipStream_t s = ctx->stream;
+  switch (wg) {
+    case 64: hipLaunchKernelGGL((k_llb<PtrT, 64, STRIDED>), dim3(grid), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL((k_llb<PtrT, 128, STRIDED>), dim3(grid), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((k_llb<PtrT, 256, STRIDED>), dim3(grid), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((k_llb<PtrT, 512, STRIDED>), dim3(grid), dim3(512), 0, s, a); break;
+    case 1024: hipLaunchKernelGGL((k_llb<PtrT, 1024, STRIDED>), dim3(grid), dim3(1024), 0, s, a); break;
+    default: fail(BFSX_ERR_ARG, "llb_wg must be 64, 128, 256, 512 or 1024");
+  }
+}
+
+void step_llb(bfsx_ctx *ctx, const StepArgs &s, bool strided) {
+  if (s.F == 0) return;
+  const KArgs a = kargs(s, ctx->params.hub_degree);
+  if (ctx->ptr_bytes == 8) {
+    if (strided) launch_llb_t<uint64_t, true>(ctx, a);
+    else launch_llb_t<uint64_t, false>(ctx, a);
+  } else {
+    if (strided) launch_llb_t<uint32_t, true>(ctx, a);
+    else launch_llb_t<uint32_t, false>(ctx, a);
+  }
+  BFSX_HIP(hipGetLastError());
+}
+
+template <typename PtrT>
+static void step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
+  hipStream_t st = ctx->stream;
+  const uint32_t F = s.F;
+  const uint32_t chunk = ctx->params.glb_chunk;
+  const uint32_t n_tiles = (F + GLB_TILE - 1) / GLB_TILE;
+  PtrT *g_loc = ctx->glb_scan.as<PtrT>();             // F entries, then F+1 global
+  PtrT *g_scan = g_loc + ((uint64_t)ctx->n_cols + 1);
+  PtrT *g_start = ctx->glb_start.as<PtrT>();
+  PtrT *tile = ctx->glb_tile.as<PtrT>();
+  PtrT *g_end = tile + ((uint64_t)ctx->n_cols / GLB_TILE + 2);
+  hipLaunchKernelGGL(k_glb_tiles<PtrT>, dim3(n_tiles), dim3(256), 0, st,
+                     static_cast<const PtrT *>(s.col_ptr), s.n_rows, s.q_in, F, g_loc, g_start,
+                     tile);
+  hipLaunchKernelGGL(k_glb_tile_offsets<PtrT>, dim3(1), dim3(1024), 0, st, tile, n_tiles, chunk,
+                     s.ctl, g_end);
+  hipLaunchKernelGGL(k_glb_offsets<PtrT>, dim3(cap_grid((F + 255) / 256, 65535)), dim3(256), 0,
+                     st, g_loc, tile, g_end, F, g_scan, chunk, ctx->glb_chunk_lo.as<uint32_t>());
+  const KArgs a = kargs(s, 0);
+  const unsigned grid = (unsigned)ctx->cu_count * ctx->params.glb_blocks_per_cu;
+  switch (ctx->params.glb_wg) {
+    case 128:
+      hipLaunchKernelGGL((k_glb_expand<PtrT, 128>), dim3(grid), dim3(128), 0, st, a, g_scan,
+                         g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      break;
+    case 256:
+      hipLaunchKernelGGL((k_glb_expand<PtrT, 256>), dim3(grid), dim3(256), 0, st, a, g_scan,
+                         g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      break;
+    case 512:
+      hipLaunchKernelGGL((k_glb_expand<PtrT, 512>), dim3(grid), dim3(512), 0, st, a, g_scan,
+                         g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      break;
+    case 1024:
+      hipLaunchKernelGGL((k_glb_expand<PtrT, 1024>), dim3(grid), dim3(1024), 0, st, a, g_scan,
+                         g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      break;
+    default: fail(BFSX_ERR_ARG, "glb_wg must be 128, 256, 512 or 1024");
+  }
+}
+
+void step_glb(bfsx_ctx *ctx, const StepArgs &s) {
+  if (s.F == 0) return;
+  const uint64_t pw = ctx->ptr_bytes;
+  // scratch: local + global scans, starts, tile sums, chunk table
+  ensure(ctx->glb_scan, pw * 2 * ((uint64_t)ctx->n_cols + 1), ctx->stream);
+  ensure(ctx->glb_start, pw * ((uint64_t)ctx->n_cols + 1), ctx->stream);
+  ensure(ctx->glb_tile, pw * ((uint64_t)ctx->n_cols / GLB_TILE + 4), ctx->stream);
+  ensure(ctx->glb_chunk_lo, sizeof(uint32_t) * (ctx->nnz / ctx->params.glb_chunk + 2),
+         ctx->stream);
+  if (ctx->ptr_bytes == 8) step_glb_t<uint64_t>(ctx, s);
+  else step_glb_t<uint32_t>(ctx, s);
+  BFSX_HIP(hipGetLastError());
+}
+
+template <typename PtrT>
+static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
+  const KArgs a = kargs(s, 0);
+  const uint32_t n_groups = (s.n_cols + 63) / 64;
+  const uint32_t wg = ctx->params.pull_wg;
+  const unsigned grid = cap_grid(((uint64_t)n_groups * 64 + wg - 1) / wg,
+                                 (unsigned)ctx->cu_count * 32);
+  hipStream_t st = ctx->stream;
+  switch (wg) {
+    case 64: hipLaunchKernelGGL((k_pull<PtrT, 64>), dim3(grid), dim3(64), 0, st, a, s.fbm_cur, s.fbm_next, n_groups); break;
+    case 128: hipLaunchKernelGGL((k_pull<PtrT, 128>), dim3(grid), dim3(128), 0, st, a, s.fbm_cur, s.fbm_next, n_groups); break;
+    case 256: hipLaunchKernelGGL((k_pull<PtrT, 256>), dim3(grid), dim3(256), 0, st, a, s.fbm_cur, s.fbm_next, n_groups); break;
+    case 512: hipLaunchKernelGGL((k_pull<PtrT, 512>), dim3(grid), dim3(512), 0, st, a, s.fbm_cur, s.fbm_next, n_groups); break;
+    default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
+  }
+}
+
+void step_pull(bfsx_ctx *ctx, const StepArgs &s) {
+  if (ctx->ptr_bytes == 8) step_pull_t<uint64_t>(ctx, s);
+  else step_pull_t<uint32_t>(ctx, s);
+  BFSX_HIP(hipGetLastError());
+}
+
+void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level, uint32_t *fbm) {
+  const uint32_t n_groups = (ctx->n_cols + 63) / 64;
+  const unsigned grid = cap_grid(((uint64_t)n_groups * 64 + 255) / 256,
+                                 (unsigned)ctx->cu_count * 32);
+  hipLaunchKernelGGL(k_levels_to_fbm, dim3(grid), dim3(256), 0, ctx->stream, levels,
+                     ctx->n_cols, level, fbm, n_groups);
+  BFSX_HIP(hipGetLastError());
+}
+
+void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *par) {
+  const uint32_t n_words = ((ctx->n_cols + 63) / 64) * 2;
+  const unsigned grid = cap_grid(((uint64_t)ctx->n_cols + 255) / 256,
+                                 (unsigned)ctx->cu_count * 16);
+  const uint32_t *iso = symmetric_iso ? ctx->iso_bm.as<uint32_t>() : nullptr;
+  if (ctx->ptr_bytes == 8)
+    hipLaunchKernelGGL(k_init<uint64_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->col_ptr.as<uint64_t>(), ctx->n_rows, ctx->n_cols, root,
+                       ctx->levels.as<int32_t>(), par, ctx->vis.as<uint32_t>(), iso, n_words,
+                       ctx->queue[0].as<uint32_t>(), ctx->ctl.as<LevelCtl>());
+  else
+    hipLaunchKernelGGL(k_init<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->col_ptr.as<uint32_t>(), ctx->n_rows, ctx->n_cols, root,
+                       ctx->levels.as<int32_t>(), par, ctx->vis.as<uint32_t>(), iso, n_words,
+                       ctx->queue[0].as<uint32_t>(), ctx->ctl.as<LevelCtl>());
+  BFSX_HIP(hipGetLastError());
+}
+
+void level_begin(bfsx_ctx *ctx) {
+  hipLaunchKernelGGL(k_level_begin, dim3(1), dim3(64), 0, ctx->stream, ctx->ctl.as<LevelCtl>());
+  BFSX_HIP(hipGetLastError());
+}
+
+void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp) {
+  hipLaunchKernelGGL(k_level_end, dim3(1), dim3(64), 0, ctx->stream, ctx->ctl.as<LevelCtl>(),
+                     consumed, produced, stamp ? 1 : 0);
+  BFSX_HIP(hipGetLastError());
+}
+
+void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out) {
+  const unsigned grid = cap_grid(((uint64_t)ctx->n_cols + 255) / 256,
+                                 (unsigned)ctx->cu_count * 8);
+  BFSX_HIP(hipMemsetAsync(d_out, 0, 3 * sizeof(unsigned long long), ctx->stream));
+  if (ctx->ptr_bytes == 8)
+    hipLaunchKernelGGL(k_summarize<uint64_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->col_ptr.as<uint64_t>(), ctx->levels.as<int32_t>(), ctx->n_rows,
+                       ctx->n_cols, d_out);
+  else
+    hipLaunchKernelGGL(k_summarize<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->col_ptr.as<uint32_t>(), ctx->levels.as<int32_t>(), ctx->n_rows,
+                       ctx->n_cols, d_out);
+  BFSX_HIP(hipGetLastError());
+}
+
+void count_traversed(bfsx_ctx *ctx, unsigned long long *d_out) { reached_summary(ctx, d_out); }
+
+}  // namespace bfsx

@@ -1,0 +1,602 @@
+// Engine entry points: context, resident graph, the level loop, C ABI.
+//
+// The level loop replaces GPURun's (src/run_bfs_gpu.hpp:679-728): per level
+// the host launches the chosen step, one small bookkeeping kernel, and reads
+// the level control block back (the reference reads the 4-byte frontier
+// count, :713-715).  Device memory for the graph and all search state is
+// allocated once per graph and reused across runs.
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "bfsx_internal.hpp"
+
+namespace bfsx {
+
+void fail(int status, const std::string &msg) { throw Error{status, msg}; }
+
+void check_hip(hipError_t e, const char *what) {
+  if (e == hipSuccess) return;
+  const int st = (e == hipErrorOutOfMemory) ? BFSX_ERR_NOMEM : BFSX_ERR_DEVICE;
+  fail(st, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void ensure(DevBuf &b, size_t bytes, hipStream_t s) {
+  (void)s;
+  if (bytes == 0) bytes = 1;
+  if (b.p && b.bytes >= bytes) return;
+  if (b.p) {
+    BFSX_HIP(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  BFSX_HIP(hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+}
+
+void release(DevBuf &b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+static void set_device(bfsx_ctx *ctx) { BFSX_HIP(hipSetDevice(ctx->device)); }
+
+static void free_search_state(bfsx_ctx *ctx) {
+  for (DevBuf *b : {&ctx->levels, &ctx->parents, &ctx->vis, &ctx->fbm[0], &ctx->fbm[1],
+                    &ctx->queue[0], &ctx->queue[1], &ctx->glb_scan, &ctx->glb_start,
+                    &ctx->glb_tile, &ctx->glb_chunk_lo})
+    release(*b);
+}
+
+static void alloc_search_state(bfsx_ctx *ctx) {
+  hipStream_t s = ctx->stream;
+  const uint64_t n = ctx->n_cols;
+  const uint64_t n_words = ((n + 63) / 64) * 2;
+  ensure(ctx->levels, sizeof(int32_t) * n, s);
+  ensure(ctx->vis, sizeof(uint32_t) * n_words, s);
+  ensure(ctx->queue[0], sizeof(uint32_t) * (n + 1), s);
+  ensure(ctx->queue[1], sizeof(uint32_t) * (n + 1), s);
+}
+
+static void ensure_pull_state(bfsx_ctx *ctx) {
+  const uint64_t n_words = (((uint64_t)ctx->n_cols + 63) / 64) * 2;
+  ensure(ctx->fbm[0], sizeof(uint32_t) * n_words, ctx->stream);
+  ensure(ctx->fbm[1], sizeof(uint32_t) * n_words, ctx->stream);
+}
+
+static void after_graph_change(bfsx_ctx *ctx) {
+  free_search_state(ctx);
+  graph_summary(ctx);
+  alloc_search_state(ctx);
+}
+
+static bfsx_params default_params() {
+  bfsx_params p{};
+  p.llb_wg = 256;
+  p.glb_wg = 256;
+  p.glb_blocks_per_cu = 8;
+  p.glb_chunk = 4096;
+  p.hub_degree = 0;
+  p.pull_wg = 256;
+  p.graph_levels = 8;
+  return p;
+}
+
+static void validate_params(const bfsx_params &p) {
+  auto pow2_in = [](uint32_t v, uint32_t lo, uint32_t hi) {
+    return v >= lo && v <= hi && (v & (v - 1)) == 0;
+  };
+  if (!pow2_in(p.llb_wg, 64, 1024)) fail(BFSX_ERR_ARG, "llb_wg must be a power of two in [64,1024]");
+  if (!pow2_in(p.glb_wg, 128, 1024)) fail(BFSX_ERR_ARG, "glb_wg must be a power of two in [128,1024]");
+  if (!pow2_in(p.pull_wg, 64, 512)) fail(BFSX_ERR_ARG, "pull_wg must be a power of two in [64,512]");
+  if (p.glb_blocks_per_cu < 1 || p.glb_blocks_per_cu > 64) fail(BFSX_ERR_ARG, "glb_blocks_per_cu must be in [1,64]");
+  if (p.glb_chunk < 256 || p.glb_chunk > (1u << 22)) fail(BFSX_ERR_ARG, "glb_chunk must be in [256, 2^22]");
+  if (p.graph_levels < 1 || p.graph_levels > 64) fail(BFSX_ERR_ARG, "graph_levels must be in [1,64]");
+}
+
+// ---------------------------------------------------------------------------
+// one search
+// ---------------------------------------------------------------------------
+struct RunOut {
+  double ms = 0;
+  uint32_t n_levels = 0;
+  uint32_t steps[BFSX_N_STEP_KINDS] = {0, 0, 0, 0};
+};
+
+static void read_ctl(bfsx_ctx *ctx) {
+  BFSX_HIP(hipMemcpyAsync(ctx->h_ctl, ctx->ctl.p, sizeof(LevelCtl), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  BFSX_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
+                     bool want_parents) {
+  if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
+  if (root >= ctx->n_cols) fail(BFSX_ERR_ARG, "root out of range");
+  if (strategy < BFSX_LLB || strategy > BFSX_PULL)
+    fail(BFSX_ERR_ARG, "unknown strategy");
+  const bool symmetric = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols;
+  if (strategy == BFSX_PULL && !symmetric)
+    fail(BFSX_ERR_UNSUPPORTED, "bottom-up steps need a symmetric graph with n_rows == n_cols");
+  const bool trace = flags & BFSX_RUN_TRACE;
+  hipStream_t s = ctx->stream;
+  if (want_parents) ensure(ctx->parents, sizeof(uint32_t) * ctx->n_cols, s);
+  if (strategy == BFSX_PULL) ensure_pull_state(ctx);
+  uint32_t *par = want_parents ? ctx->parents.as<uint32_t>() : nullptr;
+  // initial state outside the timer (host.cpp:113-121 / run_bfs_gpu.hpp:674-677)
+  init_search(ctx, root, symmetric, par);
+  BFSX_HIP(hipStreamSynchronize(s));
+  ctx->trace.clear();
+
+  RunOut out;
+  BFSX_HIP(hipEventRecord(ctx->ev0, s));
+  uint32_t F = 1, cur = 0, level = 0;
+  uint64_t f_edges = 0, visited_v = 1, visited_e = 0;
+  if (trace) {
+    read_ctl(ctx);
+    f_edges = ctx->h_ctl->q_edges[0];
+    visited_e = ctx->h_ctl->visited_edges;
+  }
+  int fbm_cur = 0;
+  bool fbm_valid = false;
+  while (F > 0) {
+    const int choice = strategy;
+    StepArgs a{};
+    a.col_ptr = ctx->col_ptr.p;
+    a.row_idx = ctx->row_idx.as<uint32_t>();
+    a.n_rows = ctx->n_rows;
+    a.n_cols = ctx->n_cols;
+    a.q_in = ctx->queue[cur].as<uint32_t>();
+    a.q_in_idx = cur;
+    a.F = F;
+    a.q_out = ctx->queue[cur ^ 1].as<uint32_t>();
+    a.q_out_idx = cur ^ 1;
+    a.vis = ctx->vis.as<uint32_t>();
+    a.levels = ctx->levels.as<int32_t>();
+    a.parents = par;
+    a.ctl = ctx->ctl.as<LevelCtl>();
+    a.next_level = (int32_t)level + 1;
+    a.track_degrees = trace;
+    if (trace) {
+      BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
+    }
+    switch (choice) {
+      case BFSX_LLB: step_llb(ctx, a, false); break;
+      case BFSX_SLB: step_llb(ctx, a, true); break;
+      case BFSX_GLB: step_glb(ctx, a); break;
+      case BFSX_PULL:
+        if (!fbm_valid) levels_to_fbm(ctx, a.levels, (int32_t)level, ctx->fbm[fbm_cur].as<uint32_t>());
+        a.fbm_cur = ctx->fbm[fbm_cur].as<uint32_t>();
+        a.fbm_next = ctx->fbm[fbm_cur ^ 1].as<uint32_t>();
+        step_pull(ctx, a);
+        break;
+    }
+    if (trace) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
+    level_end(ctx, cur, cur ^ 1, false);
+    read_ctl(ctx);
+    const uint32_t Fn = ctx->h_ctl->q_count[cur ^ 1];
+    if (choice == BFSX_PULL) {
+      fbm_cur ^= 1;
+      fbm_valid = true;
+    } else {
+      fbm_valid = false;
+    }
+    if (trace) {
+      bfsx_level_record r{};
+      r.level = level;
+      r.choice = (uint32_t)choice;
+      r.frontier_vertices = F;
+      r.frontier_edges = f_edges;
+      r.unvisited_vertices = ctx->n_cols - visited_v;
+      r.unvisited_edges = ctx->nnz - visited_e;
+      r.discovered = Fn;
+      float us = 0;
+      BFSX_HIP(hipEventElapsedTime(&us, ctx->ev_step0, ctx->ev_step1));
+      r.us = us * 1000.0;
+      ctx->trace.push_back(r);
+      f_edges = ctx->h_ctl->q_edges[cur ^ 1];
+      visited_v = ctx->h_ctl->visited_vertices;
+      visited_e = ctx->h_ctl->visited_edges;
+    }
+    out.steps[choice]++;
+    out.n_levels++;
+    F = Fn;
+    cur ^= 1;
+    level++;
+  }
+  BFSX_HIP(hipEventRecord(ctx->ev1, s));
+  BFSX_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  BFSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  out.ms = ms;
+  return out;
+}
+
+static void finish_run(bfsx_ctx *ctx, const RunOut &r, int32_t *levels_out,
+                       uint32_t *parents_out, bfsx_run_stats *stats) {
+  hipStream_t s = ctx->stream;
+  if (stats) {
+    ensure(ctx->scratch, 64, s);
+    reached_summary(ctx, ctx->scratch.as<unsigned long long>());
+    unsigned long long h[3] = {0, 0, 0};
+    BFSX_HIP(hipMemcpyAsync(h, ctx->scratch.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    BFSX_HIP(hipStreamSynchronize(s));
+    std::memset(stats, 0, sizeof(*stats));
+    stats->ms = r.ms;
+    stats->edges_count = h[0];
+    stats->reached = h[1];
+    stats->max_level = (uint32_t)h[2];
+    stats->n_levels = r.n_levels;
+    for (int i = 0; i < BFSX_N_STEP_KINDS; i++) stats->steps[i] = r.steps[i];
+  }
+  if (levels_out)
+    BFSX_HIP(hipMemcpyAsync(levels_out, ctx->levels.p, sizeof(int32_t) * ctx->n_cols,
+                            hipMemcpyDeviceToHost, s));
+  if (parents_out)
+    BFSX_HIP(hipMemcpyAsync(parents_out, ctx->parents.p, sizeof(uint32_t) * ctx->n_cols,
+                            hipMemcpyDeviceToHost, s));
+  BFSX_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace bfsx
+
+using namespace bfsx;
+
+// Runs `body`, converting every failure into a status code and message.
+template <class F>
+static int guarded(bfsx_ctx *ctx, F &&body) {
+  try {
+    body();
+    if (ctx) ctx->err.clear();
+    return BFSX_OK;
+  } catch (const Error &e) {
+    if (ctx) ctx->err = e.msg;
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    if (ctx) ctx->err = "host allocation failed";
+    return BFSX_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    if (ctx) ctx->err = e.what();
+    return BFSX_ERR_DEVICE;
+  } catch (...) {
+    if (ctx) ctx->err = "unknown failure";
+    return BFSX_ERR_DEVICE;
+  }
+}
+
+extern "C" {
+
+const char *bfsx_status_string(int st) {
+  switch (st) {
+    case BFSX_OK: return "ok";
+    case BFSX_ERR_ARG: return "invalid argument";
+    case BFSX_ERR_DEVICE: return "device error";
+    case BFSX_ERR_NOMEM: return "out of memory";
+    case BFSX_ERR_STATE: return "invalid state";
+    case BFSX_ERR_IO: return "i/o error";
+    case BFSX_ERR_OVERFLOW: return "overflow";
+    case BFSX_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+  }
+}
+
+const char *bfsx_last_error(const bfsx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int bfsx_create(int device, bfsx_ctx **out) {
+  if (!out) return BFSX_ERR_ARG;
+  *out = nullptr;
+  bfsx_ctx *ctx = nullptr;
+  try {
+    ctx = new bfsx_ctx;
+  } catch (...) {
+    return BFSX_ERR_NOMEM;
+  }
+  int rc = guarded(ctx, [&] {
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) fail(BFSX_ERR_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) fail(BFSX_ERR_ARG, "device index out of range");
+    ctx->device = device;
+    BFSX_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    BFSX_HIP(hipGetDeviceProperties(&prop, device));
+    ctx->cu_count = prop.multiProcessorCount;
+    ctx->name = prop.gcnArchName;
+    ctx->hbm_bytes = prop.totalGlobalMem;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+      ctx->wall_clock_khz = khz;
+    BFSX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    BFSX_HIP(hipEventCreate(&ctx->ev0));
+    BFSX_HIP(hipEventCreate(&ctx->ev1));
+    BFSX_HIP(hipEventCreate(&ctx->ev_step0));
+    BFSX_HIP(hipEventCreate(&ctx->ev_step1));
+    ensure(ctx->ctl, sizeof(LevelCtl), ctx->stream);
+    BFSX_HIP(hipHostMalloc((void **)&ctx->h_ctl, sizeof(LevelCtl), hipHostMallocDefault));
+    ctx->params = default_params();
+  });
+  if (rc != BFSX_OK) {
+    bfsx_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return BFSX_OK;
+}
+
+void bfsx_destroy(bfsx_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_search_state(ctx);
+  release(ctx->col_ptr);
+  release(ctx->row_idx);
+  release(ctx->iso_bm);
+  release(ctx->ctl);
+  release(ctx->scratch);
+  if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
+  for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_step0, ctx->ev_step1})
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int bfsx_device_info(bfsx_ctx *ctx, char *name, size_t name_len, int *cus, uint64_t *hbm) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (name && name_len) {
+      std::strncpy(name, ctx->name.c_str(), name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    if (cus) *cus = ctx->cu_count;
+    if (hbm) *hbm = ctx->hbm_bytes;
+  });
+}
+
+int bfsx_upload_csc(bfsx_ctx *ctx, uint32_t n_cols, uint32_t n_rows, const void *col_ptr,
+                    int col_ptr_bytes, const uint32_t *row_idx, uint64_t nnz, uint32_t flags) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (col_ptr_bytes != 4 && col_ptr_bytes != 8) fail(BFSX_ERR_ARG, "col_ptr_bytes must be 4 or 8");
+    if (n_rows > n_cols) fail(BFSX_ERR_ARG, "n_rows > n_cols");
+    if (!col_ptr || (nnz && !row_idx)) fail(BFSX_ERR_ARG, "null graph pointer");
+    // validate col_ptr on the host: monotone, starts at 0, ends at nnz
+    const uint64_t n1 = (uint64_t)n_cols + 1;
+    auto at = [&](uint64_t i) -> uint64_t {
+      return col_ptr_bytes == 8 ? static_cast<const uint64_t *>(col_ptr)[i]
+                                : static_cast<const uint32_t *>(col_ptr)[i];
+    };
+    if (at(0) != 0) fail(BFSX_ERR_ARG, "col_ptr[0] != 0");
+    for (uint64_t i = 1; i < n1; i++)
+      if (at(i) < at(i - 1)) fail(BFSX_ERR_ARG, "col_ptr is not monotone");
+    if (at(n_rows) > nnz || at(n_cols) != nnz)
+      fail(BFSX_ERR_ARG, "col_ptr end does not match nnz");
+    for (uint64_t i = 0; i < nnz; i++)
+      if (row_idx[i] >= n_cols) fail(BFSX_ERR_ARG, "row_idx entry >= n_cols");
+    ctx->has_graph = false;
+    release(ctx->col_ptr);
+    release(ctx->row_idx);
+    ensure(ctx->col_ptr, (size_t)col_ptr_bytes * n1, ctx->stream);
+    ensure(ctx->row_idx, sizeof(uint32_t) * (nnz ? nnz : 1), ctx->stream);
+    BFSX_HIP(hipMemcpyAsync(ctx->col_ptr.p, col_ptr, (size_t)col_ptr_bytes * n1,
+                            hipMemcpyHostToDevice, ctx->stream));
+    if (nnz)
+      BFSX_HIP(hipMemcpyAsync(ctx->row_idx.p, row_idx, sizeof(uint32_t) * nnz,
+                              hipMemcpyHostToDevice, ctx->stream));
+    BFSX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->n_cols = n_cols;
+    ctx->n_rows = n_rows;
+    ctx->nnz = nnz;
+    ctx->ptr_bytes = col_ptr_bytes;
+    ctx->gflags = flags & BFSX_GRAPH_SYMMETRIC;
+    ctx->has_graph = true;
+    after_graph_change(ctx);
+  });
+}
+
+static uint32_t thresh32(double p) {
+  const double v = p * 4294967296.0;
+  if (v <= 0) return 0;
+  if (v >= 4294967295.0) return 0xFFFFFFFFu;
+  return (uint32_t)v;
+}
+
+int bfsx_generate_rmat(bfsx_ctx *ctx, int scale, int edge_factor, double a, double b, double c,
+                       uint64_t seed_graph, uint64_t seed_perm, int col_ptr_bytes,
+                       uint64_t *nnz_out) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (scale < 1 || scale > 31) fail(BFSX_ERR_ARG, "scale must be in [1,31]");
+    if (edge_factor < 1 || edge_factor > 1024) fail(BFSX_ERR_ARG, "edge_factor must be in [1,1024]");
+    if (col_ptr_bytes != 4 && col_ptr_bytes != 8) fail(BFSX_ERR_ARG, "col_ptr_bytes must be 4 or 8");
+    if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0 + 1e-12) fail(BFSX_ERR_ARG, "bad quadrant probabilities");
+    const uint32_t n = 1u << scale;
+    const uint64_t m = (uint64_t)edge_factor << scale;
+    ctx->has_graph = false;
+    free_search_state(ctx);
+    release(ctx->col_ptr);
+    release(ctx->row_idx);
+    DevBuf du, dv;
+    ensure(du, sizeof(uint32_t) * m, ctx->stream);
+    ensure(dv, sizeof(uint32_t) * m, ctx->stream);
+    gen_rmat_pairs(ctx, scale, m, thresh32(a), thresh32(a + b), thresh32(a + b + c), seed_graph,
+                   seed_perm, du.as<uint32_t>(), dv.as<uint32_t>());
+    build_symmetric_csc(ctx, n, du.as<uint32_t>(), dv.as<uint32_t>(), m, col_ptr_bytes);
+    release(du);
+    release(dv);
+    after_graph_change(ctx);
+    if (nnz_out) *nnz_out = ctx->nnz;
+  });
+}
+
+int bfsx_generate_grid(bfsx_ctx *ctx, uint32_t side, double drop_prob, uint32_t n_long,
+                       uint64_t seed, int col_ptr_bytes, uint64_t *nnz_out) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (side < 2 || side > 65535) fail(BFSX_ERR_ARG, "side must be in [2,65535]");
+    if (drop_prob < 0 || drop_prob > 1) fail(BFSX_ERR_ARG, "drop_prob must be in [0,1]");
+    if (col_ptr_bytes != 4 && col_ptr_bytes != 8) fail(BFSX_ERR_ARG, "col_ptr_bytes must be 4 or 8");
+    const uint64_t V = (uint64_t)side * side;
+    const uint64_t m = 2ull * side * (side - 1) + n_long;
+    ctx->has_graph = false;
+    free_search_state(ctx);
+    release(ctx->col_ptr);
+    release(ctx->row_idx);
+    DevBuf du, dv;
+    ensure(du, sizeof(uint32_t) * m, ctx->stream);
+    ensure(dv, sizeof(uint32_t) * m, ctx->stream);
+    gen_grid_pairs(ctx, side, thresh32(drop_prob), n_long, seed, du.as<uint32_t>(), dv.as<uint32_t>());
+    build_symmetric_csc(ctx, (uint32_t)V, du.as<uint32_t>(), dv.as<uint32_t>(), m, col_ptr_bytes);
+    release(du);
+    release(dv);
+    after_graph_change(ctx);
+    if (nnz_out) *nnz_out = ctx->nnz;
+  });
+}
+
+int bfsx_graph_info_get(bfsx_ctx *ctx, bfsx_graph_info *out) {
+  if (!ctx || !out) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
+    out->n_rows = ctx->n_rows;
+    out->n_cols = ctx->n_cols;
+    out->nnz = ctx->nnz;
+    out->col_ptr_bytes = ctx->ptr_bytes;
+    out->flags = ctx->gflags;
+    out->max_degree = ctx->max_degree;
+    out->n_isolated = ctx->n_isolated;
+  });
+}
+
+int bfsx_download_csc(bfsx_ctx *ctx, void *col_ptr, uint32_t *row_idx) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
+    if (col_ptr)
+      BFSX_HIP(hipMemcpyAsync(col_ptr, ctx->col_ptr.p,
+                              (size_t)ctx->ptr_bytes * ((uint64_t)ctx->n_cols + 1),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    if (row_idx && ctx->nnz)
+      BFSX_HIP(hipMemcpyAsync(row_idx, ctx->row_idx.p, sizeof(uint32_t) * ctx->nnz,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    BFSX_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int bfsx_pick_roots(bfsx_ctx *ctx, uint32_t n_roots, uint64_t seed, int mode, uint32_t *roots_out,
+                    uint32_t *n_found) {
+  if (!ctx || (n_roots && !roots_out)) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
+    if (mode != 0 && mode != 1) fail(BFSX_ERR_ARG, "mode must be 0 (degree >= 1) or 1 (largest component)");
+    hipStream_t s = ctx->stream;
+    const int32_t *comp = nullptr;
+    std::vector<int32_t> keep_levels;
+    if (mode == 1) {
+      if (!(ctx->gflags & BFSX_GRAPH_SYMMETRIC)) fail(BFSX_ERR_UNSUPPORTED, "components need a symmetric graph");
+      // A component holding more than half the vertices is the largest one.
+      // Search from degree>=1 candidates until one reaches more than V/2.
+      const uint32_t probe_batch = 64;
+      std::vector<uint32_t> cand(probe_batch);
+      DevBuf dv, dok;
+      ensure(dv, sizeof(uint32_t) * probe_batch, s);
+      ensure(dok, probe_batch, s);
+      std::vector<uint8_t> ok(probe_batch);
+      bool found = false;
+      for (uint64_t first = 0; first < 64ull * probe_batch && !found; first += probe_batch) {
+        root_candidates(ctx, seed ^ 0x5bd1e995ull, first, probe_batch, nullptr, dv.as<uint32_t>(),
+                        dok.as<uint8_t>());
+        BFSX_HIP(hipMemcpyAsync(cand.data(), dv.p, sizeof(uint32_t) * probe_batch, hipMemcpyDeviceToHost, s));
+        BFSX_HIP(hipMemcpyAsync(ok.data(), dok.p, probe_batch, hipMemcpyDeviceToHost, s));
+        BFSX_HIP(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < probe_batch && !found; i++) {
+          if (!ok[i]) continue;
+          search(ctx, cand[i], BFSX_LLB, 0, false);
+          bfsx_run_stats st;
+          finish_run(ctx, RunOut{}, nullptr, nullptr, &st);
+          if (st.reached * 2 > ctx->n_cols) found = true;
+        }
+      }
+      release(dv);
+      release(dok);
+      if (!found) fail(BFSX_ERR_UNSUPPORTED, "no component holds more than half the vertices");
+      comp = ctx->levels.as<int32_t>();
+    }
+    const uint32_t batch = 1u << 16;
+    DevBuf dv, dok;
+    ensure(dv, sizeof(uint32_t) * batch, s);
+    ensure(dok, batch, s);
+    std::vector<uint32_t> cand(batch);
+    std::vector<uint8_t> ok(batch);
+    std::vector<uint32_t> got;
+    const uint64_t max_tries = 64ull * ctx->n_cols + 1024;
+    for (uint64_t first = 0; first < max_tries && got.size() < n_roots; first += batch) {
+      root_candidates(ctx, seed, first, batch, comp, dv.as<uint32_t>(), dok.as<uint8_t>());
+      BFSX_HIP(hipMemcpyAsync(cand.data(), dv.p, sizeof(uint32_t) * batch, hipMemcpyDeviceToHost, s));
+      BFSX_HIP(hipMemcpyAsync(ok.data(), dok.p, batch, hipMemcpyDeviceToHost, s));
+      BFSX_HIP(hipStreamSynchronize(s));
+      for (uint32_t i = 0; i < batch && got.size() < n_roots && first + i < max_tries; i++) {
+        if (!ok[i]) continue;
+        if (std::find(got.begin(), got.end(), cand[i]) == got.end()) got.push_back(cand[i]);
+      }
+    }
+    release(dv);
+    release(dok);
+    std::copy(got.begin(), got.end(), roots_out);
+    if (n_found) *n_found = (uint32_t)got.size();
+  });
+}
+
+int bfsx_run(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t run_flags, int32_t *levels_out,
+             uint32_t *parents_out, bfsx_run_stats *stats) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    const RunOut r = search(ctx, root, strategy, run_flags, parents_out != nullptr);
+    finish_run(ctx, r, levels_out, parents_out, stats);
+  });
+}
+
+int bfsx_run_batch(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, int strategy,
+                   uint32_t run_flags, int32_t *levels_out, bfsx_run_stats *stats) {
+  if (!ctx || (n_roots && !roots)) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    for (uint32_t i = 0; i < n_roots; i++) {
+      const RunOut r = search(ctx, roots[i], strategy, run_flags, false);
+      finish_run(ctx, r, levels_out ? levels_out + (size_t)i * ctx->n_cols : nullptr, nullptr,
+                 stats ? stats + i : nullptr);
+    }
+  });
+}
+
+int bfsx_last_trace(bfsx_ctx *ctx, bfsx_level_record *out, uint32_t max_records, uint32_t *n) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    const uint32_t k = (uint32_t)std::min<size_t>(ctx->trace.size(), max_records);
+    if (out && k) std::copy(ctx->trace.begin(), ctx->trace.begin() + k, out);
+    if (n) *n = (uint32_t)ctx->trace.size();
+  });
+}
+
+int bfsx_get_params(bfsx_ctx *ctx, bfsx_params *out) {
+  if (!ctx || !out) return BFSX_ERR_ARG;
+  *out = ctx->params;
+  return BFSX_OK;
+}
+
+int bfsx_set_params(bfsx_ctx *ctx, const bfsx_params *in) {
+  if (!ctx || !in) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    validate_params(*in);
+    ctx->params = *in;
+  });
+}
+
+}  // extern "C"

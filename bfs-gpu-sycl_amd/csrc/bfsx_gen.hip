@@ -1,0 +1,390 @@
+// Synthetic graphs built directly in HBM.
+//
+// The reference makes its R-MAT edge lists with networkit on the host
+// (scripts/genGraph.py:41-70) and converts them to CSC with scipy
+// (scripts/generator.py:151-195).  Here the same kind of graph is generated
+// on the GPU so that a scale-26 graph (about 2^31 stored edges) never crosses
+// PCIe: a counter-based RNG gives every edge its own random stream, the
+// permutation is a sort of random keys, and deduplication/symmetrization is a
+// radix sort of packed (source, destination) keys followed by unique.  The
+// CSC layout is the reference's: col_ptr over source vertices, row_idx holds
+// each source's destinations in ascending order (scripts/generator.py:115-147
+// writes scipy's CSC arrays, which are sorted).
+//
+// Random stream layout (ctr[3] is a stream tag; key = (lo(seed), hi(seed))):
+//   tag 0  R-MAT edge k, quadrant choice j: ctr = (lo k, hi k, j/4, 0), word j%4
+//   tag 1  permutation key of vertex x:    ctr = (x, 0, 0, 1)
+//   tag 2  root candidate k:               ctr = (lo k, hi k, 0, 2)
+//   tag 3  grid edge g keep/drop:          ctr = (lo g, hi g, 0, 3)
+//   tag 4  grid long-range edge k:         ctr = (k, 0, 0, 4)
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+
+#include "bfsx_internal.hpp"
+
+namespace bfsx {
+
+// Philox-4x32-10 (Salmon et al., SC'11): 10 rounds of two 32x32->64
+// multiplies with the Weyl key schedule.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    if (r) {
+      k.x += 0x9E3779B9u;
+      k.y += 0xBB67AE85u;
+    }
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+  }
+  return c;
+}
+
+__device__ __forceinline__ uint2 seed_key(uint64_t seed) {
+  return make_uint2((uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+__global__ void k_rmat_pairs(int scale, uint64_t m, uint32_t tA, uint32_t tAB,
+                             uint32_t tABC, uint64_t seed, uint32_t *u,
+                             uint32_t *v) {
+  const uint2 key = seed_key(seed);
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t a = 0, b = 0;
+    uint4 r = make_uint4(0, 0, 0, 0);
+    for (int j = 0; j < scale; j++) {
+      if ((j & 3) == 0)
+        r = philox4x32_10(make_uint4((uint32_t)k, (uint32_t)(k >> 32), j >> 2, 0), key);
+      const uint32_t x = (j & 3) == 0 ? r.x : (j & 3) == 1 ? r.y : (j & 3) == 2 ? r.z : r.w;
+      const uint32_t bit = 1u << (scale - 1 - j);
+      if (x >= tA) {
+        if (x < tAB) {
+          b |= bit;
+        } else if (x < tABC) {
+          a |= bit;
+        } else {
+          a |= bit;
+          b |= bit;
+        }
+      }
+    }
+    u[k] = a;
+    v[k] = b;
+  }
+}
+
+__global__ void k_perm_keys(uint32_t n, int bits, uint64_t seed, uint64_t *keys) {
+  const uint2 key = seed_key(seed);
+  for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n;
+       x += gridDim.x * blockDim.x) {
+    const uint4 r = philox4x32_10(make_uint4(x, 0, 0, 1), key);
+    const uint64_t k64 = ((uint64_t)r.x << 32) | r.y;
+    keys[x] = ((k64 >> bits) << bits) | x;
+  }
+}
+
+__global__ void k_perm_scatter(uint32_t n, uint64_t mask, const uint64_t *sorted,
+                               uint32_t *perm) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x)
+    perm[sorted[i] & mask] = i;
+}
+
+__global__ void k_apply_perm(uint64_t m, const uint32_t *perm, uint32_t *u,
+                             uint32_t *v) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    u[k] = perm[u[k]];
+    v[k] = perm[v[k]];
+  }
+}
+
+__global__ void k_grid_pairs(uint32_t side, uint32_t t_drop, uint64_t seed,
+                             uint32_t *u, uint32_t *v, uint32_t *count) {
+  const uint2 key = seed_key(seed);
+  const uint64_t H = (uint64_t)side * (side - 1);
+  const uint64_t G = 2 * H;
+  for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < G;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint4 r = philox4x32_10(make_uint4((uint32_t)g, (uint32_t)(g >> 32), 0, 3), key);
+    uint64_t a, b;
+    if (g < H) {
+      a = (g / (side - 1)) * side + g % (side - 1);
+      b = a + 1;
+    } else {
+      const uint64_t h = g - H;
+      a = (h / side) * side + h % side;
+      b = a + side;
+    }
+    const bool keep = r.x >= t_drop;
+    // Dropped edges become self-loops, which the CSC builder removes; this
+    // keeps the pair array dense without a compaction pass.
+    u[g] = (uint32_t)a;
+    v[g] = keep ? (uint32_t)b : (uint32_t)a;
+    (void)count;
+  }
+}
+
+__global__ void k_long_pairs(uint32_t n_long, uint64_t V, uint64_t seed,
+                             uint32_t *u, uint32_t *v) {
+  const uint2 key = seed_key(seed);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n_long;
+       k += gridDim.x * blockDim.x) {
+    const uint4 r = philox4x32_10(make_uint4(k, 0, 0, 4), key);
+    u[k] = (uint32_t)(((uint64_t)r.x * V) >> 32);
+    v[k] = (uint32_t)(((uint64_t)r.y * V) >> 32);
+  }
+}
+
+// keys[2i], keys[2i+1] = both directions of pair i packed as src<<B | dst;
+// self loops become the sentinel n<<B, which sorts after every real key.
+__global__ void k_sym_keys(uint64_t m, int B, uint64_t sentinel, const uint32_t *u,
+                           const uint32_t *v, uint64_t *keys) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t a = u[i], b = v[i];
+    const bool loop = a == b;
+    keys[2 * i] = loop ? sentinel : ((a << B) | b);
+    keys[2 * i + 1] = loop ? sentinel : ((b << B) | a);
+  }
+}
+
+// Splits unique sorted keys into row_idx and col_ptr.  col_ptr[x] = first key
+// index with source >= x: entry i writes col_ptr for the sources in
+// (src(i-1), src(i)]; the tail (last source, n] gets nnz.
+template <typename PtrT>
+__global__ void k_csc_from_keys(const uint64_t *uk, uint64_t nnz, int B, uint32_t n,
+                                PtrT *col_ptr, uint32_t *row_idx) {
+  const uint64_t mask = (B >= 64) ? ~0ull : ((1ull << B) - 1);
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= nnz;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t s = (i < nnz) ? (uk[i] >> B) : (uint64_t)n + 1;
+    const int64_t sp = (i == 0) ? -1 : (int64_t)(uk[i - 1] >> B);
+    const uint64_t hi = (i < nnz) ? s : (uint64_t)n;
+    for (uint64_t x = (uint64_t)(sp + 1); x <= hi; x++) col_ptr[x] = (PtrT)i;
+    if (i < nnz) row_idx[i] = (uint32_t)(uk[i] & mask);
+  }
+}
+
+static int bits_for(uint64_t n) {
+  int b = 0;
+  while ((1ull << b) < n) b++;
+  return b;
+}
+
+static unsigned grid_for(uint64_t n, unsigned block = 256) {
+  uint64_t g = (n + block - 1) / block;
+  if (g > 65536) g = 65536;
+  if (g == 0) g = 1;
+  return (unsigned)g;
+}
+
+void gen_rmat_pairs(bfsx_ctx *ctx, int scale, uint64_t m, uint32_t tA, uint32_t tAB,
+                    uint32_t tABC, uint64_t seed_graph, uint64_t seed_perm,
+                    uint32_t *d_u, uint32_t *d_v) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = 1u << scale;
+  hipLaunchKernelGGL(k_rmat_pairs, dim3(grid_for(m)), dim3(256), 0, s, scale, m, tA,
+                     tAB, tABC, seed_graph, d_u, d_v);
+  BFSX_HIP(hipGetLastError());
+  // permutation: perm[x] = rank of (random high bits | x)
+  DevBuf keys, sorted, perm, tmp;
+  ensure(keys, sizeof(uint64_t) * n, s);
+  ensure(sorted, sizeof(uint64_t) * n, s);
+  ensure(perm, sizeof(uint32_t) * n, s);
+  const int bits = bits_for(n);
+  hipLaunchKernelGGL(k_perm_keys, dim3(grid_for(n)), dim3(256), 0, s, n, bits, seed_perm,
+                     keys.as<uint64_t>());
+  size_t tbytes = 0;
+  BFSX_HIP(rocprim::radix_sort_keys(nullptr, tbytes, keys.as<uint64_t>(),
+                                    sorted.as<uint64_t>(), (size_t)n, 0, 64, s));
+  ensure(tmp, tbytes, s);
+  BFSX_HIP(rocprim::radix_sort_keys(tmp.p, tbytes, keys.as<uint64_t>(),
+                                    sorted.as<uint64_t>(), (size_t)n, 0, 64, s));
+  const uint64_t mask = (1ull << bits) - 1;
+  hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for(n)), dim3(256), 0, s, n, mask,
+                     sorted.as<uint64_t>(), perm.as<uint32_t>());
+  hipLaunchKernelGGL(k_apply_perm, dim3(grid_for(m)), dim3(256), 0, s, m,
+                     perm.as<uint32_t>(), d_u, d_v);
+  BFSX_HIP(hipGetLastError());
+  BFSX_HIP(hipStreamSynchronize(s));
+  release(keys);
+  release(sorted);
+  release(perm);
+  release(tmp);
+}
+
+uint64_t gen_grid_pairs(bfsx_ctx *ctx, uint32_t side, uint32_t t_drop, uint32_t n_long,
+                        uint64_t seed, uint32_t *d_u, uint32_t *d_v) {
+  hipStream_t s = ctx->stream;
+  const uint64_t G = 2ull * side * (side - 1);
+  const uint64_t V = (uint64_t)side * side;
+  if (G)
+    hipLaunchKernelGGL(k_grid_pairs, dim3(grid_for(G)), dim3(256), 0, s, side, t_drop,
+                       seed, d_u, d_v, (uint32_t *)nullptr);
+  if (n_long)
+    hipLaunchKernelGGL(k_long_pairs, dim3(grid_for(n_long)), dim3(256), 0, s, n_long, V,
+                       seed, d_u + G, d_v + G);
+  BFSX_HIP(hipGetLastError());
+  return G + n_long;
+}
+
+void build_symmetric_csc(bfsx_ctx *ctx, uint32_t n, uint32_t *d_u, uint32_t *d_v,
+                         uint64_t m, int ptr_bytes) {
+  hipStream_t s = ctx->stream;
+  const int B = bits_for(n) > 0 ? bits_for(n) : 1;
+  const uint64_t sentinel = (uint64_t)n << B;
+  const int end_bit = B + bits_for((uint64_t)n + 1);
+  const uint64_t K = 2 * m;
+  DevBuf keys, sorted, tmp, cnt;
+  ensure(keys, sizeof(uint64_t) * (K ? K : 1), s);
+  hipLaunchKernelGGL(k_sym_keys, dim3(grid_for(m)), dim3(256), 0, s, m, B, sentinel, d_u,
+                     d_v, keys.as<uint64_t>());
+  BFSX_HIP(hipGetLastError());
+  BFSX_HIP(hipStreamSynchronize(s));
+  // the pair arrays are no longer needed; the caller frees them after return
+  ensure(sorted, sizeof(uint64_t) * (K ? K : 1), s);
+  size_t tbytes = 0;
+  BFSX_HIP(rocprim::radix_sort_keys(nullptr, tbytes, keys.as<uint64_t>(),
+                                    sorted.as<uint64_t>(), (size_t)K, 0, end_bit, s));
+  ensure(tmp, tbytes, s);
+  BFSX_HIP(rocprim::radix_sort_keys(tmp.p, tbytes, keys.as<uint64_t>(),
+                                    sorted.as<uint64_t>(), (size_t)K, 0, end_bit, s));
+  ensure(cnt, sizeof(unsigned long long), s);
+  size_t ubytes = 0;
+  BFSX_HIP(rocprim::unique(nullptr, ubytes, sorted.as<uint64_t>(), keys.as<uint64_t>(),
+                           cnt.as<unsigned long long>(), (size_t)K,
+                           rocprim::equal_to<uint64_t>(), s));
+  if (ubytes > tmp.bytes) {
+    release(tmp);
+    ensure(tmp, ubytes, s);
+  }
+  BFSX_HIP(rocprim::unique(tmp.p, ubytes, sorted.as<uint64_t>(), keys.as<uint64_t>(),
+                           cnt.as<unsigned long long>(), (size_t)K,
+                           rocprim::equal_to<uint64_t>(), s));
+  unsigned long long n_unique = 0;
+  BFSX_HIP(hipMemcpyAsync(&n_unique, cnt.p, sizeof(n_unique), hipMemcpyDeviceToHost, s));
+  BFSX_HIP(hipStreamSynchronize(s));
+  uint64_t nnz = n_unique;
+  if (nnz) {
+    uint64_t last = 0;
+    BFSX_HIP(hipMemcpyAsync(&last, keys.as<uint64_t>() + nnz - 1, sizeof(last),
+                            hipMemcpyDeviceToHost, s));
+    BFSX_HIP(hipStreamSynchronize(s));
+    if (last == sentinel) nnz--;
+  }
+  release(sorted);
+  release(tmp);
+  release(cnt);
+  if (ptr_bytes == 4 && nnz > 0xFFFFFFFFull)
+    fail(BFSX_ERR_OVERFLOW, "graph has more than 2^32-1 edges; use 8-byte col_ptr");
+  release(ctx->col_ptr);
+  release(ctx->row_idx);
+  ensure(ctx->col_ptr, (size_t)ptr_bytes * ((uint64_t)n + 1), s);
+  ensure(ctx->row_idx, sizeof(uint32_t) * (nnz ? nnz : 1), s);
+  if (ptr_bytes == 8)
+    hipLaunchKernelGGL(k_csc_from_keys<uint64_t>, dim3(grid_for(nnz + 1)), dim3(256), 0, s,
+                       keys.as<uint64_t>(), nnz, B, n, ctx->col_ptr.as<uint64_t>(),
+                       ctx->row_idx.as<uint32_t>());
+  else
+    hipLaunchKernelGGL(k_csc_from_keys<uint32_t>, dim3(grid_for(nnz + 1)), dim3(256), 0, s,
+                       keys.as<uint64_t>(), nnz, B, n, ctx->col_ptr.as<uint32_t>(),
+                       ctx->row_idx.as<uint32_t>());
+  BFSX_HIP(hipGetLastError());
+  BFSX_HIP(hipStreamSynchronize(s));
+  release(keys);
+  ctx->n_rows = ctx->n_cols = n;
+  ctx->nnz = nnz;
+  ctx->ptr_bytes = ptr_bytes;
+  ctx->gflags = BFSX_GRAPH_SYMMETRIC;
+  ctx->has_graph = true;
+}
+
+// ---- graph summary: max degree, isolated-vertex bitmap and count ---------
+template <typename PtrT>
+__global__ void k_summary(const PtrT *col_ptr, uint32_t n_rows, uint32_t n_cols,
+                          uint32_t *iso_bm, unsigned int *max_deg,
+                          unsigned int *n_iso) {
+  const uint32_t nwords = ((n_cols + 63) / 64) * 2;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+       w += gridDim.x * blockDim.x) {
+    uint32_t bits = 0, mx = 0;
+    for (int b = 0; b < 32; b++) {
+      const uint32_t x = w * 32 + b;
+      if (x < n_rows) {
+        const uint64_t d = (uint64_t)(col_ptr[x + 1] - col_ptr[x]);
+        const uint32_t dc = d > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)d;
+        if (dc == 0) bits |= 1u << b;
+        mx = dc > mx ? dc : mx;
+      }
+    }
+    iso_bm[w] = bits;
+    if (bits) atomicAdd(n_iso, (unsigned)__popc(bits));
+    if (mx) atomicMax(max_deg, mx);
+  }
+}
+
+void graph_summary(bfsx_ctx *ctx) {
+  hipStream_t s = ctx->stream;
+  const uint32_t nwords = ((ctx->n_cols + 63) / 64) * 2;
+  ensure(ctx->iso_bm, sizeof(uint32_t) * (nwords ? nwords : 2), s);
+  DevBuf out;
+  ensure(out, 2 * sizeof(unsigned int), s);
+  BFSX_HIP(hipMemsetAsync(out.p, 0, 2 * sizeof(unsigned int), s));
+  if (ctx->ptr_bytes == 8)
+    hipLaunchKernelGGL(k_summary<uint64_t>, dim3(grid_for(nwords)), dim3(256), 0, s,
+                       ctx->col_ptr.as<uint64_t>(), ctx->n_rows, ctx->n_cols,
+                       ctx->iso_bm.as<uint32_t>(), out.as<unsigned int>(),
+                       out.as<unsigned int>() + 1);
+  else
+    hipLaunchKernelGGL(k_summary<uint32_t>, dim3(grid_for(nwords)), dim3(256), 0, s,
+                       ctx->col_ptr.as<uint32_t>(), ctx->n_rows, ctx->n_cols,
+                       ctx->iso_bm.as<uint32_t>(), out.as<unsigned int>(),
+                       out.as<unsigned int>() + 1);
+  BFSX_HIP(hipGetLastError());
+  unsigned int h[2] = {0, 0};
+  BFSX_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  BFSX_HIP(hipStreamSynchronize(s));
+  release(out);
+  ctx->max_degree = h[0];
+  ctx->n_isolated = h[1];
+}
+
+// ---- seeded root candidates ------------------------------------------------
+// Candidate k is x = (w0 * V) >> 32 of its tag-2 draw; ok[k] says whether it
+// qualifies (degree >= 1, or reached in component_levels when given).
+template <typename PtrT>
+__global__ void k_root_candidates(const PtrT *col_ptr, uint32_t n_rows, uint32_t n_cols,
+                                  uint64_t seed, uint64_t first, uint32_t count,
+                                  const int32_t *comp, uint32_t *out_v, uint8_t *ok) {
+  const uint2 key = seed_key(seed);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += gridDim.x * blockDim.x) {
+    const uint64_t k = first + i;
+    const uint4 r = philox4x32_10(make_uint4((uint32_t)k, (uint32_t)(k >> 32), 0, 2), key);
+    const uint32_t x = (uint32_t)(((uint64_t)r.x * n_cols) >> 32);
+    bool good;
+    if (comp) good = comp[x] >= 0;
+    else good = x < n_rows && col_ptr[x + 1] > col_ptr[x];
+    out_v[i] = x;
+    ok[i] = good ? 1 : 0;
+  }
+}
+
+void root_candidates(bfsx_ctx *ctx, uint64_t seed, uint64_t first, uint32_t count,
+                     const int32_t *d_comp, uint32_t *d_out_v, uint8_t *d_ok) {
+  hipStream_t s = ctx->stream;
+  if (ctx->ptr_bytes == 8)
+    hipLaunchKernelGGL(k_root_candidates<uint64_t>, dim3(grid_for(count)), dim3(256), 0, s,
+                       ctx->col_ptr.as<uint64_t>(), ctx->n_rows, ctx->n_cols, seed, first,
+                       count, d_comp, d_out_v, d_ok);
+  else
+    hipLaunchKernelGGL(k_root_candidates<uint32_t>, dim3(grid_for(count)), dim3(256), 0, s,
+                       ctx->col_ptr.as<uint32_t>(), ctx->n_rows, ctx->n_cols, seed, first,
+                       count, d_comp, d_out_v, d_ok);
+  BFSX_HIP(hipGetLastError());
+}
+
+}  // namespace bfsx

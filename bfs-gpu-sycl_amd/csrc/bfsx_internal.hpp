@@ -1,0 +1,136 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "bfsx.h"
+
+namespace bfsx {
+
+// Device-side control block of a search: frontier sizes of the two queue
+// buffers, their degree sums, the GLB work-queue cursor and running totals.
+// Lives in device memory; a pinned host mirror receives it once per level.
+struct LevelCtl {
+  uint32_t q_count[2];             // entries in queue buffer 0 / 1
+  uint32_t level;                  // level of the frontier being expanded
+  uint32_t overflow;               // set when a result overflows its type
+  unsigned long long q_edges[2];   // sum of out-degree of queue 0 / 1
+  unsigned long long glb_total;    // edges of the current GLB frontier
+  uint32_t glb_cursor;             // GLB atomic work-queue head
+  uint32_t glb_chunks;             // GLB chunk count
+  unsigned long long visited_vertices;
+  unsigned long long visited_edges;
+  unsigned long long discovered;   // vertices found by the last step
+  unsigned long long t_begin;      // device wall clock at step start
+  unsigned long long t_end;        // device wall clock at step end
+  uint32_t choice;                 // step kind picked for the next level
+  uint32_t done;                   // frontier became empty
+  float sched_out[4];              // scheduler outputs of the last decision
+  float sched_in[8];               // scheduler inputs of the last decision
+};
+
+// Growable device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace bfsx
+
+struct bfsx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+  int wall_clock_khz = 100000;
+  std::string name;
+  uint64_t hbm_bytes = 0;
+  std::string err;
+
+  // resident graph
+  bool has_graph = false;
+  uint32_t n_rows = 0, n_cols = 0;
+  uint64_t nnz = 0;
+  int ptr_bytes = 4;
+  uint32_t gflags = 0;
+  uint32_t max_degree = 0, n_isolated = 0;
+  bfsx::DevBuf col_ptr, row_idx, iso_bm;
+
+  // per-search state, sized for the resident graph
+  bfsx::DevBuf levels, parents, vis, fbm[2], queue[2], ctl;
+  bfsx::DevBuf glb_scan, glb_start, glb_tile, glb_chunk_lo;
+  bfsx::DevBuf scratch;  // generic scratch (reductions, root picking)
+  bfsx::LevelCtl *h_ctl = nullptr;  // pinned mirror
+
+  bfsx_params params{};
+  std::vector<bfsx_level_record> trace;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_step0 = nullptr, ev_step1 = nullptr;
+};
+
+namespace bfsx {
+
+// Error plumbing: every entry point catches and converts to a status code.
+struct Error {
+  int status;
+  std::string msg;
+};
+[[noreturn]] void fail(int status, const std::string &msg);
+void check_hip(hipError_t e, const char *what);
+#define BFSX_HIP(call) ::bfsx::check_hip((call), #call)
+
+void ensure(DevBuf &b, size_t bytes, hipStream_t s);
+void release(DevBuf &b);
+
+// ---- graph construction (bfsx_gen.hip) -----------------------------------
+// Builds a symmetric CSC in ctx from n undirected pairs (u[i], v[i]) resident
+// on the device; u/v buffers are consumed.
+void build_symmetric_csc(bfsx_ctx *ctx, uint32_t n_vertices, uint32_t *d_u,
+                         uint32_t *d_v, uint64_t m, int ptr_bytes);
+void gen_rmat_pairs(bfsx_ctx *ctx, int scale, uint64_t m, uint32_t tA,
+                    uint32_t tAB, uint32_t tABC, uint64_t seed_graph,
+                    uint64_t seed_perm, uint32_t *d_u, uint32_t *d_v);
+uint64_t gen_grid_pairs(bfsx_ctx *ctx, uint32_t side, uint32_t t_drop,
+                        uint32_t n_long, uint64_t seed, uint32_t *d_u,
+                        uint32_t *d_v);
+void graph_summary(bfsx_ctx *ctx);  // max degree, isolated bitmap/count
+void root_candidates(bfsx_ctx *ctx, uint64_t seed, uint64_t first,
+                     uint32_t count, const int32_t *d_component_levels,
+                     uint32_t *d_out_vertex, uint8_t *d_out_ok);
+
+// ---- search kernels (bfsx_bfs.hip) ---------------------------------------
+struct StepArgs {
+  const void *col_ptr;
+  const uint32_t *row_idx;
+  uint32_t n_rows, n_cols;
+  const uint32_t *q_in;
+  uint32_t q_in_idx;       // which LevelCtl counter describes q_in
+  uint32_t F;              // frontier size (host-known)
+  uint32_t *q_out;
+  uint32_t q_out_idx;
+  uint32_t *vis;
+  int32_t *levels;
+  uint32_t *parents;       // nullable
+  uint32_t *fbm_cur;       // frontier bitmap of this level (pull)
+  uint32_t *fbm_next;      // next frontier bitmap (pull writes it)
+  LevelCtl *ctl;
+  int32_t next_level;
+  bool track_degrees;      // accumulate degree sum of discovered vertices
+};
+
+void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *parents);
+void step_llb(bfsx_ctx *ctx, const StepArgs &a, bool strided);
+void step_glb(bfsx_ctx *ctx, const StepArgs &a);
+void step_pull(bfsx_ctx *ctx, const StepArgs &a);
+void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level,
+                   uint32_t *fbm);
+void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
+               bool stamp);
+void level_begin(bfsx_ctx *ctx);
+void count_traversed(bfsx_ctx *ctx, unsigned long long *d_out);
+void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out);
+
+}  // namespace bfsx

@@ -1,0 +1,171 @@
+/*
+ * bfsx — MI355X (gfx950) breadth-first-search engine, C ABI.
+ *
+ * This is the drop-in boundary for the reference's GPU path,
+ * src/run_bfs_gpu.hpp (GPURun, :534-542, and the kernels it launches).  All
+ * entry points take plain pointers and sizes, return a status code
+ * (BFSX_OK = 0) and never let a C++ exception cross the ABI; the message of
+ * the last failure is available from bfsx_last_error().
+ *
+ * A context owns one HIP device, one stream (default priority) and all device
+ * memory.  The graph stays resident in HBM across runs (the reference
+ * re-uploads it every run, run_bfs_gpu.hpp:620-668, outside its timer).  A
+ * context is single-threaded: calls on one context must not overlap.
+ *
+ * Reference interface each entry point replaces (paths relative to the
+ * reference repository):
+ *   bfsx_create / bfsx_destroy    queue setup and teardown,
+ *                                 src/run_bfs_gpu.hpp:546-557, :732-746
+ *   bfsx_upload_csc               malloc_device + copyToDevice of col_ptr and
+ *                                 row_idx, src/run_bfs_gpu.hpp:620-665
+ *   bfsx_run                      one run of the timed level loop
+ *                                 (explore + levelgen per level),
+ *                                 src/run_bfs_gpu.hpp:671-730
+ *   bfsx_run_batch                the num_runs loop of GPURun over roots,
+ *                                 src/run_bfs_gpu.hpp:614-747
+ *   bfsx_set_params               SM_FACTOR and work-group size,
+ *                                 src/CMakeLists.txt:12, run_bfs_gpu.hpp:194-197
+ */
+#ifndef BFSX_H
+#define BFSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define BFSX_OK 0
+#define BFSX_ERR_ARG 1          /* invalid argument (bad root, size, width) */
+#define BFSX_ERR_DEVICE 2       /* HIP runtime error or no usable device */
+#define BFSX_ERR_NOMEM 3        /* device or host allocation failed */
+#define BFSX_ERR_STATE 4        /* call out of order (e.g. run before upload) */
+#define BFSX_ERR_IO 5           /* file missing, empty, short or inconsistent */
+#define BFSX_ERR_OVERFLOW 6     /* a result does not fit its output type */
+#define BFSX_ERR_UNSUPPORTED 7  /* strategy not valid for this graph */
+
+/* ---- strategies (the reference's compile-time switch,
+ *      src/CMakeLists.txt:10-11,63-64, made a run-time enum) -------------- */
+#define BFSX_LLB 0      /* local load balance: ExploreNeighbours, :388-477 */
+#define BFSX_GLB 1      /* global load balance: ExploreNeighbours1..4, :175-385 */
+#define BFSX_SLB 2      /* strided local load balance: :416-417 */
+#define BFSX_PULL 3     /* bottom-up step (symmetric graphs only) */
+#define BFSX_LEARNED 4  /* per-level choice of the four by the MLP scheduler */
+#define BFSX_N_STEP_KINDS 4
+
+/* graph flags */
+#define BFSX_GRAPH_SYMMETRIC 1u /* every stored edge u->v has v->u */
+
+/* run flags */
+#define BFSX_RUN_GRAPH_LOOP 1u  /* capture the level loop in a hipGraph with a
+                                   device-side termination flag */
+#define BFSX_RUN_TRACE 2u       /* record per-level statistics and timings */
+
+typedef struct bfsx_ctx bfsx_ctx;
+
+typedef struct bfsx_graph_info {
+  uint32_t n_rows;       /* vertices expanded (the reference's numRows) */
+  uint32_t n_cols;       /* vertices / level-array length (numCols) */
+  uint64_t nnz;          /* stored edges */
+  int32_t col_ptr_bytes; /* 4 or 8 */
+  uint32_t flags;        /* BFSX_GRAPH_* */
+  uint32_t max_degree;
+  uint32_t n_isolated;   /* vertices < n_rows with no stored edge */
+} bfsx_graph_info;
+
+typedef struct bfsx_run_stats {
+  double ms;             /* level loop only: after upload/initialisation,
+                            before the copy of levels to the host
+                            (src/run_bfs_gpu.hpp:679,728) */
+  uint64_t edges_count;  /* the reference's edgesCount: sum of out-degree
+                            over reached vertices < n_rows
+                            (src/run_bfs_cpu.hpp:36,84) */
+  uint32_t n_levels;     /* level iterations that expanded a frontier */
+  uint32_t max_level;    /* deepest level reached */
+  uint64_t reached;      /* vertices with a level >= 0 */
+  uint32_t steps[BFSX_N_STEP_KINDS]; /* how many levels used each step kind */
+} bfsx_run_stats;
+
+/* one level of the last run (BFSX_RUN_TRACE) */
+typedef struct bfsx_level_record {
+  uint32_t level;           /* frontier level being expanded */
+  uint32_t choice;          /* BFSX_LLB..BFSX_PULL */
+  uint64_t frontier_vertices;
+  uint64_t frontier_edges;  /* sum of out-degree of the frontier */
+  uint64_t unvisited_vertices;
+  uint64_t unvisited_edges;
+  uint64_t discovered;      /* vertices found by this step */
+  double us;                /* device time of this step */
+} bfsx_level_record;
+
+/* launch-shape parameters (the SM_FACTOR / work-group knobs) */
+typedef struct bfsx_params {
+  uint32_t llb_wg;          /* work-group size for LLB/SLB: 64..1024, pow2 */
+  uint32_t glb_wg;          /* work-group size of the GLB expand kernel */
+  uint32_t glb_blocks_per_cu; /* persistent GLB grid = CUs * this (SM_FACTOR) */
+  uint32_t glb_chunk;       /* edges per GLB work-queue item */
+  uint32_t hub_degree;      /* LLB/SLB: frontier vertices with at least this
+                               degree are expanded by a whole wave (0 = off) */
+  uint32_t pull_wg;         /* work-group size of the bottom-up step */
+  uint32_t graph_levels;    /* levels per captured hipGraph launch */
+} bfsx_params;
+
+/* ---- context ------------------------------------------------------------ */
+int bfsx_create(int device, bfsx_ctx **out);
+void bfsx_destroy(bfsx_ctx *ctx);
+const char *bfsx_last_error(const bfsx_ctx *ctx);
+const char *bfsx_status_string(int status);
+int bfsx_device_info(bfsx_ctx *ctx, char *name, size_t name_len,
+                     int *compute_units, uint64_t *hbm_bytes);
+
+/* ---- graph -------------------------------------------------------------- */
+/* Uploads a CSC graph in the layout of src/graph_load.hpp: col_ptr has
+ * n_cols+1 entries of col_ptr_bytes (4 or 8) each, row_idx has nnz entries.
+ * Vertices >= n_rows are never expanded (src/run_bfs_cpu.hpp:31). */
+int bfsx_upload_csc(bfsx_ctx *ctx, uint32_t n_cols, uint32_t n_rows,
+                    const void *col_ptr, int col_ptr_bytes,
+                    const uint32_t *row_idx, uint64_t nnz, uint32_t flags);
+/* Builds an R-MAT graph directly in HBM: edge_factor * 2^scale generated
+ * edges with quadrant probabilities a, b, c (d = 1-a-b-c), a seeded vertex
+ * permutation, self-loops and duplicates dropped, symmetrized. */
+int bfsx_generate_rmat(bfsx_ctx *ctx, int scale, int edge_factor, double a,
+                       double b, double c, uint64_t seed_graph,
+                       uint64_t seed_perm, int col_ptr_bytes, uint64_t *nnz);
+/* Builds a side x side 4-neighbour grid in HBM with each undirected edge
+ * dropped with probability drop_prob and n_long random long-range edges. */
+int bfsx_generate_grid(bfsx_ctx *ctx, uint32_t side, double drop_prob,
+                       uint32_t n_long, uint64_t seed, int col_ptr_bytes,
+                       uint64_t *nnz);
+int bfsx_graph_info_get(bfsx_ctx *ctx, bfsx_graph_info *out);
+/* Copies the resident graph to host buffers of n_cols+1 / nnz entries. */
+int bfsx_download_csc(bfsx_ctx *ctx, void *col_ptr, uint32_t *row_idx);
+/* Seeded roots: mode 0 = vertices of degree >= 1, mode 1 = vertices of the
+ * largest connected component (symmetric graphs). */
+int bfsx_pick_roots(bfsx_ctx *ctx, uint32_t n_roots, uint64_t seed, int mode,
+                    uint32_t *roots_out, uint32_t *n_found);
+
+/* ---- search ------------------------------------------------------------- */
+/* One BFS from root.  levels_out: n_cols int32 (-1 = unreached) or NULL;
+ * parents_out: n_cols uint32 (0xFFFFFFFF = unreached, root -> root) or NULL;
+ * stats: optional. */
+int bfsx_run(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t run_flags,
+             int32_t *levels_out, uint32_t *parents_out, bfsx_run_stats *stats);
+/* n_roots searches back to back; levels_out is n_roots x n_cols (or NULL),
+ * stats is n_roots entries (or NULL). */
+int bfsx_run_batch(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots,
+                   int strategy, uint32_t run_flags, int32_t *levels_out,
+                   bfsx_run_stats *stats);
+/* Per-level records of the last run made with BFSX_RUN_TRACE. */
+int bfsx_last_trace(bfsx_ctx *ctx, bfsx_level_record *out, uint32_t max_records,
+                    uint32_t *n_records);
+
+/* ---- launch shape ------------------------------------------------------- */
+int bfsx_get_params(bfsx_ctx *ctx, bfsx_params *out);
+int bfsx_set_params(bfsx_ctx *ctx, const bfsx_params *in);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BFSX_H */

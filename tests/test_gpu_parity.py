@@ -1,0 +1,231 @@
+"""GPU parity: levels from the HIP kernels (through the C ABI) are bit-exact
+against the CPU oracle for every root and every step kind; the GPU graph
+builder is bit-exact against the oracle's restatement of the recipe."""
+import numpy as np
+import pytest
+
+import bfsx
+import workloads
+from graphs import (binary_tree, csc_from_edges, path_graph, random_graph, star_graph,
+                    symmetrize)
+from oracle import cpu
+
+pytestmark = pytest.mark.gpu
+
+STEPS = ["llb", "slb", "glb", "pull"]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def assert_levels(eng, cp, ri, roots, strategies, n_rows=None, parents=False):
+    for r in roots:
+        want, ec, _ = cpu.bfs_queue(cp, ri, int(r), n_rows=n_rows)
+        for s in strategies:
+            out = eng.run(int(r), s, parents=parents)
+            lev, st = out[0], out[1]
+            bad = np.flatnonzero(lev != want)
+            assert bad.size == 0, (s, int(r), bad[:10], lev[bad[:10]], want[bad[:10]])
+            assert st["edges_count"] == ec, (s, int(r))
+            assert st["reached"] == int((want >= 0).sum())
+            assert st["max_level"] == int(want.max())
+            if parents:
+                assert cpu.check_parents(cp, ri, int(r), want, out[2], n_rows=n_rows) == 0, s
+
+
+# ---------------------------------------------------------------------------
+# graph builder
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("scale,ptr", [(8, 4), (11, 4), (13, 8), (15, 4)])
+def test_rmat_builder_matches_oracle(engine, scale, ptr):
+    say("rmat builder", scale, ptr)
+    nnz = engine.generate_rmat(scale, 16, 0.57, 0.19, 0.19, 1000 + scale, 2000 + scale, ptr)
+    cp, ri = engine.download_csc()
+    wcp, wri = cpu.rmat_csc(scale, 16, 0.57, 0.19, 0.19, 1000 + scale, 2000 + scale, ptr)
+    assert nnz == len(wri)
+    assert cp.dtype == wcp.dtype
+    assert (cp == wcp).all() and (ri == wri).all()
+    info = engine.info()
+    deg = np.diff(cp.astype(np.int64))
+    assert info["max_degree"] == deg.max()
+    assert info["n_isolated"] == int((deg == 0).sum())
+    assert info["flags"] & bfsx.GRAPH_SYMMETRIC
+
+
+def test_grid_builder_matches_oracle(engine):
+    say("grid builder")
+    nnz = engine.generate_grid(64, 0.05, 7, 77, 4)
+    cp, ri = engine.download_csc()
+    wcp, wri = cpu.grid_csc(64, 0.05, 7, 77, 4)
+    assert nnz == len(wri) and (cp == wcp).all() and (ri == wri).all()
+
+
+def test_roots_match_oracle(engine):
+    w, _ = workloads.build(engine, "rmat16")
+    roots = workloads.roots(engine, w)
+    cp, ri = engine.download_csc()
+    deg = np.diff(cp.astype(np.int64))
+    want = cpu.pick_roots((deg > 0).astype(np.uint8), 64, w["seed_roots"])
+    assert len(roots) == 64 and (roots == want).all()
+
+
+# ---------------------------------------------------------------------------
+# parity on the workloads
+# ---------------------------------------------------------------------------
+def test_rmat16_all_roots_all_steps(engine):
+    w, _ = workloads.build(engine, "rmat16")
+    roots = workloads.roots(engine, w)
+    cp, ri = engine.download_csc()
+    say("rmat16", len(ri))
+    assert_levels(engine, cp, ri, roots, STEPS)
+
+
+def test_rmat19_all_roots_all_steps(engine):
+    w, _ = workloads.build(engine, "rmat19")
+    roots = workloads.roots(engine, w)
+    cp, ri = engine.download_csc()
+    say("rmat19", len(ri))
+    assert len(roots) == 64
+    assert_levels(engine, cp, ri, roots, STEPS)
+
+
+def test_rmat_uint64_col_ptr(engine):
+    engine.generate_rmat(14, 16, 0.57, 0.19, 0.19, 5, 6, 8)
+    cp, ri = engine.download_csc()
+    assert cp.dtype == np.uint64
+    roots = engine.pick_roots(8, 7)
+    assert_levels(engine, cp, ri, roots, STEPS)
+
+
+def test_parents_are_valid(engine):
+    engine.generate_rmat(14, 8, 0.57, 0.19, 0.19, 15, 16, 4)
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(6, 17)
+    assert_levels(engine, cp, ri, roots, STEPS, parents=True)
+
+
+def test_run_batch_equals_single_runs(engine):
+    engine.generate_rmat(13, 16, 0.57, 0.19, 0.19, 21, 22, 4)
+    roots = engine.pick_roots(5, 23)
+    lev, stats = engine.run_batch(roots, "glb", levels=True)
+    for i, r in enumerate(roots):
+        one, st = engine.run(int(r), "llb")
+        assert (lev[i] == one).all()
+        assert stats[i]["edges_count"] == st["edges_count"]
+
+
+# ---------------------------------------------------------------------------
+# launch shapes and edge cases
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("wg", [64, 128, 512, 1024])
+def test_llb_work_group_sizes(engine, wg):
+    engine.generate_rmat(13, 16, 0.57, 0.19, 0.19, 31, 32, 4)
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(4, 33)
+    old = engine.params
+    try:
+        engine.set_params(llb_wg=wg, hub_degree=0)
+        assert_levels(engine, cp, ri, roots, ["llb", "slb"])
+        engine.set_params(hub_degree=64)
+        assert_levels(engine, cp, ri, roots, ["llb", "slb"])
+    finally:
+        engine.set_params(**old)
+
+
+@pytest.mark.parametrize("glb_wg,chunk,bpc", [(128, 256, 1), (256, 512, 4), (1024, 65536, 2)])
+def test_glb_shapes(engine, glb_wg, chunk, bpc):
+    engine.generate_rmat(13, 16, 0.57, 0.19, 0.19, 41, 42, 4)
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(4, 43)
+    old = engine.params
+    try:
+        engine.set_params(glb_wg=glb_wg, glb_chunk=chunk, glb_blocks_per_cu=bpc)
+        assert_levels(engine, cp, ri, roots, ["glb"])
+    finally:
+        engine.set_params(**old)
+
+
+def test_directed_graph_with_fewer_rows(engine):
+    """Directed input, duplicates, self loops, n_rows < n_cols (reference quirk)."""
+    rng = np.random.default_rng(9)
+    n = 5000
+    cp, ri = random_graph(rng, n, 30000, directed=True)
+    n_rows = n - 3
+    engine.upload_csc(cp, ri, n_rows=n_rows, symmetric=False)
+    assert_levels(engine, cp, ri, rng.integers(0, n, 6), ["llb", "slb", "glb"], n_rows=n_rows)
+    with pytest.raises(bfsx.BfsxError) as e:
+        engine.run(0, "pull")
+    assert e.value.status == 7
+
+
+def test_closed_form_graphs(engine):
+    for cp, ri, root in [(path_graph(3000)[0], path_graph(3000)[1], 1500),
+                         (star_graph(70000)[0], star_graph(70000)[1], 5),
+                         (binary_tree(100000)[0], binary_tree(100000)[1], 0)]:
+        engine.upload_csc(cp, ri, symmetric=True)
+        assert_levels(engine, cp, ri, [root], STEPS)
+
+
+def test_tiny_and_empty_graphs(engine):
+    cp, ri = csc_from_edges(1, [], [])
+    engine.upload_csc(cp, ri, symmetric=True)
+    lev, st = engine.run(0, "llb")
+    assert list(lev) == [0] and st["edges_count"] == 0 and st["n_levels"] == 1
+    cp, ri = csc_from_edges(100, [], [])
+    engine.upload_csc(cp, ri, symmetric=True)
+    for s in STEPS:
+        lev, st = engine.run(42, s)
+        assert lev[42] == 0 and (np.delete(lev, 42) == -1).all()
+    cp, ri = csc_from_edges(4, *symmetrize([0, 1], [1, 2]))
+    engine.upload_csc(cp, ri, symmetric=True)
+    for s in STEPS:
+        lev, _ = engine.run(3, s)  # isolated root
+        assert list(lev) == [-1, -1, -1, 0]
+
+
+def test_bad_arguments(engine):
+    cp, ri = path_graph(10)
+    engine.upload_csc(cp, ri, symmetric=True)
+    with pytest.raises(bfsx.BfsxError) as e:
+        engine.run(10, "llb")
+    assert e.value.status == 1
+    with pytest.raises(bfsx.BfsxError):
+        engine.run(0, 9)
+    with pytest.raises(bfsx.BfsxError):
+        engine.set_params(llb_wg=96)
+    bad = cp.copy()
+    bad[3] = 0
+    with pytest.raises(bfsx.BfsxError):
+        engine.upload_csc(bad, ri)
+    with pytest.raises(bfsx.BfsxError):
+        engine.upload_csc(cp, np.full_like(ri, 10))
+
+
+def test_trace_is_consistent(engine):
+    engine.generate_rmat(14, 16, 0.57, 0.19, 0.19, 51, 52, 4)
+    cp, ri = engine.download_csc()
+    r = int(engine.pick_roots(1, 53)[0])
+    want, ec, _ = cpu.bfs_queue(cp, ri, r)
+    hist = np.bincount(want[want >= 0])
+    deg = np.diff(cp.astype(np.int64))
+    for s in STEPS:
+        lev, st = engine.run(r, s, trace=True)
+        tr = engine.last_trace()
+        assert len(tr) == st["n_levels"] == len(hist)
+        for k, rec in enumerate(tr):
+            assert rec["level"] == k and rec["choice"] == bfsx.STRATEGIES[s]
+            assert rec["frontier_vertices"] == hist[k]
+            assert rec["frontier_edges"] == deg[want == k].sum()
+            assert rec["discovered"] == (hist[k + 1] if k + 1 < len(hist) else 0)
+            assert rec["us"] > 0
+
+
+def test_largest_component_roots(engine):
+    engine.generate_grid(256, 0.05, 6, 99, 4)
+    roots = engine.pick_roots(16, 5, mode=1)
+    assert len(roots) == 16
+    cp, ri = engine.download_csc()
+    lev, _, _ = cpu.bfs_queue(cp, ri, int(roots[0]))
+    assert (lev >= 0).sum() * 2 > len(lev)
+    assert (lev[roots] >= 0).all()
