@@ -25,6 +25,7 @@ STEP_NAMES = ["llb", "glb", "slb", "pull"]
 GRAPH_SYMMETRIC = 1
 RUN_GRAPH_LOOP = 1
 RUN_TRACE = 2
+RUN_STEP_TIMES = 4
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "out of memory",
           4: "invalid state", 5: "i/o error", 6: "overflow", 7: "unsupported"}
@@ -65,7 +66,7 @@ class Params(C.Structure):
     _fields_ = [("llb_wg", C.c_uint32), ("glb_wg", C.c_uint32),
                 ("glb_blocks_per_cu", C.c_uint32), ("glb_chunk", C.c_uint32),
                 ("hub_degree", C.c_uint32), ("pull_wg", C.c_uint32),
-                ("graph_levels", C.c_uint32)]
+                ("graph_levels", C.c_uint32), ("mark_ratio_q10", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -95,6 +96,8 @@ SIGNATURES = {
     "bfsx_run": (C.c_int, [P, C.c_uint32, C.c_int, C.c_uint32, P, P, C.POINTER(RunStats)]),
     "bfsx_run_batch": (C.c_int, [P, P, C.c_uint32, C.c_int, C.c_uint32, P, P]),
     "bfsx_last_trace": (C.c_int, [P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "bfsx_level_histogram": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "bfsx_stream_copy": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "bfsx_get_params": (C.c_int, [P, C.POINTER(Params)]),
     "bfsx_set_params": (C.c_int, [P, C.POINTER(Params)]),
 }
@@ -208,12 +211,13 @@ class Engine:
 
     # ---- search ---------------------------------------------------------
     def run(self, root, strategy="llb", trace=False, parents=False, levels=True,
-            graph_loop=False):
+            graph_loop=False, step_times=False):
         g = self.info()
         lev = np.empty(g["n_cols"], np.int32) if levels else None
         par = np.empty(g["n_cols"], np.uint32) if parents else None
         st = RunStats()
-        flags = (RUN_TRACE if trace else 0) | (RUN_GRAPH_LOOP if graph_loop else 0)
+        flags = ((RUN_TRACE if trace else 0) | (RUN_GRAPH_LOOP if graph_loop else 0)
+                 | (RUN_STEP_TIMES if step_times else 0))
         self._check(lib().bfsx_run(self._h, int(root), strategy_id(strategy), flags, _ptr(lev),
                                    _ptr(par), C.byref(st)))
         out = [lev, st.as_dict()]
@@ -221,12 +225,14 @@ class Engine:
             out.append(par)
         return tuple(out)
 
-    def run_batch(self, roots, strategy="llb", levels=False, trace=False, graph_loop=False):
+    def run_batch(self, roots, strategy="llb", levels=False, trace=False, graph_loop=False,
+                  step_times=False):
         roots = np.ascontiguousarray(roots, np.uint32)
         g = self.info()
         lev = np.empty((len(roots), g["n_cols"]), np.int32) if levels else None
         st = (RunStats * max(len(roots), 1))()
-        flags = (RUN_TRACE if trace else 0) | (RUN_GRAPH_LOOP if graph_loop else 0)
+        flags = ((RUN_TRACE if trace else 0) | (RUN_GRAPH_LOOP if graph_loop else 0)
+                 | (RUN_STEP_TIMES if step_times else 0))
         self._check(lib().bfsx_run_batch(self._h, _ptr(roots), len(roots), strategy_id(strategy),
                                          flags, _ptr(lev), C.cast(st, P)))
         return lev, [st[i].as_dict() for i in range(len(roots))]
@@ -237,6 +243,25 @@ class Engine:
         recs = (LevelRecord * max(n.value, 1))()
         self._check(lib().bfsx_last_trace(self._h, C.cast(recs, P), n.value, C.byref(n)))
         return [{k: getattr(recs[i], k) for k, _ in LevelRecord._fields_} for i in range(n.value)]
+
+    def level_histogram(self, max_levels=1024):
+        """(counts, edges) per level of the last run; grows the table if needed."""
+        while True:
+            counts = np.zeros(max_levels, np.uint64)
+            edges = np.zeros(max_levels, np.uint64)
+            n = C.c_uint32(0)
+            rc = lib().bfsx_level_histogram(self._h, _ptr(counts), _ptr(edges), max_levels,
+                                            C.byref(n))
+            if rc == 6 and n.value > max_levels:
+                max_levels = n.value
+                continue
+            self._check(rc)
+            return counts[:n.value].copy(), edges[:n.value].copy()
+
+    def stream_copy(self, nbytes, repeats=10):
+        ms = C.c_double(0)
+        self._check(lib().bfsx_stream_copy(self._h, nbytes, repeats, C.byref(ms)))
+        return ms.value
 
     @property
     def params(self):

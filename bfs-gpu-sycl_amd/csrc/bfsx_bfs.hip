@@ -125,11 +125,27 @@ struct KArgs {
   uint32_t *parents;  // nullable
   int32_t next_level;
   uint32_t hub_degree;
+  uint8_t *mark;      // non-null: mark-then-compact mode
 };
 
+// Two ways to discover a neighbour n of a frontier vertex:
+//  * exact-once: atomicOr on the visited bitmap picks one winner, which
+//    writes the level and appends n right away (few edges: thin levels);
+//  * mark: a plain byte store into `mark` for every edge that sees n
+//    unvisited in the bitmap as it was at the start of the level; all
+//    writers store the same value, so no atomic is needed.  k_compact then
+//    turns marks into levels, bitmap bits and the next queue (heavy levels,
+//    where per-edge device-scope atomics would dominate).
 template <typename PtrT>
 __device__ __forceinline__ void discover(const KArgs &a, const PtrT *col_ptr, uint32_t n,
                                          uint32_t from) {
+  if (a.mark) {
+    if (!((a.vis[n >> 5] >> (n & 31)) & 1u)) {
+      a.mark[n] = 1;
+      if (a.parents) a.parents[n] = from;
+    }
+    return;
+  }
   const bool isnew = try_visit(a.vis, n);
   if (isnew) {
     a.levels[n] = a.next_level;
@@ -401,6 +417,71 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
   }
 }
 
+// ---------------------------------------------------------------------------
+// level generation after a mark step (the reference's LevelGen, :481-521)
+// ---------------------------------------------------------------------------
+// 4-bit mask of the non-zero bytes of w.
+__device__ __forceinline__ uint32_t nz_bytes(uint32_t w) {
+  uint32_t t = (((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u;
+  return ((t >> 7) & 1u) | ((t >> 14) & 2u) | ((t >> 21) & 4u) | ((t >> 28) & 8u);
+}
+
+// A wave owns a 1024-vertex chunk: each lane reads 16 mark bytes as one
+// uint4, so the whole mark array streams through once.  New vertices get
+// their level, are appended to the next queue with one atomicAdd per wave
+// (positions by a wave prefix sum of per-lane counts), their visited bits
+// are OR-ed into words this wave owns (no atomics), and the next frontier
+// bitmap words are written in full.  Marks are cleared where set.
+template <typename PtrT>
+__global__ __launch_bounds__(256) void k_compact(const PtrT *col_ptr, uint32_t n_rows,
+                                                 uint8_t *mark, uint32_t n_chunks,
+                                                 uint32_t *vis, int32_t *levels,
+                                                 int32_t next_level, uint32_t *q_out,
+                                                 uint32_t *q_count,
+                                                 unsigned long long *q_edges,
+                                                 uint32_t *fbm_next) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
+  for (uint32_t c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); c < n_chunks;
+       c += nwaves) {
+    uint4 *mp = reinterpret_cast<uint4 *>(mark) + (size_t)c * 64 + lane;
+    const uint4 m = *mp;
+    uint32_t bits = 0;
+    if (m.x | m.y | m.z | m.w) {
+      bits = nz_bytes(m.x) | (nz_bytes(m.y) << 4) | (nz_bytes(m.z) << 8) | (nz_bytes(m.w) << 12);
+      *mp = make_uint4(0, 0, 0, 0);
+    }
+    const uint32_t cnt = __popc(bits);
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t base = 0;
+    if (total) {
+      if (lane == 63) base = atomicAdd(q_count, total);
+      base = __shfl(base, 63, 64);
+    }
+    uint32_t pos = base + incl - cnt;
+    const uint32_t v0 = c * 1024 + lane * 16;
+    unsigned long long dsum = 0;
+    for (uint32_t b = bits; b; b &= b - 1) {
+      const uint32_t v = v0 + (__ffs(b) - 1);
+      levels[v] = next_level;
+      q_out[pos++] = v;
+      if (q_edges && v < n_rows) dsum += (unsigned long long)(col_ptr[v + 1] - col_ptr[v]);
+    }
+    const uint32_t partner = __shfl_xor(bits, 1, 64);
+    if ((lane & 1) == 0) {
+      const uint32_t word = bits | (partner << 16);
+      const uint32_t wi = c * 32 + (lane >> 1);
+      if (word) vis[wi] |= word;
+      if (fbm_next) fbm_next[wi] = word;
+    }
+    if (q_edges && total) {
+      dsum = wave_sum(dsum);
+      if (lane == 0 && dsum) atomicAdd(q_edges, dsum);
+    }
+  }
+}
+
 // frontier bitmap of one level from the levels array (a wave owns 64 vertices)
 __global__ void k_levels_to_fbm(const int32_t *levels, uint32_t n_cols, int32_t level,
                                 uint32_t *fbm, uint32_t n_groups) {
@@ -498,6 +579,51 @@ __global__ void k_summarize(const PtrT *col_ptr, const int32_t *levels, uint32_t
   }
 }
 
+// per-level vertex counts and out-degree sums; LDS histogram per block when
+// the level count is small, global atomics otherwise
+constexpr uint32_t HIST_LDS_LEVELS = 1024;
+
+template <typename PtrT>
+__global__ __launch_bounds__(256) void k_level_hist(const PtrT *col_ptr, const int32_t *levels,
+                                                    uint32_t n_rows, uint32_t n_cols,
+                                                    uint32_t max_levels,
+                                                    unsigned long long *counts,
+                                                    unsigned long long *edges) {
+  __shared__ unsigned long long s_cnt[HIST_LDS_LEVELS];
+  __shared__ unsigned long long s_edg[HIST_LDS_LEVELS];
+  const bool lds = max_levels <= HIST_LDS_LEVELS;
+  if (lds)
+    for (uint32_t i = threadIdx.x; i < max_levels; i += blockDim.x) s_cnt[i] = s_edg[i] = 0;
+  __syncthreads();
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n_cols;
+       v += gridDim.x * blockDim.x) {
+    const int32_t l = levels[v];
+    if (l < 0 || (uint32_t)l >= max_levels) continue;
+    const unsigned long long d =
+        v < n_rows ? (unsigned long long)(col_ptr[v + 1] - col_ptr[v]) : 0ull;
+    if (lds) {
+      atomicAdd(&s_cnt[l], 1ull);
+      if (d) atomicAdd(&s_edg[l], d);
+    } else {
+      atomicAdd(&counts[l], 1ull);
+      if (d) atomicAdd(&edges[l], d);
+    }
+  }
+  __syncthreads();
+  if (lds)
+    for (uint32_t i = threadIdx.x; i < max_levels; i += blockDim.x) {
+      if (s_cnt[i]) atomicAdd(&counts[i], s_cnt[i]);
+      if (s_edg[i]) atomicAdd(&edges[i], s_edg[i]);
+    }
+}
+
+__global__ void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                              uint64_t n16) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -522,6 +648,7 @@ static KArgs kargs(const StepArgs &s, uint32_t hub) {
   a.parents = s.parents;
   a.next_level = s.next_level;
   a.hub_degree = hub;
+  a.mark = s.mark;
   return a;
 }
 
@@ -631,6 +758,22 @@ void step_pull(bfsx_ctx *ctx, const StepArgs &s) {
   BFSX_HIP(hipGetLastError());
 }
 
+void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next) {
+  const uint32_t n_chunks = (s.n_cols + 1023) / 1024;
+  const unsigned grid = cap_grid(((uint64_t)n_chunks + 3) / 4, (unsigned)ctx->cu_count * 16);
+  unsigned long long *qe = s.track_degrees ? &s.ctl->q_edges[s.q_out_idx] : nullptr;
+  uint32_t *qc = &s.ctl->q_count[s.q_out_idx];
+  if (ctx->ptr_bytes == 8)
+    hipLaunchKernelGGL(k_compact<uint64_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       static_cast<const uint64_t *>(s.col_ptr), s.n_rows, mark, n_chunks, s.vis,
+                       s.levels, s.next_level, s.q_out, qc, qe, fbm_next);
+  else
+    hipLaunchKernelGGL(k_compact<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       static_cast<const uint32_t *>(s.col_ptr), s.n_rows, mark, n_chunks, s.vis,
+                       s.levels, s.next_level, s.q_out, qc, qe, fbm_next);
+  BFSX_HIP(hipGetLastError());
+}
+
 void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level, uint32_t *fbm) {
   const uint32_t n_groups = (ctx->n_cols + 63) / 64;
   const unsigned grid = cap_grid(((uint64_t)n_groups * 64 + 255) / 256,
@@ -641,7 +784,7 @@ void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level, uint32_t
 }
 
 void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *par) {
-  const uint32_t n_words = ((ctx->n_cols + 63) / 64) * 2;
+  const uint32_t n_words = (uint32_t)bitmap_words(ctx->n_cols);
   const unsigned grid = cap_grid(((uint64_t)ctx->n_cols + 255) / 256,
                                  (unsigned)ctx->cu_count * 16);
   const uint32_t *iso = symmetric_iso ? ctx->iso_bm.as<uint32_t>() : nullptr;
@@ -684,6 +827,29 @@ void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out) {
   BFSX_HIP(hipGetLastError());
 }
 
-void count_traversed(bfsx_ctx *ctx, unsigned long long *d_out) { reached_summary(ctx, d_out); }
+void level_histogram(bfsx_ctx *ctx, uint32_t max_levels, unsigned long long *d_counts,
+                     unsigned long long *d_edges) {
+  const unsigned grid = cap_grid(((uint64_t)ctx->n_cols + 255) / 256,
+                                 (unsigned)ctx->cu_count * 4);
+  BFSX_HIP(hipMemsetAsync(d_counts, 0, sizeof(unsigned long long) * max_levels, ctx->stream));
+  BFSX_HIP(hipMemsetAsync(d_edges, 0, sizeof(unsigned long long) * max_levels, ctx->stream));
+  if (ctx->ptr_bytes == 8)
+    hipLaunchKernelGGL(k_level_hist<uint64_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->col_ptr.as<uint64_t>(), ctx->levels.as<int32_t>(), ctx->n_rows,
+                       ctx->n_cols, max_levels, d_counts, d_edges);
+  else
+    hipLaunchKernelGGL(k_level_hist<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->col_ptr.as<uint32_t>(), ctx->levels.as<int32_t>(), ctx->n_rows,
+                       ctx->n_cols, max_levels, d_counts, d_edges);
+  BFSX_HIP(hipGetLastError());
+}
+
+void stream_copy(bfsx_ctx *ctx, const void *src, void *dst, uint64_t bytes) {
+  const uint64_t n16 = bytes / 16;
+  const unsigned grid = cap_grid((n16 + 255) / 256, (unsigned)ctx->cu_count * 16);
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream,
+                     static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n16);
+  BFSX_HIP(hipGetLastError());
+}
 
 }  // namespace bfsx

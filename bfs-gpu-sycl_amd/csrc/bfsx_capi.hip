@@ -46,24 +46,26 @@ static void set_device(bfsx_ctx *ctx) { BFSX_HIP(hipSetDevice(ctx->device)); }
 static void free_search_state(bfsx_ctx *ctx) {
   for (DevBuf *b : {&ctx->levels, &ctx->parents, &ctx->vis, &ctx->fbm[0], &ctx->fbm[1],
                     &ctx->queue[0], &ctx->queue[1], &ctx->glb_scan, &ctx->glb_start,
-                    &ctx->glb_tile, &ctx->glb_chunk_lo})
+                    &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark})
     release(*b);
 }
 
 static void alloc_search_state(bfsx_ctx *ctx) {
   hipStream_t s = ctx->stream;
   const uint64_t n = ctx->n_cols;
-  const uint64_t n_words = ((n + 63) / 64) * 2;
   ensure(ctx->levels, sizeof(int32_t) * n, s);
-  ensure(ctx->vis, sizeof(uint32_t) * n_words, s);
+  ensure(ctx->vis, sizeof(uint32_t) * bitmap_words(n), s);
   ensure(ctx->queue[0], sizeof(uint32_t) * (n + 1), s);
   ensure(ctx->queue[1], sizeof(uint32_t) * (n + 1), s);
+  ensure(ctx->mark, mark_bytes(n), s);
+  BFSX_HIP(hipMemsetAsync(ctx->mark.p, 0, mark_bytes(n), s));
+  BFSX_HIP(hipStreamSynchronize(s));
 }
 
 static void ensure_pull_state(bfsx_ctx *ctx) {
-  const uint64_t n_words = (((uint64_t)ctx->n_cols + 63) / 64) * 2;
-  ensure(ctx->fbm[0], sizeof(uint32_t) * n_words, ctx->stream);
-  ensure(ctx->fbm[1], sizeof(uint32_t) * n_words, ctx->stream);
+  const uint64_t w = bitmap_words(ctx->n_cols);
+  ensure(ctx->fbm[0], sizeof(uint32_t) * w, ctx->stream);
+  ensure(ctx->fbm[1], sizeof(uint32_t) * w, ctx->stream);
 }
 
 static void after_graph_change(bfsx_ctx *ctx) {
@@ -81,6 +83,7 @@ static bfsx_params default_params() {
   p.hub_degree = 0;
   p.pull_wg = 256;
   p.graph_levels = 8;
+  p.mark_ratio_q10 = 64;
   return p;
 }
 
@@ -111,21 +114,34 @@ static void read_ctl(bfsx_ctx *ctx) {
   BFSX_HIP(hipStreamSynchronize(ctx->stream));
 }
 
+static bool mark_step(const bfsx_ctx *ctx, uint32_t F) {
+  const uint32_t r = ctx->params.mark_ratio_q10;
+  if (r == 0) return true;
+  if (r == 0xFFFFFFFFu) return false;
+  const double avg_deg = ctx->n_rows ? (double)ctx->nnz / ctx->n_rows : 0.0;
+  return (double)F * avg_deg * 1024.0 >= (double)ctx->n_cols * r;
+}
+
 static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
                      bool want_parents) {
   if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
   if (root >= ctx->n_cols) fail(BFSX_ERR_ARG, "root out of range");
-  if (strategy < BFSX_LLB || strategy > BFSX_PULL)
-    fail(BFSX_ERR_ARG, "unknown strategy");
+  if (strategy < BFSX_LLB || strategy > BFSX_PULL) fail(BFSX_ERR_ARG, "unknown strategy");
   const bool symmetric = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols;
   if (strategy == BFSX_PULL && !symmetric)
     fail(BFSX_ERR_UNSUPPORTED, "bottom-up steps need a symmetric graph with n_rows == n_cols");
   const bool trace = flags & BFSX_RUN_TRACE;
+  const bool timed = trace || (flags & BFSX_RUN_STEP_TIMES);
+  const bool pull_possible = strategy == BFSX_PULL;
   hipStream_t s = ctx->stream;
   if (want_parents) ensure(ctx->parents, sizeof(uint32_t) * ctx->n_cols, s);
-  if (strategy == BFSX_PULL) ensure_pull_state(ctx);
+  if (pull_possible) ensure_pull_state(ctx);
   uint32_t *par = want_parents ? ctx->parents.as<uint32_t>() : nullptr;
   // initial state outside the timer (host.cpp:113-121 / run_bfs_gpu.hpp:674-677)
+  if (ctx->mark_dirty) {
+    BFSX_HIP(hipMemsetAsync(ctx->mark.p, 0, mark_bytes(ctx->n_cols), s));
+    ctx->mark_dirty = false;
+  }
   init_search(ctx, root, symmetric, par);
   BFSX_HIP(hipStreamSynchronize(s));
   ctx->trace.clear();
@@ -139,8 +155,8 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     f_edges = ctx->h_ctl->q_edges[0];
     visited_e = ctx->h_ctl->visited_edges;
   }
-  int fbm_cur = 0;
-  bool fbm_valid = false;
+  int fbm_cur = 0;          // buffer holding the current frontier's bitmap
+  bool fbm_valid = false;   // whether it is up to date
   while (F > 0) {
     const int choice = strategy;
     StepArgs a{};
@@ -159,46 +175,54 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     a.ctl = ctx->ctl.as<LevelCtl>();
     a.next_level = (int32_t)level + 1;
     a.track_degrees = trace;
-    if (trace) {
-      BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
-    }
+    const bool use_mark = choice != BFSX_PULL && mark_step(ctx, F);
+    a.mark = use_mark ? ctx->mark.as<uint8_t>() : nullptr;
+    if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
+    bool new_fbm = false;
     switch (choice) {
       case BFSX_LLB: step_llb(ctx, a, false); break;
       case BFSX_SLB: step_llb(ctx, a, true); break;
       case BFSX_GLB: step_glb(ctx, a); break;
       case BFSX_PULL:
-        if (!fbm_valid) levels_to_fbm(ctx, a.levels, (int32_t)level, ctx->fbm[fbm_cur].as<uint32_t>());
+        if (!fbm_valid)
+          levels_to_fbm(ctx, a.levels, (int32_t)level, ctx->fbm[fbm_cur].as<uint32_t>());
         a.fbm_cur = ctx->fbm[fbm_cur].as<uint32_t>();
         a.fbm_next = ctx->fbm[fbm_cur ^ 1].as<uint32_t>();
         step_pull(ctx, a);
+        new_fbm = true;
         break;
     }
-    if (trace) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
+    if (use_mark) {
+      ctx->mark_dirty = true;
+      uint32_t *fbm_next = pull_possible ? ctx->fbm[fbm_cur ^ 1].as<uint32_t>() : nullptr;
+      compact(ctx, a, a.mark, fbm_next);
+      new_fbm = fbm_next != nullptr;
+    }
+    if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
     level_end(ctx, cur, cur ^ 1, false);
     read_ctl(ctx);
+    if (use_mark) ctx->mark_dirty = false;
     const uint32_t Fn = ctx->h_ctl->q_count[cur ^ 1];
-    if (choice == BFSX_PULL) {
-      fbm_cur ^= 1;
-      fbm_valid = true;
-    } else {
-      fbm_valid = false;
-    }
-    if (trace) {
+    if (new_fbm) fbm_cur ^= 1;
+    fbm_valid = new_fbm;
+    if (timed) {
       bfsx_level_record r{};
       r.level = level;
       r.choice = (uint32_t)choice;
       r.frontier_vertices = F;
-      r.frontier_edges = f_edges;
-      r.unvisited_vertices = ctx->n_cols - visited_v;
-      r.unvisited_edges = ctx->nnz - visited_e;
       r.discovered = Fn;
+      if (trace) {
+        r.frontier_edges = f_edges;
+        r.unvisited_vertices = ctx->n_cols - visited_v;
+        r.unvisited_edges = ctx->nnz - visited_e;
+        f_edges = ctx->h_ctl->q_edges[cur ^ 1];
+        visited_v = ctx->h_ctl->visited_vertices;
+        visited_e = ctx->h_ctl->visited_edges;
+      }
       float us = 0;
       BFSX_HIP(hipEventElapsedTime(&us, ctx->ev_step0, ctx->ev_step1));
       r.us = us * 1000.0;
       ctx->trace.push_back(r);
-      f_edges = ctx->h_ctl->q_edges[cur ^ 1];
-      visited_v = ctx->h_ctl->visited_vertices;
-      visited_e = ctx->h_ctl->visited_edges;
     }
     out.steps[choice]++;
     out.n_levels++;
@@ -582,6 +606,55 @@ int bfsx_last_trace(bfsx_ctx *ctx, bfsx_level_record *out, uint32_t max_records,
     const uint32_t k = (uint32_t)std::min<size_t>(ctx->trace.size(), max_records);
     if (out && k) std::copy(ctx->trace.begin(), ctx->trace.begin() + k, out);
     if (n) *n = (uint32_t)ctx->trace.size();
+  });
+}
+
+int bfsx_level_histogram(bfsx_ctx *ctx, uint64_t *counts, uint64_t *edges, uint32_t max_levels,
+                         uint32_t *n_levels) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (!ctx->has_graph || !ctx->levels.p) fail(BFSX_ERR_STATE, "no search has run");
+    if (max_levels == 0 || max_levels > (1u << 24)) fail(BFSX_ERR_ARG, "max_levels out of range");
+    hipStream_t s = ctx->stream;
+    ensure(ctx->scratch, sizeof(unsigned long long) * (2 * (size_t)max_levels + 3), s);
+    unsigned long long *d = ctx->scratch.as<unsigned long long>();
+    reached_summary(ctx, d);
+    level_histogram(ctx, max_levels, d + 3, d + 3 + max_levels);
+    std::vector<unsigned long long> h(2 * (size_t)max_levels + 3);
+    BFSX_HIP(hipMemcpyAsync(h.data(), d, sizeof(unsigned long long) * h.size(),
+                            hipMemcpyDeviceToHost, s));
+    BFSX_HIP(hipStreamSynchronize(s));
+    const uint32_t nl = h[1] ? (uint32_t)h[2] + 1 : 0;
+    if (n_levels) *n_levels = nl;
+    if (nl > max_levels) fail(BFSX_ERR_OVERFLOW, "more levels than max_levels");
+    for (uint32_t i = 0; i < max_levels; i++) {
+      if (counts) counts[i] = h[3 + i];
+      if (edges) edges[i] = h[3 + max_levels + i];
+    }
+  });
+}
+
+int bfsx_stream_copy(bfsx_ctx *ctx, uint64_t bytes, int repeats, double *ms) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (bytes < 16 || repeats < 1) fail(BFSX_ERR_ARG, "bytes >= 16 and repeats >= 1 required");
+    bytes &= ~15ull;
+    DevBuf a, b;
+    ensure(a, bytes, ctx->stream);
+    ensure(b, bytes, ctx->stream);
+    BFSX_HIP(hipMemsetAsync(a.p, 1, bytes, ctx->stream));
+    stream_copy(ctx, a.p, b.p, bytes);  // warm-up
+    BFSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int i = 0; i < repeats; i++) stream_copy(ctx, a.p, b.p, bytes);
+    BFSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    BFSX_HIP(hipEventSynchronize(ctx->ev1));
+    float t = 0;
+    BFSX_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    if (ms) *ms = t / repeats;
+    release(a);
+    release(b);
   });
 }
 

@@ -307,7 +307,7 @@ template <typename PtrT>
 __global__ void k_summary(const PtrT *col_ptr, uint32_t n_rows, uint32_t n_cols,
                           uint32_t *iso_bm, unsigned int *max_deg,
                           unsigned int *n_iso) {
-  const uint32_t nwords = ((n_cols + 63) / 64) * 2;
+  const uint32_t nwords = (uint32_t)bitmap_words(n_cols);
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
        w += gridDim.x * blockDim.x) {
     uint32_t bits = 0, mx = 0;
@@ -328,7 +328,7 @@ __global__ void k_summary(const PtrT *col_ptr, uint32_t n_rows, uint32_t n_cols,
 
 void graph_summary(bfsx_ctx *ctx) {
   hipStream_t s = ctx->stream;
-  const uint32_t nwords = ((ctx->n_cols + 63) / 64) * 2;
+  const uint32_t nwords = (uint32_t)bitmap_words(ctx->n_cols);
   ensure(ctx->iso_bm, sizeof(uint32_t) * (nwords ? nwords : 2), s);
   DevBuf out;
   ensure(out, 2 * sizeof(unsigned int), s);

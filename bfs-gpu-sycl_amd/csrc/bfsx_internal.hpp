@@ -33,6 +33,11 @@ struct LevelCtl {
   float sched_in[8];               // scheduler inputs of the last decision
 };
 
+// Bitmaps hold whole 1024-vertex chunks (32 words) so that a wave can own a
+// chunk; mark arrays hold one byte per vertex, padded the same way.
+__host__ __device__ inline uint64_t bitmap_words(uint64_t n) { return ((n + 1023) / 1024) * 32; }
+__host__ __device__ inline uint64_t mark_bytes(uint64_t n) { return ((n + 1023) / 1024) * 1024; }
+
 // Growable device allocation.
 struct DevBuf {
   void *p = nullptr;
@@ -61,10 +66,11 @@ struct bfsx_ctx {
   bfsx::DevBuf col_ptr, row_idx, iso_bm;
 
   // per-search state, sized for the resident graph
-  bfsx::DevBuf levels, parents, vis, fbm[2], queue[2], ctl;
+  bfsx::DevBuf levels, parents, vis, fbm[2], queue[2], ctl, mark;
   bfsx::DevBuf glb_scan, glb_start, glb_tile, glb_chunk_lo;
   bfsx::DevBuf scratch;  // generic scratch (reductions, root picking)
   bfsx::LevelCtl *h_ctl = nullptr;  // pinned mirror
+  bool mark_dirty = false;          // a mark step did not reach its compaction
 
   bfsx_params params{};
   std::vector<bfsx_level_record> trace;
@@ -119,6 +125,7 @@ struct StepArgs {
   LevelCtl *ctl;
   int32_t next_level;
   bool track_degrees;      // accumulate degree sum of discovered vertices
+  uint8_t *mark;           // non-null: mark-then-compact step
 };
 
 void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *parents);
@@ -127,10 +134,13 @@ void step_glb(bfsx_ctx *ctx, const StepArgs &a);
 void step_pull(bfsx_ctx *ctx, const StepArgs &a);
 void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level,
                    uint32_t *fbm);
+void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next);
 void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
                bool stamp);
 void level_begin(bfsx_ctx *ctx);
-void count_traversed(bfsx_ctx *ctx, unsigned long long *d_out);
 void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out);
+void level_histogram(bfsx_ctx *ctx, uint32_t max_levels, unsigned long long *d_counts,
+                     unsigned long long *d_edges);
+void stream_copy(bfsx_ctx *ctx, const void *src, void *dst, uint64_t bytes);
 
 }  // namespace bfsx

@@ -62,6 +62,8 @@ extern "C" {
 #define BFSX_RUN_GRAPH_LOOP 1u  /* capture the level loop in a hipGraph with a
                                    device-side termination flag */
 #define BFSX_RUN_TRACE 2u       /* record per-level statistics and timings */
+#define BFSX_RUN_STEP_TIMES 4u  /* record per-level step times only (no
+                                   extra per-vertex work in the kernels) */
 
 typedef struct bfsx_ctx bfsx_ctx;
 
@@ -110,6 +112,12 @@ typedef struct bfsx_params {
                                degree are expanded by a whole wave (0 = off) */
   uint32_t pull_wg;         /* work-group size of the bottom-up step */
   uint32_t graph_levels;    /* levels per captured hipGraph launch */
+  uint32_t mark_ratio_q10;  /* top-down steps whose estimated frontier edges
+                               reach n_cols * mark_ratio_q10 / 1024 mark
+                               neighbours with plain byte stores and compact
+                               afterwards; smaller ones discover exact-once
+                               with atomics (0 = always mark,
+                               0xFFFFFFFF = never) */
 } bfsx_params;
 
 /* ---- context ------------------------------------------------------------ */
@@ -157,9 +165,20 @@ int bfsx_run(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t run_flags,
 int bfsx_run_batch(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots,
                    int strategy, uint32_t run_flags, int32_t *levels_out,
                    bfsx_run_stats *stats);
-/* Per-level records of the last run made with BFSX_RUN_TRACE. */
+/* Per-level records of the last run made with BFSX_RUN_TRACE or
+ * BFSX_RUN_STEP_TIMES (the latter fills level, choice, frontier_vertices,
+ * discovered and us). */
 int bfsx_last_trace(bfsx_ctx *ctx, bfsx_level_record *out, uint32_t max_records,
                     uint32_t *n_records);
+/* Level histogram of the last run, computed on the device: counts[l] =
+ * vertices at level l, edges[l] = their out-degree sum (vertices < n_rows).
+ * The reference prints and stores the same histogram (src/host.cpp:27-54,
+ * :205-208, "LevelsGPU").  *n_levels = max level + 1. */
+int bfsx_level_histogram(bfsx_ctx *ctx, uint64_t *counts, uint64_t *edges,
+                         uint32_t max_levels, uint32_t *n_levels);
+/* Device-to-device copy of `bytes` through a plain streaming kernel; *ms is
+ * its duration.  Used to calibrate HBM counters and bandwidth. */
+int bfsx_stream_copy(bfsx_ctx *ctx, uint64_t bytes, int repeats, double *ms);
 
 /* ---- launch shape ------------------------------------------------------- */
 int bfsx_get_params(bfsx_ctx *ctx, bfsx_params *out);

@@ -133,6 +133,26 @@ def test_llb_work_group_sizes(engine, wg):
         engine.set_params(**old)
 
 
+@pytest.mark.parametrize("ratio", [0, 16, 0xFFFFFFFF])
+def test_mark_and_atomic_discovery_modes(engine, ratio):
+    """Every level marked+compacted (0), every level exact-once (never), mixed."""
+    engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 71, 72, 4)
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(6, 73)
+    old = engine.params
+    try:
+        engine.set_params(mark_ratio_q10=ratio)
+        assert_levels(engine, cp, ri, roots, STEPS, parents=True)
+        r = int(roots[0])
+        want, _, _ = cpu.bfs_queue(cp, ri, r)
+        hist = np.bincount(want[want >= 0])
+        for s in STEPS:
+            engine.run(r, s, trace=True)
+            assert [t["frontier_vertices"] for t in engine.last_trace()] == list(hist)
+    finally:
+        engine.set_params(**old)
+
+
 @pytest.mark.parametrize("glb_wg,chunk,bpc", [(128, 256, 1), (256, 512, 4), (1024, 65536, 2)])
 def test_glb_shapes(engine, glb_wg, chunk, bpc):
     engine.generate_rmat(13, 16, 0.57, 0.19, 0.19, 41, 42, 4)
@@ -229,3 +249,25 @@ def test_largest_component_roots(engine):
     lev, _, _ = cpu.bfs_queue(cp, ri, int(roots[0]))
     assert (lev >= 0).sum() * 2 > len(lev)
     assert (lev[roots] >= 0).all()
+
+
+def test_level_histogram_and_step_times(engine):
+    engine.generate_rmat(14, 16, 0.57, 0.19, 0.19, 61, 62, 8)
+    cp, ri = engine.download_csc()
+    r = int(engine.pick_roots(1, 63)[0])
+    want, _, _ = cpu.bfs_queue(cp, ri, r)
+    deg = np.diff(cp.astype(np.int64))
+    for s in STEPS:
+        engine.run(r, s, step_times=True, levels=False)
+        tr = engine.last_trace()
+        counts, edges = engine.level_histogram()
+        hist = np.bincount(want[want >= 0])
+        assert (counts == hist).all()
+        assert (edges == np.array([deg[want == k].sum() for k in range(len(hist))])).all()
+        assert [t["frontier_vertices"] for t in tr] == list(hist)
+        assert all(t["us"] > 0 for t in tr)
+
+
+def test_stream_copy(engine):
+    ms = engine.stream_copy(1 << 26, 4)
+    assert ms > 0
