@@ -184,7 +184,10 @@ def main():
         roots = roots[:args.roots]
     strategy = args.strategy
     if strategy == "auto":
-        strategy = "learned" if getattr(bfsx, "have_scheduler_weights", lambda: False)() else "llb"
+        strategy = "learned" if bfsx.have_scheduler_weights() else "llb"
+    sched_meta = None
+    if strategy == "learned":
+        sched_meta = eng.load_scheduler()
     log("strategy", strategy, "roots", len(roots))
 
     # warm-up, then the timed steps (each a sweep over all roots)
@@ -226,6 +229,10 @@ def main():
         result["fixed_llb_gteps"] = round(float(llb), 3)
         result["speedup_vs_fixed_llb"] = round(float(value / llb), 3)
         result["steps_used"] = {k: int(sum(s["steps"][k] for s in stats)) for k in bfsx.STEP_NAMES}
+        if sched_meta is not None:
+            result["scheduler"] = {"train_workloads": sched_meta["train_workloads"],
+                                   "rows": sched_meta["rows"],
+                                   "loss_final": round(sched_meta["loss_final"], 4)}
     if not args.no_roofline:
         rl = roofline_leg(eng, roots, strategy, info["col_ptr_bytes"])
         rl["traffic"] = traffic_from_profiles(args.workload, rl["step"])

@@ -98,6 +98,12 @@ SIGNATURES = {
     "bfsx_last_trace": (C.c_int, [P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "bfsx_level_histogram": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "bfsx_stream_copy": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "bfsx_sched_set_weights": (C.c_int, [P, P, C.c_uint32]),
+    "bfsx_mlp_forward": (C.c_int, [P, P, C.c_uint32, P, P]),
+    "bfsx_mlp_loss_grad": (C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
+    "bfsx_mlp_train": (C.c_int, [P, P, P, P, C.c_uint32, P, C.c_uint32, C.c_float, P]),
+    "bfsx_sched_record": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, P, P, P,
+                                    C.c_uint32, C.POINTER(C.c_uint32)]),
     "bfsx_get_params": (C.c_int, [P, C.POINTER(Params)]),
     "bfsx_set_params": (C.c_int, [P, C.POINTER(Params)]),
 }
@@ -263,6 +269,55 @@ class Engine:
         self._check(lib().bfsx_stream_copy(self._h, nbytes, repeats, C.byref(ms)))
         return ms.value
 
+    # ---- learned scheduler ----------------------------------------------
+    def set_scheduler_weights(self, w):
+        if w is None:
+            self._check(lib().bfsx_sched_set_weights(self._h, None, 0))
+            return
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(lib().bfsx_sched_set_weights(self._h, _ptr(w), len(w)))
+
+    def load_scheduler(self, path=None):
+        """Loads the committed trained weights; returns the metadata dict."""
+        meta = load_scheduler_file(path)
+        self.set_scheduler_weights(np.asarray(meta["weights"], np.float32))
+        return meta
+
+    def mlp_forward(self, X, W):
+        X = np.ascontiguousarray(X, np.float32)
+        W = np.ascontiguousarray(W, np.float32)
+        Y = np.empty((len(X), 4), np.float32)
+        self._check(lib().bfsx_mlp_forward(self._h, _ptr(X), len(X), _ptr(W), _ptr(Y)))
+        return Y
+
+    def mlp_loss_grad(self, X, T, M, W):
+        X, T, M, W = (np.ascontiguousarray(a, np.float32) for a in (X, T, M, W))
+        g = np.empty(SCHED_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(lib().bfsx_mlp_loss_grad(self._h, _ptr(X), _ptr(T), _ptr(M), len(X), _ptr(W),
+                                             _ptr(g), _ptr(loss)))
+        return float(loss[0]), g
+
+    def mlp_train(self, X, T, M, W, epochs, lr):
+        X, T, M = (np.ascontiguousarray(a, np.float32) for a in (X, T, M))
+        W = np.array(W, np.float32, copy=True)
+        curve = np.empty(max(epochs, 1), np.float32)
+        self._check(lib().bfsx_mlp_train(self._h, _ptr(X), _ptr(T), _ptr(M), len(X), _ptr(W),
+                                         epochs, lr, _ptr(curve)))
+        return W, curve[:epochs]
+
+    def sched_record(self, roots, repeats=2, epsilon=0.0, seed=0, max_rows=1 << 20):
+        roots = np.ascontiguousarray(roots, np.uint32)
+        X = np.empty((max_rows, 8), np.float32)
+        T = np.empty((max_rows, 4), np.float32)
+        M = np.empty((max_rows, 4), np.float32)
+        n = C.c_uint32(0)
+        self._check(lib().bfsx_sched_record(self._h, _ptr(roots), len(roots), repeats, epsilon,
+                                            seed, _ptr(X), _ptr(T), _ptr(M), max_rows,
+                                            C.byref(n)))
+        k = n.value
+        return X[:k].copy(), T[:k].copy(), M[:k].copy()
+
     @property
     def params(self):
         p = Params()
@@ -275,6 +330,23 @@ class Engine:
         for k, v in kw.items():
             setattr(p, k, int(v))
         self._check(lib().bfsx_set_params(self._h, C.byref(p)))
+
+
+SCHED_PARAMS = 1476
+SCHED_FILE = os.path.join(HERE, "sched_mlp.json")
+
+
+def have_scheduler_weights(path=None):
+    return os.path.exists(path or SCHED_FILE)
+
+
+def load_scheduler_file(path=None):
+    import json
+    with open(path or SCHED_FILE) as f:
+        meta = json.load(f)
+    if len(meta["weights"]) != SCHED_PARAMS:
+        raise ValueError("scheduler file holds the wrong number of parameters")
+    return meta
 
 
 def harmonic_gteps(edges, ms):

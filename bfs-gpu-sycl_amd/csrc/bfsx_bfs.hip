@@ -370,6 +370,13 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
 // ---------------------------------------------------------------------------
 // PULL (bottom-up)
 // ---------------------------------------------------------------------------
+// Candidates with more neighbours than this are scanned by the whole wave.
+constexpr uint32_t PULL_WAVE_DEG = 64;
+
+__device__ __forceinline__ bool in_frontier(const uint32_t *fbm, uint32_t w) {
+  return (fbm[w >> 5] >> (w & 31)) & 1u;
+}
+
 template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
                                              uint32_t *fbm_next, uint32_t n_groups) {
@@ -382,17 +389,56 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
     const bool cand = u < a.n_rows && !((vword >> (u & 31)) & 1u);
     bool found = false;
     uint32_t par = 0;
-    unsigned long long deg = 0;
+    PtrT b = 0, e = 0;
     if (cand) {
-      const PtrT b = col_ptr[u], e = col_ptr[u + 1];
-      deg = (unsigned long long)(e - b);
-      for (PtrT k = b; k < e; k++) {
-        const uint32_t w = a.row_idx[k];
-        if ((fbm_cur[w >> 5] >> (w & 31)) & 1u) {
+      b = col_ptr[u];
+      e = col_ptr[u + 1];
+    }
+    const unsigned long long deg = (unsigned long long)(e - b);
+    // small lists: each lane scans its own, four independent loads per round
+    // (indices past the end repeat the last neighbour, which is harmless)
+    if (cand && deg <= PULL_WAVE_DEG) {
+      for (PtrT k = b; k < e; k += 4) {
+        const PtrT last = e - 1;
+        const uint32_t w0 = a.row_idx[k];
+        const uint32_t w1 = a.row_idx[k + 1 < e ? k + 1 : last];
+        const uint32_t w2 = a.row_idx[k + 2 < e ? k + 2 : last];
+        const uint32_t w3 = a.row_idx[k + 3 < e ? k + 3 : last];
+        const bool h0 = in_frontier(fbm_cur, w0), h1 = in_frontier(fbm_cur, w1);
+        const bool h2 = in_frontier(fbm_cur, w2), h3 = in_frontier(fbm_cur, w3);
+        if (h0 | h1 | h2 | h3) {
           found = true;
-          par = w;
+          par = h0 ? w0 : h1 ? w1 : h2 ? w2 : w3;
           break;
         }
+      }
+    }
+    // long lists: the wave scans one list at a time, 64 neighbours per round
+    unsigned long long big = __ballot(cand && deg > PULL_WAVE_DEG);
+    while (big) {
+      const int src = __ffsll(big) - 1;
+      big &= big - 1;
+      const PtrT bb = __shfl(b, src, 64), ee = __shfl(e, src, 64);
+      bool hit = false;
+      uint32_t hp = 0;
+      for (PtrT base = bb; base < ee; base += 64) {
+        const PtrT k = base + lane;
+        uint32_t w = 0;
+        bool h = false;
+        if (k < ee) {
+          w = a.row_idx[k];
+          h = in_frontier(fbm_cur, w);
+        }
+        const unsigned long long hm = __ballot(h);
+        if (hm) {
+          hp = __shfl(w, __ffsll(hm) - 1, 64);
+          hit = true;
+          break;
+        }
+      }
+      if ((int)lane == src) {
+        found = hit;
+        par = hp;
       }
     }
     const unsigned long long m = __ballot(found);

@@ -6,6 +6,7 @@
 // count, :713-715).  Device memory for the graph and all search state is
 // allocated once per graph and reused across runs.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -122,43 +123,26 @@ static bool mark_step(const bfsx_ctx *ctx, uint32_t F) {
   return (double)F * avg_deg * 1024.0 >= (double)ctx->n_cols * r;
 }
 
-static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
-                     bool want_parents) {
-  if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
-  if (root >= ctx->n_cols) fail(BFSX_ERR_ARG, "root out of range");
-  if (strategy < BFSX_LLB || strategy > BFSX_PULL) fail(BFSX_ERR_ARG, "unknown strategy");
-  const bool symmetric = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols;
-  if (strategy == BFSX_PULL && !symmetric)
-    fail(BFSX_ERR_UNSUPPORTED, "bottom-up steps need a symmetric graph with n_rows == n_cols");
-  const bool trace = flags & BFSX_RUN_TRACE;
-  const bool timed = trace || (flags & BFSX_RUN_STEP_TIMES);
-  const bool pull_possible = strategy == BFSX_PULL;
-  hipStream_t s = ctx->stream;
-  if (want_parents) ensure(ctx->parents, sizeof(uint32_t) * ctx->n_cols, s);
-  if (pull_possible) ensure_pull_state(ctx);
-  uint32_t *par = want_parents ? ctx->parents.as<uint32_t>() : nullptr;
-  // initial state outside the timer (host.cpp:113-121 / run_bfs_gpu.hpp:674-677)
-  if (ctx->mark_dirty) {
-    BFSX_HIP(hipMemsetAsync(ctx->mark.p, 0, mark_bytes(ctx->n_cols), s));
-    ctx->mark_dirty = false;
-  }
-  init_search(ctx, root, symmetric, par);
-  BFSX_HIP(hipStreamSynchronize(s));
-  ctx->trace.clear();
+static bool graph_symmetric(const bfsx_ctx *ctx) {
+  return (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols;
+}
 
-  RunOut out;
-  BFSX_HIP(hipEventRecord(ctx->ev0, s));
+// Host-side state of one level-synchronous search; the device side lives in
+// ctx (levels, vis, queues, ctl).  One call of step() runs one level.
+struct Walker {
+  bfsx_ctx *ctx;
   uint32_t F = 1, cur = 0, level = 0;
-  uint64_t f_edges = 0, visited_v = 1, visited_e = 0;
-  if (trace) {
-    read_ctl(ctx);
-    f_edges = ctx->h_ctl->q_edges[0];
-    visited_e = ctx->h_ctl->visited_edges;
-  }
-  int fbm_cur = 0;          // buffer holding the current frontier's bitmap
-  bool fbm_valid = false;   // whether it is up to date
-  while (F > 0) {
-    const int choice = strategy;
+  int fbm_cur = 0;         // buffer holding the current frontier's bitmap
+  bool fbm_valid = false;  // whether that bitmap is up to date
+  bool pull_possible = false;
+  bool track = false;      // accumulate frontier degree sums
+  bool stamp = false;      // device clock around each step
+  uint32_t *par = nullptr;
+
+  // Launches the kernels of one level with step kind `choice`, then the
+  // level bookkeeping kernel.  The caller reads the control block back and
+  // calls advance().
+  bool launch(int choice) {
     StepArgs a{};
     a.col_ptr = ctx->col_ptr.p;
     a.row_idx = ctx->row_idx.as<uint32_t>();
@@ -174,16 +158,17 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     a.parents = par;
     a.ctl = ctx->ctl.as<LevelCtl>();
     a.next_level = (int32_t)level + 1;
-    a.track_degrees = trace;
+    a.track_degrees = track;
     const bool use_mark = choice != BFSX_PULL && mark_step(ctx, F);
     a.mark = use_mark ? ctx->mark.as<uint8_t>() : nullptr;
-    if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
+    if (stamp) level_begin(ctx);
     bool new_fbm = false;
     switch (choice) {
       case BFSX_LLB: step_llb(ctx, a, false); break;
       case BFSX_SLB: step_llb(ctx, a, true); break;
       case BFSX_GLB: step_glb(ctx, a); break;
       case BFSX_PULL:
+        if (!pull_possible) fail(BFSX_ERR_UNSUPPORTED, "bottom-up step on a non-symmetric graph");
         if (!fbm_valid)
           levels_to_fbm(ctx, a.levels, (int32_t)level, ctx->fbm[fbm_cur].as<uint32_t>());
         a.fbm_cur = ctx->fbm[fbm_cur].as<uint32_t>();
@@ -191,6 +176,7 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
         step_pull(ctx, a);
         new_fbm = true;
         break;
+      default: fail(BFSX_ERR_ARG, "unknown step kind");
     }
     if (use_mark) {
       ctx->mark_dirty = true;
@@ -198,19 +184,86 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
       compact(ctx, a, a.mark, fbm_next);
       new_fbm = fbm_next != nullptr;
     }
-    if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
-    level_end(ctx, cur, cur ^ 1, false);
-    read_ctl(ctx);
-    if (use_mark) ctx->mark_dirty = false;
-    const uint32_t Fn = ctx->h_ctl->q_count[cur ^ 1];
+    level_end(ctx, cur, cur ^ 1, stamp);
+    return new_fbm;
+  }
+
+  // Host bookkeeping after the control block of the level has been read.
+  void advance(bool new_fbm) {
+    ctx->mark_dirty = false;
     if (new_fbm) fbm_cur ^= 1;
     fbm_valid = new_fbm;
+    F = ctx->h_ctl->q_count[cur ^ 1];
+    cur ^= 1;
+    level++;
+  }
+};
+
+static uint32_t allow_mask(const Walker &w) { return 0x7u | (w.pull_possible ? 0x8u : 0u); }
+
+static Walker start_search(bfsx_ctx *ctx, uint32_t root, bool pull_possible, bool track,
+                           bool stamp, uint32_t *par) {
+  if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
+  if (root >= ctx->n_cols) fail(BFSX_ERR_ARG, "root out of range");
+  hipStream_t s = ctx->stream;
+  if (pull_possible) ensure_pull_state(ctx);
+  if (ctx->mark_dirty) {
+    BFSX_HIP(hipMemsetAsync(ctx->mark.p, 0, mark_bytes(ctx->n_cols), s));
+    ctx->mark_dirty = false;
+  }
+  init_search(ctx, root, graph_symmetric(ctx), par);
+  Walker w;
+  w.ctx = ctx;
+  w.pull_possible = pull_possible;
+  w.track = track;
+  w.stamp = stamp;
+  w.par = par;
+  return w;
+}
+
+static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
+                     bool want_parents) {
+  if (strategy < BFSX_LLB || strategy > BFSX_LEARNED) fail(BFSX_ERR_ARG, "unknown strategy");
+  const bool symmetric = ctx->has_graph && graph_symmetric(ctx);
+  if (strategy == BFSX_PULL && !symmetric)
+    fail(BFSX_ERR_UNSUPPORTED, "bottom-up steps need a symmetric graph with n_rows == n_cols");
+  const bool learned = strategy == BFSX_LEARNED && ctx->sched_loaded;
+  const int fixed = strategy == BFSX_LEARNED ? BFSX_LLB : strategy;  // fallback: fixed LLB
+  const bool trace = flags & BFSX_RUN_TRACE;
+  const bool timed = trace || (flags & BFSX_RUN_STEP_TIMES);
+  hipStream_t s = ctx->stream;
+  if (want_parents) ensure(ctx->parents, sizeof(uint32_t) * ctx->n_cols, s);
+  uint32_t *par = want_parents ? ctx->parents.as<uint32_t>() : nullptr;
+  // initial state outside the timer (host.cpp:113-121 / run_bfs_gpu.hpp:674-677)
+  Walker w = start_search(ctx, root, fixed == BFSX_PULL || (learned && symmetric),
+                          trace || learned, learned, par);
+  BFSX_HIP(hipStreamSynchronize(s));
+  ctx->trace.clear();
+
+  RunOut out;
+  BFSX_HIP(hipEventRecord(ctx->ev0, s));
+  uint64_t f_edges = 0, visited_v = 1, visited_e = 0;
+  if (learned) decide(ctx, 0, true, allow_mask(w));
+  if (trace || learned) {
+    read_ctl(ctx);
+    f_edges = ctx->h_ctl->q_edges[0];
+    visited_e = ctx->h_ctl->visited_edges;
+  }
+  while (w.F > 0) {
+    const int choice = learned ? (int)ctx->h_ctl->choice : fixed;
+    const uint32_t level = w.level, F = w.F, cur = w.cur;
+    if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
+    const bool new_fbm = w.launch(choice);
+    if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
+    if (learned) decide(ctx, cur ^ 1, false, allow_mask(w));
+    read_ctl(ctx);
+    w.advance(new_fbm);
     if (timed) {
       bfsx_level_record r{};
       r.level = level;
       r.choice = (uint32_t)choice;
       r.frontier_vertices = F;
-      r.discovered = Fn;
+      r.discovered = w.F;
       if (trace) {
         r.frontier_edges = f_edges;
         r.unvisited_vertices = ctx->n_cols - visited_v;
@@ -226,9 +279,6 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     }
     out.steps[choice]++;
     out.n_levels++;
-    F = Fn;
-    cur ^= 1;
-    level++;
   }
   BFSX_HIP(hipEventRecord(ctx->ev1, s));
   BFSX_HIP(hipEventSynchronize(ctx->ev1));
@@ -236,6 +286,107 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   BFSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   out.ms = ms;
   return out;
+}
+
+// ---------------------------------------------------------------------------
+// training rows for the scheduler: time every step kind from the same state
+// ---------------------------------------------------------------------------
+static void snapshot(bfsx_ctx *ctx, bool save) {
+  hipStream_t s = ctx->stream;
+  const size_t lb = sizeof(int32_t) * ctx->n_cols;
+  const size_t vb = sizeof(uint32_t) * bitmap_words(ctx->n_cols);
+  ensure(ctx->snap_levels, lb, s);
+  ensure(ctx->snap_vis, vb, s);
+  ensure(ctx->snap_ctl, sizeof(LevelCtl), s);
+  if (save) {
+    BFSX_HIP(hipMemcpyAsync(ctx->snap_levels.p, ctx->levels.p, lb, hipMemcpyDeviceToDevice, s));
+    BFSX_HIP(hipMemcpyAsync(ctx->snap_vis.p, ctx->vis.p, vb, hipMemcpyDeviceToDevice, s));
+    BFSX_HIP(hipMemcpyAsync(ctx->snap_ctl.p, ctx->ctl.p, sizeof(LevelCtl), hipMemcpyDeviceToDevice, s));
+  } else {
+    BFSX_HIP(hipMemcpyAsync(ctx->levels.p, ctx->snap_levels.p, lb, hipMemcpyDeviceToDevice, s));
+    BFSX_HIP(hipMemcpyAsync(ctx->vis.p, ctx->snap_vis.p, vb, hipMemcpyDeviceToDevice, s));
+    BFSX_HIP(hipMemcpyAsync(ctx->ctl.p, ctx->snap_ctl.p, sizeof(LevelCtl), hipMemcpyDeviceToDevice, s));
+  }
+}
+
+static double step_us(bfsx_ctx *ctx) {
+  const LevelCtl &c = *ctx->h_ctl;
+  const double ticks = c.t_end > c.t_begin ? (double)(c.t_end - c.t_begin) : 0.0;
+  return ticks * 1000.0 / ctx->wall_clock_khz;
+}
+
+static uint32_t record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, uint32_t repeats,
+                       float epsilon, uint64_t seed, float *X, float *T, float *M,
+                       uint32_t max_rows) {
+  if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
+  if (repeats < 1) fail(BFSX_ERR_ARG, "repeats must be >= 1");
+  const bool symmetric = graph_symmetric(ctx);
+  // scheduler inputs come from the same kernel as at inference; the
+  // network's outputs are not used here
+  ensure(ctx->sched_w, sizeof(float) * BFSX_SCHED_PARAMS, ctx->stream);
+  if (!ctx->sched_loaded)
+    BFSX_HIP(hipMemsetAsync(ctx->sched_w.p, 0, sizeof(float) * BFSX_SCHED_PARAMS, ctx->stream));
+  uint64_t rng = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto next_u01 = [&]() {  // SplitMix64
+    uint64_t z = (rng += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) / 9007199254740992.0;
+  };
+  uint32_t rows = 0;
+  for (uint32_t ri = 0; ri < n_roots && rows < max_rows; ri++) {
+    Walker w = start_search(ctx, roots[ri], symmetric, true, true, nullptr);
+    const uint32_t allow = allow_mask(w);
+    decide(ctx, 0, true, allow);
+    read_ctl(ctx);
+    while (w.F > 0 && rows < max_rows) {
+      for (int k = 0; k < 8; k++) X[(size_t)rows * 8 + k] = ctx->h_ctl->sched_in[k];
+      snapshot(ctx, true);
+      const Walker saved = w;
+      double best_us = 1e300;
+      int best = BFSX_LLB;
+      for (int kind = 0; kind < BFSX_N_STEP_KINDS; kind++) {
+        const bool ok = (allow >> kind) & 1u;
+        M[(size_t)rows * 4 + kind] = ok ? 1.f : 0.f;
+        T[(size_t)rows * 4 + kind] = 0.f;
+        if (!ok) continue;
+        double t = 1e300;
+        for (uint32_t rep = 0; rep < repeats; rep++) {
+          snapshot(ctx, false);
+          w = saved;
+          w.launch(kind);
+          read_ctl(ctx);
+          t = std::min(t, step_us(ctx));
+        }
+        T[(size_t)rows * 4 + kind] = (float)std::log2(std::max(t, 0.01));
+        if (t < best_us) {
+          best_us = t;
+          best = kind;
+        }
+      }
+      int take = best;
+      if (epsilon > 0 && next_u01() < epsilon) {
+        int opts[4], n = 0;
+        for (int kind = 0; kind < 4; kind++)
+          if ((allow >> kind) & 1u) opts[n++] = kind;
+        take = opts[std::min(n - 1, (int)(next_u01() * n))];
+      }
+      // run the chosen kind for real; its device time feeds the next inputs
+      snapshot(ctx, false);
+      w = saved;
+      const uint32_t cur = w.cur;
+      BFSX_HIP(hipMemcpyAsync(&ctx->ctl.as<LevelCtl>()->choice, &take, sizeof(uint32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+      const bool new_fbm = w.launch(take);
+      decide(ctx, cur ^ 1, false, allow);
+      read_ctl(ctx);
+      w.advance(new_fbm);
+      rows++;
+    }
+  }
+  BFSX_HIP(hipStreamSynchronize(ctx->stream));
+  return rows;
 }
 
 static void finish_run(bfsx_ctx *ctx, const RunOut &r, int32_t *levels_out,
@@ -359,6 +510,8 @@ void bfsx_destroy(bfsx_ctx *ctx) {
   release(ctx->iso_bm);
   release(ctx->ctl);
   release(ctx->scratch);
+  for (DevBuf *b : {&ctx->sched_w, &ctx->snap_levels, &ctx->snap_vis, &ctx->snap_ctl})
+    release(*b);
   if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
   for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_step0, ctx->ev_step1})
     if (e) (void)hipEventDestroy(e);
@@ -655,6 +808,63 @@ int bfsx_stream_copy(bfsx_ctx *ctx, uint64_t bytes, int repeats, double *ms) {
     if (ms) *ms = t / repeats;
     release(a);
     release(b);
+  });
+}
+
+int bfsx_sched_set_weights(bfsx_ctx *ctx, const float *w, uint32_t n) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (!w) {
+      ctx->sched_loaded = false;
+      return;
+    }
+    if (n != BFSX_SCHED_PARAMS) fail(BFSX_ERR_ARG, "scheduler needs 1476 parameters");
+    for (uint32_t i = 0; i < n; i++)
+      if (!std::isfinite(w[i])) fail(BFSX_ERR_ARG, "scheduler weight is not finite");
+    ensure(ctx->sched_w, sizeof(float) * n, ctx->stream);
+    BFSX_HIP(hipMemcpyAsync(ctx->sched_w.p, w, sizeof(float) * n, hipMemcpyHostToDevice,
+                            ctx->stream));
+    BFSX_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->sched_loaded = true;
+  });
+}
+
+int bfsx_mlp_forward(bfsx_ctx *ctx, const float *X, uint32_t n, const float *W, float *Y) {
+  if (!ctx || !X || !W || !Y || n == 0) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    mlp_forward(ctx, X, n, W, Y);
+  });
+}
+
+int bfsx_mlp_loss_grad(bfsx_ctx *ctx, const float *X, const float *T, const float *M, uint32_t n,
+                       const float *W, float *grad, float *loss) {
+  if (!ctx || !X || !T || !M || !W || !grad || !loss || n == 0) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    mlp_loss_grad(ctx, X, T, M, n, W, grad, loss);
+  });
+}
+
+int bfsx_mlp_train(bfsx_ctx *ctx, const float *X, const float *T, const float *M, uint32_t n,
+                   float *W, uint32_t epochs, float lr, float *loss_curve) {
+  if (!ctx || !X || !T || !M || !W || n == 0) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (!(lr > 0) || epochs > 1000000) fail(BFSX_ERR_ARG, "bad learning rate or epoch count");
+    mlp_train(ctx, X, T, M, n, W, epochs, lr, loss_curve);
+  });
+}
+
+int bfsx_sched_record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, uint32_t repeats,
+                      float epsilon, uint64_t seed, float *X, float *T, float *M,
+                      uint32_t max_rows, uint32_t *n_rows) {
+  if (!ctx || (n_roots && !roots) || !X || !T || !M) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    const uint32_t r = record(ctx, roots, n_roots, repeats, epsilon, seed, X, T, M, max_rows);
+    if (n_rows) *n_rows = r;
   });
 }
 

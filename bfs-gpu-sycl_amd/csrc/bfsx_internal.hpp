@@ -72,6 +72,11 @@ struct bfsx_ctx {
   bfsx::LevelCtl *h_ctl = nullptr;  // pinned mirror
   bool mark_dirty = false;          // a mark step did not reach its compaction
 
+  // learned scheduler weights and the snapshot used to time alternatives
+  bfsx::DevBuf sched_w;
+  bool sched_loaded = false;
+  bfsx::DevBuf snap_levels, snap_vis, snap_ctl;
+
   bfsx_params params{};
   std::vector<bfsx_level_record> trace;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_step0 = nullptr, ev_step1 = nullptr;
@@ -142,5 +147,14 @@ void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out);
 void level_histogram(bfsx_ctx *ctx, uint32_t max_levels, unsigned long long *d_counts,
                      unsigned long long *d_edges);
 void stream_copy(bfsx_ctx *ctx, const void *src, void *dst, uint64_t bytes);
+
+// ---- learned scheduler (bfsx_sched.hip) -----------------------------------
+// Features of the frontier in queue `idx` -> MLP -> ctl->choice.
+void decide(bfsx_ctx *ctx, uint32_t idx, bool first, uint32_t allow_mask);
+void mlp_forward(bfsx_ctx *ctx, const float *X, uint32_t n, const float *W, float *Y);
+void mlp_loss_grad(bfsx_ctx *ctx, const float *X, const float *T, const float *M, uint32_t n,
+                   const float *W, float *grad, float *loss);
+void mlp_train(bfsx_ctx *ctx, const float *X, const float *T, const float *M, uint32_t n,
+               float *W, uint32_t epochs, float lr, float *loss_curve);
 
 }  // namespace bfsx

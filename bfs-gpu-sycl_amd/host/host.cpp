@@ -1,0 +1,321 @@
+// bfs_1.gpu: the reference's command line (src/host.cpp:57-276) on the MI355X
+// engine.
+//
+//   bfs_1.gpu --dataset=<name> --root=<v> --num_runs=<n> --output=<file.json>
+//
+// reads $PWD/dataset/<name>-csc-<P>/ (graph_load.hpp), runs GPURun, checks
+// the levels, prints the level histogram and merges the same JSON record as
+// the reference under "<dataset>_<num_runs>_<strategy>" -> "1".
+//
+// Additions (all optional): --strategy local|global|local_strided|pull|learned
+// (the reference fixes it at compile time), --partitions P (load a
+// P-partition dataset onto the one GPU), --generate rmat:S:E[:seed] or
+// grid:SIDE[:seed] (build a graph in HBM instead of loading one),
+// --write_dataset NAME (store the graph in the reference's binary layout),
+// --sched FILE (scheduler weights), --device N.
+//
+// Validation: the reference re-runs the search on the CPU and compares
+// (host.cpp:172-188).  Here the levels are checked against the BFS
+// certificate in validate.hpp (exactly equivalent: it holds iff the levels
+// are the hop distances), and every run must equal the last one.
+#include <algorithm>
+#include <climits>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include <unistd.h>
+
+#include "bfsx.h"
+#include "bfsx_json.hpp"
+#include "functions.hpp"
+#include "graph_load.hpp"
+#include "run_bfs_gpu.hpp"
+#include "validate.hpp"
+
+#ifndef NUM_GPU
+#define NUM_GPU 1
+#endif
+
+using bfsx::Json;
+
+namespace {
+
+const char *strategy_key(int s) {
+  switch (s) {
+    case BFSX_GLB: return "global";
+    case BFSX_SLB: return "local_strided";
+    case BFSX_PULL: return "pull";
+    case BFSX_LEARNED: return "learned";
+    default: return "local";
+  }
+}
+
+int strategy_from(const std::string &s) {
+  if (s == "local" || s == "llb") return BFSX_LLB;
+  if (s == "global" || s == "glb") return BFSX_GLB;
+  if (s == "local_strided" || s == "slb") return BFSX_SLB;
+  if (s == "pull") return BFSX_PULL;
+  if (s == "learned") return BFSX_LEARNED;
+  throw std::runtime_error("unknown --strategy " + s);
+}
+
+std::string exe_dir() {
+  char buf[PATH_MAX];
+  const ssize_t n = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+  if (n <= 0) return ".";
+  std::string p(buf, (size_t)n);
+  return p.substr(0, p.find_last_of('/'));
+}
+
+template <class PtrT>
+bool sorted_symmetric(const std::vector<PtrT> &cp, const std::vector<uint32_t> &ri, uint32_t n_rows) {
+  const size_t n = cp.size() - 1;
+  if (n_rows != n) return false;
+  for (size_t u = 0; u < n; u++)
+    for (uint64_t e = cp[u] + 1; e < (uint64_t)cp[u + 1]; e++)
+      if (ri[e - 1] > ri[e]) return false;
+  for (size_t u = 0; u < n; u++)
+    for (uint64_t e = cp[u]; e < (uint64_t)cp[u + 1]; e++) {
+      const uint32_t v = ri[e];
+      if (!std::binary_search(ri.begin() + (ptrdiff_t)cp[v], ri.begin() + (ptrdiff_t)cp[v + 1],
+                              (uint32_t)u))
+        return false;
+    }
+  return true;
+}
+
+std::vector<float> load_weights(const std::string &path) {
+  std::ifstream f(path);
+  if (!f) return {};
+  Json j = Json::parse(f);
+  std::vector<float> w;
+  for (const Json &x : j.at("weights").elements()) w.push_back((float)x.as_double());
+  return w;
+}
+
+struct Loaded {
+  uint32_t numRows = 0, numCols = 0;
+  uint64_t numEdges = 0;
+  std::vector<uint64_t> col_ptr;  // widened copy for validation / writing
+  std::vector<uint32_t> row_idx;
+  bfsx::CSRGraph graph;
+  bool from_dataset = false;
+};
+
+Loaded generate(const std::string &spec, int device) {
+  std::vector<std::string> f;
+  std::stringstream ss(spec);
+  for (std::string t; std::getline(ss, t, ':');) f.push_back(t);
+  bfsx_ctx *ctx = nullptr;
+  bfsx::bfsx_check(nullptr, bfsx_create(device, &ctx), "bfsx_create");
+  bfsx::CtxGuard g{ctx};
+  uint64_t nnz = 0;
+  if (f.size() >= 3 && f[0] == "rmat") {
+    const int S = std::stoi(f[1]), E = std::stoi(f[2]);
+    const uint64_t seed = f.size() > 3 ? std::stoull(f[3]) : 1000;
+    bfsx::bfsx_check(ctx, bfsx_generate_rmat(ctx, S, E, 0.57, 0.19, 0.19, seed, seed + 1000,
+                                             S >= 26 ? 8 : 4, &nnz),
+                     "bfsx_generate_rmat");
+  } else if (f.size() >= 2 && f[0] == "grid") {
+    const uint32_t side = (uint32_t)std::stoul(f[1]);
+    const uint64_t seed = f.size() > 2 ? std::stoull(f[2]) : 1003;
+    const uint32_t n_long = (uint32_t)std::llround(1e-4 * 2.0 * side * (side - 1));
+    bfsx::bfsx_check(ctx, bfsx_generate_grid(ctx, side, 0.05, n_long, seed, 4, &nnz),
+                     "bfsx_generate_grid");
+  } else {
+    throw std::runtime_error("--generate expects rmat:S:E[:seed] or grid:SIDE[:seed]");
+  }
+  bfsx_graph_info info{};
+  bfsx::bfsx_check(ctx, bfsx_graph_info_get(ctx, &info), "bfsx_graph_info_get");
+  Loaded L;
+  L.numRows = info.n_rows;
+  L.numCols = info.n_cols;
+  L.numEdges = info.nnz;
+  L.row_idx.resize(info.nnz);
+  if (info.col_ptr_bytes == 8) {
+    L.col_ptr.resize((size_t)info.n_cols + 1);
+    bfsx::bfsx_check(ctx, bfsx_download_csc(ctx, L.col_ptr.data(), L.row_idx.data()), "download");
+  } else {
+    std::vector<uint32_t> cp((size_t)info.n_cols + 1);
+    bfsx::bfsx_check(ctx, bfsx_download_csc(ctx, cp.data(), L.row_idx.data()), "download");
+    L.col_ptr.assign(cp.begin(), cp.end());
+  }
+  return L;
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+  try {
+    Json newJsonObj = Json::object();
+    bfsx::CommandLineParser parser;
+    parser.addArgument("num_runs", "1");
+    parser.addArgument("dataset", "default");
+    parser.addArgument("root", "1");
+    parser.addArgument("output", "data.json");
+    parser.addArgument("strategy", strategy_key(bfsx::gpurun_config().strategy));
+    parser.addArgument("partitions", std::to_string(NUM_GPU));
+    parser.addArgument("generate", "");
+    parser.addArgument("write_dataset", "");
+    parser.addArgument("sched", exe_dir() + "/sched_mlp.json");
+    parser.addArgument("device", "0");
+    parser.parseArguments(argc, argv);
+
+    const int num_runs = std::stoi(parser.getArgument("num_runs"));
+    const std::string datasetName = parser.getArgument("dataset");
+    const long long start_ll = std::stoll(parser.getArgument("root"));
+    const std::string output_json_name = parser.getArgument("output");
+    const int strategy = strategy_from(parser.getArgument("strategy"));
+    const uint32_t P = (uint32_t)std::stoul(parser.getArgument("partitions"));
+    bfsx::GPURunConfig &cfg = bfsx::gpurun_config();
+    cfg.strategy = strategy;
+    cfg.device = std::stoi(parser.getArgument("device"));
+    parser.printArguments();
+    std::cout << std::setw(20) << std::left << "- num_gpu" << std::setw(20) << NUM_GPU << std::endl;
+    if (num_runs < 1) throw std::runtime_error("--num_runs must be >= 1");
+
+    Loaded L;
+    const std::string gen = parser.getArgument("generate");
+    if (!gen.empty()) {
+      L = generate(gen, cfg.device);
+    } else {
+      L.graph = bfsx::loadMatrix(P, datasetName);
+      L.from_dataset = true;
+      L.numCols = L.graph.meta32[1];
+      for (size_t i = 0; i < L.graph.meta32.size(); i += 3) L.numRows += L.graph.meta32[i];
+      for (uint64_t x : L.graph.meta64) L.numEdges += x;
+      if (L.graph.isMultiDimensional) {
+        bfsx::concat_partitions(L.graph.indptrMulti, L.graph.indsMulti, L.col_ptr, L.row_idx);
+      } else if (L.graph.wideOffsets) {
+        L.col_ptr = L.graph.indptr64;
+        L.row_idx = L.graph.inds;
+      } else {
+        L.col_ptr.assign(L.graph.indptr.begin(), L.graph.indptr.end());
+        L.row_idx = L.graph.inds;
+      }
+    }
+    std::cout << std::setw(20) << std::left << "- # vertices" << std::setw(20) << L.numCols << std::endl;
+    std::cout << std::setw(20) << std::left << "- # edges" << std::setw(20) << L.numEdges << std::endl;
+    std::cout << std::setw(20) << std::left << "- dataset" << std::setw(20) << "Load [OK]" << std::endl;
+    std::cout << std::setw(20) << std::left << "- load balance" << std::setw(20)
+              << std::string("[") + strategy_key(strategy) + "]" << std::endl;
+    std::cout << "----------------------------------------" << std::endl;
+    if (L.numRows > L.numCols) throw std::runtime_error("dataset has more rows than columns");
+    if (start_ll < 0 || start_ll >= (long long)L.numCols)
+      throw std::runtime_error("--root out of range");
+    const int start_vertex = (int)start_ll;
+
+    const std::string wd = parser.getArgument("write_dataset");
+    if (!wd.empty()) {
+      const char *pwd = std::getenv("PWD");
+      auto br = bfsx::write_dataset(pwd ? pwd : ".", wd, P, L.col_ptr, L.row_idx);
+      std::cout << "- wrote dataset " << wd << " with " << P << " partition(s)" << std::endl;
+    }
+
+    cfg.symmetric = sorted_symmetric(L.col_ptr, L.row_idx, L.numRows);
+    if (strategy == BFSX_LEARNED) {
+      cfg.sched_weights = load_weights(parser.getArgument("sched"));
+      if (cfg.sched_weights.empty())
+        std::cout << "- scheduler weights not found: learned runs as fixed LLB" << std::endl;
+    }
+
+    std::vector<int> h_dist(L.numCols, -1);
+    h_dist[(size_t)start_vertex] = 0;
+    std::vector<std::vector<int>> h_distancesGPU((size_t)num_runs, h_dist);
+    std::vector<uint8_t> h_updating_graph_mask(L.numCols, 0);
+    std::vector<uint8_t> h_graph_visited(L.numCols, 0);
+    h_graph_visited[(size_t)start_vertex] = 1;
+    std::vector<uint32_t> h_visit_offsets(P + 1, 0);
+    if (L.from_dataset)
+      for (uint32_t k = 0; k < P; k++) h_visit_offsets[k + 1] = h_visit_offsets[k] + L.graph.meta32[3 * k];
+    else
+      h_visit_offsets.back() = L.numRows;
+
+    if (L.from_dataset && L.graph.isMultiDimensional) {
+      bfsx::GPURun(L.numRows, L.graph.indsMulti, L.graph.indptrMulti, h_updating_graph_mask,
+                   h_graph_visited, h_distancesGPU, start_vertex, num_runs, newJsonObj,
+                   h_visit_offsets);
+    } else if (L.from_dataset && !L.graph.wideOffsets) {
+      bfsx::GPURun(L.numRows, L.graph.inds, L.graph.indptr, h_updating_graph_mask,
+                   h_graph_visited, h_distancesGPU, start_vertex, num_runs, newJsonObj,
+                   h_visit_offsets);
+    } else {
+      // 8-byte offsets (or a generated graph): one partition holding them all
+      std::vector<std::vector<uint32_t>> ip, ix;
+      if (L.numEdges > 0xFFFFFFFFull) throw std::runtime_error("graphs above 2^32-1 edges: use the C ABI");
+      ip.emplace_back(L.col_ptr.begin(), L.col_ptr.end());
+      ix.push_back(L.row_idx);
+      bfsx::GPURun(L.numRows, ix, ip, h_updating_graph_mask, h_graph_visited, h_distancesGPU,
+                   start_vertex, num_runs, newJsonObj, h_visit_offsets);
+    }
+
+    // validation and metrics (host.cpp:177-208)
+    const std::vector<int> &last = h_distancesGPU.back();
+    const int maxLevel = *std::max_element(h_distancesGPU[0].begin(), h_distancesGPU[0].end());
+    const int maxLevelCPU = maxLevel + 2;
+    const std::string why = bfsx::validate_levels(L.numRows, L.col_ptr.data(), L.row_idx.data(),
+                                                  (uint32_t)start_vertex, last);
+    const bool status = why.empty() && bfsx::areAllRowsEqualToLast(h_distancesGPU, maxLevelCPU);
+    const uint64_t edgesCount = bfsx::traversed_edges(L.numRows, L.col_ptr.data(), last);
+    std::cout << "----------------------------------------" << std::endl;
+    std::cout << "- GPU : ";
+    Json levels_gpu = Json::array();
+    for (int i = 0; i < maxLevelCPU - 1; i++) {
+      const uint64_t c = (uint64_t)std::count(last.begin(), last.end(), i);
+      std::cout << c << ", ";
+      levels_gpu.push_back(c);
+    }
+    if (!why.empty()) std::cout << "\n- check: " << why;
+    std::cout << "\n----------------------------------------" << std::endl;
+    std::cout << (status ? "[SUCCESS]" : "[FAILURE]") << "\tAll results on GPUs (" << num_runs
+              << " runs) checked against the BFS certificate\n";
+    std::cout << "----------------------------------------" << std::endl;
+
+    newJsonObj["edgesCount"] = edgesCount;
+    newJsonObj["valid"] = status;
+    newJsonObj["startVertex"] = start_vertex;
+    newJsonObj["numLevels"] = maxLevelCPU - 1;
+    const double e = (double)edgesCount;
+    newJsonObj["avgMTEPS"] = e / (1e6 * newJsonObj["avgExecutionTime"].as_double() * 1e-3);
+    newJsonObj["avgMTEPS90f"] = e / (1e6 * newJsonObj["avgExecutionTime90f"].as_double() * 1e-3);
+    newJsonObj["avgGTEPS90f"] =
+        bfsx::formatDouble(e / (1e9 * newJsonObj["avgExecutionTime90f"].as_double() * 1e-3));
+    newJsonObj["avgMTEPSFilter"] = e / (1e6 * newJsonObj["avgExecutionTimeFiltered"].as_double() * 1e-3);
+    newJsonObj["maxMTEPSFilter"] = e / (1e6 * newJsonObj["minExecutionTimeFiltered"].as_double() * 1e-3);
+    // host.cpp:203 divides integers before scaling (0 or 100); kept as is,
+    // with the exact percentage beside it
+    newJsonObj["edgesCoverage"] = (double)(L.numEdges ? edgesCount / L.numEdges : 0) * 100.0;
+    newJsonObj["edgesCoverageExact"] = L.numEdges ? 100.0 * e / (double)L.numEdges : 0.0;
+    newJsonObj["LevelsGPU"] = levels_gpu;
+    newJsonObj["strategy"] = strategy_key(strategy);
+
+    // read-modify-write of the output file (host.cpp:228-271)
+    Json combined = Json::object();
+    {
+      std::ifstream in(output_json_name);
+      if (in.is_open()) {
+        try {
+          in >> combined;
+          if (!combined.is_object()) combined = Json::object();
+        } catch (const std::exception &ex) {
+          std::cerr << "Error reading or parsing existing JSON data: " << ex.what() << std::endl;
+          combined = Json::object();
+        }
+      }
+    }
+    const std::string key = datasetName + "_" + std::to_string(num_runs) + "_" + strategy_key(strategy);
+    combined[key][std::to_string(NUM_GPU)] = newJsonObj;
+    std::ofstream out(output_json_name);
+    if (out.is_open()) out << combined.dump(4);
+    else std::cerr << "Error opening file for writing." << std::endl;
+    return 0;
+  } catch (const std::exception &ex) {
+    std::cerr << "[ERROR] " << ex.what() << std::endl;
+    return 1;
+  }
+}

@@ -1,0 +1,157 @@
+// Drop-in replacement for the reference's GPU entry point
+// (src/run_bfs_gpu.hpp:534-807), header-only, on top of the C ABI
+// (include/bfsx.h).  Same signature and contract as the reference:
+//
+//   template <typename vectorT> void GPURun(uint32_t vertexCount,
+//       vectorT &IndexHost, vectorT &OffsetHost,
+//       std::vector<uint8_t> &h_visit_mask, std::vector<uint8_t> &h_visit,
+//       std::vector<std::vector<int>> &h_distance, int sourceNode,
+//       const int num_runs, nlohmann::json &newJsonObj,
+//       std::vector<uint32_t> &h_visit_offsets)
+//
+// with the JSON type a template parameter, so callers using nlohmann::json
+// (the reference's host.cpp) and callers using bfsx::Json compile unchanged.
+//   * vectorT = std::vector<uint32_t> (one partition) or
+//     std::vector<std::vector<uint32_t>> (one per partition; concatenated into
+//     one CSC, because one GPU holds the whole graph).
+//   * Runs `num_runs` searches from sourceNode, writing h_distance[i] for
+//     every run, and the reference's 7 timing keys into newJsonObj
+//     (:784-790): rawExecutionTimeAll, avgExecutionTime,
+//     avgExecutionTimeFiltered, minExecutionTimeFiltered,
+//     avgExecutionTime90f, minExecutionTime90f, totalExecutionTime.
+//   * Times cover the level loop only (graph resident, levels not yet
+//     copied back), the reference's boundary (:679, :728).
+//   * The load-balancing strategy is the reference's compile-time choice
+//     (USE_GLOBAL_LOAD_BALANCE / USE_STRIDED_LOCAL_LOAD_BALANCE) unless set
+//     at run time with bfsx::gpurun_config().
+//   * Errors: the reference prints and calls std::terminate on a SYCL
+//     exception (:794-806); this shim throws std::runtime_error with the
+//     engine's message instead (the signature is noexcept(false)).
+#pragma once
+
+#include <cstdint>
+#include <iostream>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "bfsx.h"
+#include "functions.hpp"
+#include "graph_load.hpp"
+
+namespace bfsx {
+
+struct GPURunConfig {
+  int strategy =
+#if defined(USE_GLOBAL_LOAD_BALANCE) && USE_GLOBAL_LOAD_BALANCE == 1
+      BFSX_GLB;
+#elif defined(USE_STRIDED_LOCAL_LOAD_BALANCE) && USE_STRIDED_LOCAL_LOAD_BALANCE == 1
+      BFSX_SLB;
+#else
+      BFSX_LLB;
+#endif
+  int device = 0;
+  bool symmetric = false;           // allow bottom-up steps (graph is symmetric)
+  std::vector<float> sched_weights; // learned scheduler weights (BFSX_LEARNED)
+  bool have_params = false;
+  bfsx_params params{};
+  uint64_t last_edges_count = 0;    // edgesCount of the last run (device-side)
+};
+
+inline GPURunConfig &gpurun_config() {
+  static GPURunConfig c;
+  return c;
+}
+
+inline void bfsx_check(bfsx_ctx *ctx, int rc, const char *what) {
+  if (rc != BFSX_OK)
+    throw std::runtime_error(std::string(what) + ": " + bfsx_status_string(rc) + ": " +
+                             (ctx ? bfsx_last_error(ctx) : ""));
+}
+
+struct CtxGuard {
+  bfsx_ctx *ctx = nullptr;
+  ~CtxGuard() { bfsx_destroy(ctx); }
+};
+
+template <typename vectorT, typename JsonT>
+void GPURun(uint32_t vertexCount, vectorT &IndexHost, vectorT &OffsetHost,
+            std::vector<uint8_t> &h_visit_mask, std::vector<uint8_t> &h_visit,
+            std::vector<std::vector<int>> &h_distance, int sourceNode, const int num_runs,
+            JsonT &newJsonObj, std::vector<uint32_t> &h_visit_offsets) noexcept(false) {
+  (void)h_visit_mask;
+  (void)h_visit;
+  (void)h_visit_offsets;
+  GPURunConfig &cfg = gpurun_config();
+  if (num_runs < 1 || (int)h_distance.size() < num_runs)
+    throw std::runtime_error("GPURun: h_distance must hold num_runs rows");
+  const uint32_t n_cols = (uint32_t)h_distance.back().size();
+  CtxGuard g;
+  bfsx_check(nullptr, bfsx_create(cfg.device, &g.ctx), "bfsx_create");
+  if (cfg.have_params) bfsx_check(g.ctx, bfsx_set_params(g.ctx, &cfg.params), "bfsx_set_params");
+  const uint32_t flags = cfg.symmetric ? BFSX_GRAPH_SYMMETRIC : 0u;
+  if constexpr (std::is_same_v<vectorT, std::vector<uint32_t>>) {
+    if (OffsetHost.size() != (size_t)n_cols + 1)
+      throw std::runtime_error("GPURun: offsets do not match the level array length");
+    bfsx_check(g.ctx,
+               bfsx_upload_csc(g.ctx, n_cols, vertexCount, OffsetHost.data(), 4, IndexHost.data(),
+                               IndexHost.size(), flags),
+               "bfsx_upload_csc");
+  } else {
+    std::vector<uint64_t> cp;
+    std::vector<uint32_t> ri;
+    concat_partitions(OffsetHost, IndexHost, cp, ri);
+    if (cp.size() != (size_t)n_cols + 1)
+      throw std::runtime_error("GPURun: offsets do not match the level array length");
+    bfsx_check(g.ctx,
+               bfsx_upload_csc(g.ctx, n_cols, vertexCount, cp.data(), 8, ri.data(), ri.size(), flags),
+               "bfsx_upload_csc");
+  }
+  if (cfg.strategy == BFSX_LEARNED && !cfg.sched_weights.empty())
+    bfsx_check(g.ctx,
+               bfsx_sched_set_weights(g.ctx, cfg.sched_weights.data(),
+                                      (uint32_t)cfg.sched_weights.size()),
+               "bfsx_sched_set_weights");
+
+  std::vector<double> run_times((size_t)num_runs, 0.0);
+  for (int i = 0; i < num_runs; i++) {
+    bfsx_run_stats st{};
+    bfsx_check(g.ctx,
+               bfsx_run(g.ctx, (uint32_t)sourceNode, cfg.strategy, 0, h_distance[(size_t)i].data(),
+                        nullptr, &st),
+               "bfsx_run");
+    run_times[(size_t)i] = st.ms;
+    cfg.last_edges_count = st.edges_count;
+  }
+
+  const double threshold = 1.5;  // run_bfs_gpu.hpp:757
+  const std::vector<double> filtered = removeAnomalies(run_times, threshold);
+  const std::vector<double> filtered90 = removeAnomaliesnofilter(run_times);
+  auto mean = [](const std::vector<double> &v) {
+    return v.empty() ? 0.0 : std::accumulate(v.begin(), v.end(), 0.0) / v.size();
+  };
+  auto minv = [](const std::vector<double> &v) {
+    double m = v.empty() ? 0.0 : v[0];
+    for (double x : v) m = x < m ? x : m;
+    return m;
+  };
+  const double total = std::accumulate(run_times.begin(), run_times.end(), 0.0);
+  std::cout << "\n----------------------------------------" << std::endl;
+  printRow("Average (filtered) Time:", formatDouble(mean(filtered)) + " (ms)");
+  printRow("Minimum (filtered) Time:", formatDouble(minv(filtered)) + " (ms)");
+  printRow("Average (90%) Time:", formatDouble(mean(filtered90)) + " (ms)");
+  printRow("Minimum (90%) Time:", formatDouble(minv(filtered90)) + " (ms)");
+  printRow("Average Execution Time:", formatDouble(total / num_runs) + " (ms)");
+  printRow("Total Execution Time:", formatDouble(total) + " (ms)");
+  newJsonObj["rawExecutionTimeAll"] = run_times;
+  newJsonObj["avgExecutionTime"] = total / num_runs;
+  newJsonObj["avgExecutionTimeFiltered"] = mean(filtered);
+  newJsonObj["minExecutionTimeFiltered"] = minv(filtered);
+  newJsonObj["avgExecutionTime90f"] = mean(filtered90);
+  newJsonObj["minExecutionTime90f"] = minv(filtered90);
+  newJsonObj["totalExecutionTime"] = total;
+}
+
+}  // namespace bfsx

@@ -180,6 +180,36 @@ int bfsx_level_histogram(bfsx_ctx *ctx, uint64_t *counts, uint64_t *edges,
  * its duration.  Used to calibrate HBM counters and bandwidth. */
 int bfsx_stream_copy(bfsx_ctx *ctx, uint64_t bytes, int repeats, double *ms);
 
+/* ---- learned per-level scheduler (no counterpart in the reference) ------
+ * An FP32 MLP 8 -> 32 -> 32 -> 4 (ReLU) predicts log2(device microseconds)
+ * of each step kind (LLB, GLB, SLB, PULL) from frontier statistics; with
+ * strategy BFSX_LEARNED each level runs the kind with the smallest
+ * prediction (PULL only on symmetric graphs).  Without weights, or if the
+ * network returns NaN, BFSX_LEARNED runs fixed LLB.  Parameters are 1476
+ * floats: W1[32][8], b1[32], W2[32][32], b2[32], W3[4][32], b3[4]. */
+#define BFSX_SCHED_PARAMS 1476
+/* Loads scheduler weights (n == BFSX_SCHED_PARAMS); w == NULL unloads. */
+int bfsx_sched_set_weights(bfsx_ctx *ctx, const float *w, uint32_t n);
+/* Batch forward on the GPU: X is n x 8, Y is n x 4 (row-major). */
+int bfsx_mlp_forward(bfsx_ctx *ctx, const float *X, uint32_t n, const float *W, float *Y);
+/* Loss sum_ij M_ij (Y_ij - T_ij)^2 / sum M and its gradient (1476 floats)
+ * computed by the GPU forward/backward kernels. */
+int bfsx_mlp_loss_grad(bfsx_ctx *ctx, const float *X, const float *T, const float *M,
+                       uint32_t n, const float *W, float *grad, float *loss);
+/* Full-batch Adam on the GPU for `epochs` steps; W is updated in place;
+ * loss_curve (optional) receives the loss before each step. */
+int bfsx_mlp_train(bfsx_ctx *ctx, const float *X, const float *T, const float *M, uint32_t n,
+                   float *W, uint32_t epochs, float lr, float *loss_curve);
+/* Records training rows on the resident graph: for every level of a search
+ * from each root, the 8 scheduler inputs (X, n x 8) and, for every step kind,
+ * log2 of its device time from the same saved state (T, n x 4; M marks the
+ * kinds that were allowed).  The search continues with the fastest kind, or
+ * with a random one with probability epsilon.  Each kind is timed `repeats`
+ * times (minimum kept). */
+int bfsx_sched_record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots,
+                      uint32_t repeats, float epsilon, uint64_t seed, float *X, float *T,
+                      float *M, uint32_t max_rows, uint32_t *n_rows);
+
 /* ---- launch shape ------------------------------------------------------- */
 int bfsx_get_params(bfsx_ctx *ctx, bfsx_params *out);
 int bfsx_set_params(bfsx_ctx *ctx, const bfsx_params *in);

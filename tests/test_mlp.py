@@ -1,0 +1,173 @@
+"""Scheduler MLP: the numpy oracle is pinned by finite differences and closed
+forms; the GPU forward/backward/Adam kernels are checked against it; learned
+searches stay bit-exact whatever the network chooses."""
+import numpy as np
+import pytest
+
+from oracle import cpu
+from oracle import mlp_ref
+
+TOL = dict(rtol=2e-4, atol=2e-5)  # fp32 GPU vs float64 oracle
+
+
+def random_batch(seed, n=300, mask_frac=0.8):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(0, 1, (n, 8)).astype(np.float32)
+    T = rng.normal(5, 2, (n, 4)).astype(np.float32)
+    M = (rng.uniform(0, 1, (n, 4)) < mask_frac).astype(np.float32)
+    M[0] = 1
+    W = mlp_ref.init_weights(seed)
+    W[mlp_ref.NPARAM - 4:] = rng.normal(0, 1, 4)  # nonzero output bias
+    return X, T, M, W
+
+
+# ---------------------------------------------------------------------------
+# oracle pins (CPU)
+# ---------------------------------------------------------------------------
+def test_param_count():
+    assert mlp_ref.NPARAM == 1476
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_oracle_gradient_matches_finite_differences(seed):
+    X, T, M, W = random_batch(seed, n=40)
+    W = W.astype(np.float64)
+    loss, g = mlp_ref.loss_grad(W, X, T, M)
+    assert np.isclose(loss, mlp_ref.loss_only(W, X, T, M))
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(mlp_ref.NPARAM, 120, replace=False)
+    h = 1e-6
+    for i in idx:
+        wp = W.copy(); wp[i] += h
+        wm = W.copy(); wm[i] -= h
+        fd = (mlp_ref.loss_only(wp, X, T, M) - mlp_ref.loss_only(wm, X, T, M)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_oracle_closed_forms():
+    X, T, M, W = random_batch(7, n=10)
+    p = mlp_ref.unpack(W)
+    p["W3"] = np.zeros_like(p["W3"])
+    y = mlp_ref.forward(mlp_ref.pack(p), X)
+    assert np.allclose(y, p["b3"])
+    # identity-like path: x -> W1 = [I;0] -> W2 = I -> W3 picks input 2
+    q = {k: np.zeros_like(v) for k, v in p.items()}
+    q["W1"][:8, :8] = np.eye(8)
+    q["W2"][:32, :32] = np.eye(32)
+    q["W3"][1, 2] = 1.0
+    y = mlp_ref.forward(mlp_ref.pack(q), X)
+    assert np.allclose(y[:, 1], X[:, 2]) and np.allclose(y[:, [0, 2, 3]], 0)
+
+
+def test_planted_gradient_mistake_is_caught():
+    X, T, M, W = random_batch(3, n=30)
+    W = W.astype(np.float64)
+    _, g = mlp_ref.loss_grad(W, X, T, M)
+    bad = g.copy()
+    bad[300:340] *= -1  # sign error in a W2 block
+    i = 300 + int(np.argmax(np.abs(g[300:340])))
+    h = 1e-6
+    wp = W.copy(); wp[i] += h
+    wm = W.copy(); wm[i] -= h
+    fd = (mlp_ref.loss_only(wp, X, T, M) - mlp_ref.loss_only(wm, X, T, M)) / (2 * h)
+    assert abs(fd - g[i]) < 1e-5 + 1e-4 * abs(fd)
+    assert abs(fd - bad[i]) > 1e-4 * max(abs(fd), 1e-8)
+
+
+def test_feature_formulas():
+    x = mlp_ref.features(1, 255, 1023, 0, 3, 2, 0)
+    assert np.allclose(x, [1 / 32, 8 / 32, 10 / 32, 0, 3 / 64, 0, 1, 0])
+
+
+# ---------------------------------------------------------------------------
+# GPU kernels against the oracle
+# ---------------------------------------------------------------------------
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(3))
+def test_gpu_forward_matches_oracle(engine, seed):
+    X, _, _, W = random_batch(seed, n=1000)
+    Y = engine.mlp_forward(X, W)
+    np.testing.assert_allclose(Y, mlp_ref.forward(W, X), **TOL)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(3))
+def test_gpu_gradient_matches_oracle(engine, seed):
+    X, T, M, W = random_batch(seed, n=777)
+    loss, g = engine.mlp_loss_grad(X, T, M, W)
+    lo, go = mlp_ref.loss_grad(W, X, T, M)
+    assert np.isclose(loss, lo, rtol=1e-4)
+    np.testing.assert_allclose(g, go, rtol=1e-3, atol=1e-5 * np.abs(go).max())
+
+
+@pytest.mark.gpu
+def test_gpu_training_fits_a_teacher_and_is_deterministic(engine):
+    rng = np.random.default_rng(0)
+    X = rng.uniform(0, 1, (2000, 8)).astype(np.float32)
+    teacher = mlp_ref.init_weights(99)
+    T = mlp_ref.forward(teacher, X).astype(np.float32)
+    M = np.ones_like(T)
+    W0 = mlp_ref.init_weights(5)
+    W1, curve = engine.mlp_train(X, T, M, W0, 1500, 3e-3)
+    W2, curve2 = engine.mlp_train(X, T, M, W0, 1500, 3e-3)
+    assert np.array_equal(W1, W2) and np.array_equal(curve, curve2)
+    assert np.isclose(curve[0], mlp_ref.loss_only(W0, X, T, M), rtol=1e-4)
+    final = mlp_ref.loss_only(W1, X, T, M)
+    assert final < 0.05 * curve[0], (curve[0], final)
+
+
+# ---------------------------------------------------------------------------
+# learned searches
+# ---------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_learned_search_is_exact_whatever_it_chooses(engine):
+    engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 81, 82, 4)
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(8, 83)
+    seen = set()
+    for seed in range(6):
+        W = mlp_ref.init_weights(1000 + seed) * 3
+        engine.set_scheduler_weights(W)
+        for r in roots[:4]:
+            want, ec, _ = cpu.bfs_queue(cp, ri, int(r))
+            lev, st, par = engine.run(int(r), "learned", parents=True)
+            assert np.array_equal(lev, want) and st["edges_count"] == ec
+            assert cpu.check_parents(cp, ri, int(r), want, par) == 0
+            seen |= {k for k, v in st["steps"].items() if v}
+    assert len(seen) >= 2, seen
+    engine.set_scheduler_weights(None)
+    lev, st = engine.run(int(roots[0]), "learned")
+    assert st["steps"]["llb"] == st["n_levels"]  # fallback: fixed LLB
+
+
+@pytest.mark.gpu
+def test_learned_decision_follows_the_network(engine):
+    """A network whose output bias favours one kind picks it at every level."""
+    engine.generate_rmat(13, 16, 0.57, 0.19, 0.19, 91, 92, 4)
+    r = int(engine.pick_roots(1, 93)[0])
+    for kind, name in enumerate(["llb", "glb", "slb", "pull"]):
+        W = np.zeros(mlp_ref.NPARAM, np.float32)
+        W[mlp_ref.NPARAM - 4:] = 5.0
+        W[mlp_ref.NPARAM - 4 + kind] = 1.0
+        engine.set_scheduler_weights(W)
+        _, st = engine.run(r, "learned")
+        assert st["steps"][name] == st["n_levels"], (name, st["steps"])
+    engine.set_scheduler_weights(None)
+
+
+@pytest.mark.gpu
+def test_record_rows(engine):
+    engine.generate_rmat(14, 16, 0.57, 0.19, 0.19, 101, 102, 4)
+    cp, ri = engine.download_csc()
+    info = engine.info()
+    roots = engine.pick_roots(3, 103)
+    X, T, M = engine.sched_record(roots, repeats=1, epsilon=0.3, seed=1)
+    n_levels = sum(int(cpu.bfs_queue(cp, ri, int(r))[0].max()) + 1 for r in roots)
+    assert len(X) == n_levels
+    assert (M == 1).all()  # symmetric graph: all four kinds timed
+    assert np.isfinite(T).all() and (T > -7).all()
+    deg = np.diff(cp.astype(np.int64))
+    r0 = int(roots[0])
+    want = mlp_ref.features(1, deg[r0], info["n_cols"] - info["n_isolated"] - 1,
+                            info["nnz"] - deg[r0], 0, 0, 0)
+    np.testing.assert_allclose(X[0], want, rtol=1e-6, atol=1e-6)
