@@ -155,21 +155,88 @@ __device__ __forceinline__ void discover(const KArgs &a, const PtrT *col_ptr, ui
   if (a.q_out_edges) wave_track_degree(isnew, n, a.n_rows, col_ptr, a.q_out_edges);
 }
 
+// Expands the edge range [lo, hi) of a work-group whose frontier entries are
+// staged in LDS: s_scan[j] = first edge of slot j (exclusive scan; slots past
+// the valid ones hold values >= hi), s_start[j] = its first row_idx offset.
+// Each lane takes EPT consecutive edges per round: one binary search finds
+// the owner slot of the first, the others advance it; all row_idx loads,
+// then all visited-bitmap loads are issued before any store, so every lane
+// keeps EPT independent memory chains in flight.
+constexpr int EPT = 4;
+
+template <typename PtrT, int WG>
+__device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr,
+                                             const PtrT *s_scan, const PtrT *s_start,
+                                             const uint32_t *s_v, PtrT lo, PtrT hi) {
+  for (PtrT base = lo + (PtrT)threadIdx.x * EPT; base < hi; base += (PtrT)WG * EPT) {
+    uint32_t j = 0;
+#pragma unroll
+    for (uint32_t step = WG / 2; step > 0; step >>= 1)
+      if (s_scan[j + step] <= base) j += step;
+    uint32_t n[EPT], from[EPT];
+    bool valid[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; k++) {
+      const PtrT e = base + k;
+      valid[k] = e < hi;
+      while (valid[k] && j + 1 < WG && s_scan[j + 1] <= e) j++;
+      n[k] = valid[k] ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0u;
+      from[k] = s_v[j];
+    }
+    if (a.mark) {
+      uint32_t w[EPT];
+#pragma unroll
+      for (int k = 0; k < EPT; k++) w[k] = valid[k] ? a.vis[n[k] >> 5] : ~0u;
+#pragma unroll
+      for (int k = 0; k < EPT; k++)
+        if (!((w[k] >> (n[k] & 31)) & 1u)) {
+          a.mark[n[k]] = 1;
+          if (a.parents) a.parents[n[k]] = from[k];
+        }
+    } else {
+#pragma unroll
+      for (int k = 0; k < EPT; k++) {
+        const bool isnew = valid[k] && try_visit(a.vis, n[k]);
+        if (isnew) {
+          a.levels[n[k]] = a.next_level;
+          if (a.parents) a.parents[n[k]] = from[k];
+        }
+        wave_append(isnew, n[k], a.q_out, a.q_out_count);
+        if (a.q_out_edges) wave_track_degree(isnew, n[k], a.n_rows, col_ptr, a.q_out_edges);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // LLB / SLB
 // ---------------------------------------------------------------------------
-template <typename PtrT, int WG, bool STRIDED>
-__global__ __launch_bounds__(WG) void k_llb(KArgs a) {
+template <typename PtrT, int WG>
+struct LlbShared {
+  PtrT start[WG];
+  PtrT scan[WG];
+  PtrT wave[WG / 64 + 1];
+  uint32_t v[WG];
+  uint32_t hub_n;
+  uint32_t hub[WG];
+};
+
+// One LLB tile: WG frontier entries (contiguous, or strided by n_tiles for
+// SLB), block scan of their degrees, cooperative expansion of their edges.
+template <typename PtrT, int WG>
+__device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh, uint32_t tile,
+                                         uint32_t n_tiles, bool strided) {
   constexpr int NW = WG / 64;
-  __shared__ PtrT s_start[WG];
-  __shared__ PtrT s_scan[WG];
-  __shared__ PtrT s_wave[NW + 1];
-  __shared__ uint32_t s_v[WG];
-  __shared__ uint32_t s_hub_n;
-  __shared__ uint32_t s_hub[WG];
+  PtrT *s_start = sh.start;
+  PtrT *s_scan = sh.scan;
+  PtrT *s_wave = sh.wave;
+  uint32_t *s_v = sh.v;
+  uint32_t &s_hub_n = sh.hub_n;
+  uint32_t *s_hub = sh.hub;
   const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
   const uint32_t t = threadIdx.x;
-  const uint32_t idx = STRIDED ? blockIdx.x + t * gridDim.x : blockIdx.x * WG + t;
+  const uint32_t idx = strided ? tile + t * n_tiles : tile * WG + t;
+  __syncthreads();  // LDS of the previous tile is no longer read
   if (t == 0) s_hub_n = 0;
   PtrT deg = 0, st = 0;
   uint32_t v = 0;
@@ -192,16 +259,9 @@ __global__ __launch_bounds__(WG) void k_llb(KArgs a) {
   const PtrT ex = block_excl_scan<WG>(sdeg, total, s_wave);
   s_scan[t] = ex;
   __syncthreads();
-  for (PtrT i = t; i < total; i += WG) {
-    // last slot j with s_scan[j] <= i (its range holds i: empty slots tie
-    // with the next one and are skipped by taking the last)
-    uint32_t j = 0;
-#pragma unroll
-    for (uint32_t step = WG / 2; step > 0; step >>= 1)
-      if (s_scan[j + step] <= i) j += step;
-    const PtrT e = s_start[j] + (i - s_scan[j]);
-    discover<PtrT>(a, col_ptr, a.row_idx[e], s_v[j]);
-  }
+  // owner of edge i = last slot j with s_scan[j] <= i (empty slots tie with
+  // the next one and are skipped by taking the last)
+  expand_edges<PtrT, WG>(a, col_ptr, s_scan, s_start, s_v, PtrT(0), total);
   if (a.hub_degree) {
     const uint32_t nh = s_hub_n;
     const uint32_t w = t >> 6, l = t & 63;
@@ -212,6 +272,37 @@ __global__ __launch_bounds__(WG) void k_llb(KArgs a) {
       for (PtrT e = b + l; e < e1; e += 64) discover<PtrT>(a, col_ptr, a.row_idx[e], s_v[slot]);
     }
   }
+}
+
+// Host-sized launch: one tile per work-group, grid = ceil(F / WG).
+template <typename PtrT, int WG, bool STRIDED>
+__global__ __launch_bounds__(WG) void k_llb(KArgs a) {
+  __shared__ LlbShared<PtrT, WG> sh;
+  llb_tile<PtrT, WG>(a, sh, blockIdx.x, gridDim.x, STRIDED);
+}
+
+// Device-driven top-down step for captured level loops: the frontier size,
+// level, discovery mode and LLB/SLB choice come from the control block, and
+// a fixed grid walks the tiles.  Kernels after the search ended see F = 0.
+struct DevStep {
+  uint32_t cur;          // queue buffer holding the frontier
+  uint32_t mark_min_F;   // frontiers at least this large use mark + compact
+  int fixed;             // BFSX_LLB or BFSX_SLB, or -1: read ctl->choice
+};
+
+template <typename PtrT, int WG>
+__global__ __launch_bounds__(WG) void k_td_dev(KArgs a, LevelCtl *ctl, DevStep d) {
+  __shared__ LlbShared<PtrT, WG> sh;
+  const uint32_t F = ctl->q_count[d.cur];
+  if (F == 0) return;
+  const int choice = d.fixed >= 0 ? d.fixed : (int)ctl->choice;
+  if (choice != BFSX_LLB && choice != BFSX_SLB) return;
+  a.F = F;
+  a.next_level = (int32_t)ctl->level + 1;
+  if (F < d.mark_min_F) a.mark = nullptr;
+  const uint32_t n_tiles = (F + WG - 1) / WG;
+  for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
+    llb_tile<PtrT, WG>(a, sh, tile, n_tiles, choice == BFSX_SLB);
 }
 
 // ---------------------------------------------------------------------------
@@ -354,14 +445,7 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
       __syncthreads();
       const PtrT lo = e0 > s_win_lo ? e0 : s_win_lo;
       const PtrT hi = e1 < s_win_hi ? e1 : s_win_hi;
-      for (PtrT i = lo + t; i < hi; i += WG) {
-        uint32_t j = 0;
-#pragma unroll
-        for (uint32_t step = WG / 2; step > 0; step >>= 1)
-          if (s_scan[j + step] <= i) j += step;
-        const PtrT e = s_start[j] + (i - s_scan[j]);
-        discover<PtrT>(a, col_ptr, a.row_idx[e], a.parents ? s_v[j] : 0u);
-      }
+      expand_edges<PtrT, WG>(a, col_ptr, s_scan, s_start, s_v, lo, hi);
       __syncthreads();
     }
   }
@@ -479,13 +563,11 @@ __device__ __forceinline__ uint32_t nz_bytes(uint32_t w) {
 // are OR-ed into words this wave owns (no atomics), and the next frontier
 // bitmap words are written in full.  Marks are cleared where set.
 template <typename PtrT>
-__global__ __launch_bounds__(256) void k_compact(const PtrT *col_ptr, uint32_t n_rows,
-                                                 uint8_t *mark, uint32_t n_chunks,
-                                                 uint32_t *vis, int32_t *levels,
-                                                 int32_t next_level, uint32_t *q_out,
-                                                 uint32_t *q_count,
-                                                 unsigned long long *q_edges,
-                                                 uint32_t *fbm_next) {
+__device__ __forceinline__ void compact_body(const PtrT *col_ptr, uint32_t n_rows, uint8_t *mark,
+                                             uint32_t n_chunks, uint32_t *vis, int32_t *levels,
+                                             int32_t next_level, uint32_t *q_out,
+                                             uint32_t *q_count, unsigned long long *q_edges,
+                                             uint32_t *fbm_next) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
   for (uint32_t c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); c < n_chunks;
@@ -526,6 +608,33 @@ __global__ __launch_bounds__(256) void k_compact(const PtrT *col_ptr, uint32_t n
       if (lane == 0 && dsum) atomicAdd(q_edges, dsum);
     }
   }
+}
+
+template <typename PtrT>
+__global__ __launch_bounds__(256) void k_compact(const PtrT *col_ptr, uint32_t n_rows,
+                                                 uint8_t *mark, uint32_t n_chunks,
+                                                 uint32_t *vis, int32_t *levels,
+                                                 int32_t next_level, uint32_t *q_out,
+                                                 uint32_t *q_count,
+                                                 unsigned long long *q_edges,
+                                                 uint32_t *fbm_next) {
+  compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, vis, levels, next_level, q_out, q_count,
+                     q_edges, fbm_next);
+}
+
+// Compaction inside a captured level loop: runs only when the step of this
+// level marked (same rule as k_td_dev), level from the control block.
+template <typename PtrT>
+__global__ __launch_bounds__(256) void k_compact_dev(const PtrT *col_ptr, uint32_t n_rows,
+                                                     uint8_t *mark, uint32_t n_chunks,
+                                                     uint32_t *vis, int32_t *levels,
+                                                     LevelCtl *ctl, DevStep d, uint32_t *q_out,
+                                                     int track, uint32_t *fbm_next) {
+  const uint32_t F = ctl->q_count[d.cur];
+  if (F == 0 || F < d.mark_min_F) return;
+  const uint32_t out = d.cur ^ 1;
+  compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, vis, levels, (int32_t)ctl->level + 1,
+                     q_out, &ctl->q_count[out], track ? &ctl->q_edges[out] : nullptr, fbm_next);
 }
 
 // frontier bitmap of one level from the levels array (a wave owns 64 vertices)
@@ -581,9 +690,19 @@ __global__ void k_level_begin(LevelCtl *ctl) {
   if (threadIdx.x == 0) ctl->t_begin = (unsigned long long)wall_clock64();
 }
 
-__global__ void k_level_end(LevelCtl *ctl, uint32_t consumed, uint32_t produced, int stamp) {
+// kind >= 0: the step kind that ran; kind < 0: the one in ctl->choice.  A
+// level with an empty frontier (after the end, in a captured loop) changes
+// nothing but the done flag.
+__global__ void k_level_end(LevelCtl *ctl, uint32_t consumed, uint32_t produced, int stamp,
+                            int kind) {
   if (threadIdx.x == 0) {
+    if (ctl->q_count[consumed] == 0) {
+      ctl->done = 1;
+      return;
+    }
     if (stamp) ctl->t_end = (unsigned long long)wall_clock64();
+    const uint32_t k = kind >= 0 ? (uint32_t)kind : ctl->choice;
+    if (k < 4) ctl->kind_count[k]++;
     const uint32_t n = ctl->q_count[produced];
     ctl->discovered = n;
     ctl->visited_vertices += n;
@@ -852,10 +971,56 @@ void level_begin(bfsx_ctx *ctx) {
   BFSX_HIP(hipGetLastError());
 }
 
-void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp) {
+void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, int kind) {
   hipLaunchKernelGGL(k_level_end, dim3(1), dim3(64), 0, ctx->stream, ctx->ctl.as<LevelCtl>(),
-                     consumed, produced, stamp ? 1 : 0);
+                     consumed, produced, stamp ? 1 : 0, kind);
   BFSX_HIP(hipGetLastError());
+}
+
+// One device-driven top-down level (LLB or SLB, or the choice in the control
+// block when fixed < 0) for a captured loop: step, compaction, bookkeeping.
+template <typename PtrT>
+static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, uint32_t mark_min_F) {
+  KArgs a{};
+  a.col_ptr = ctx->col_ptr.p;
+  a.row_idx = ctx->row_idx.as<uint32_t>();
+  a.n_rows = ctx->n_rows;
+  a.n_cols = ctx->n_cols;
+  a.q_in = ctx->queue[cur].as<uint32_t>();
+  a.q_out = ctx->queue[cur ^ 1].as<uint32_t>();
+  LevelCtl *ctl = ctx->ctl.as<LevelCtl>();
+  a.q_out_count = &ctl->q_count[cur ^ 1];
+  a.q_out_edges = track ? &ctl->q_edges[cur ^ 1] : nullptr;
+  a.vis = ctx->vis.as<uint32_t>();
+  a.levels = ctx->levels.as<int32_t>();
+  a.parents = nullptr;
+  a.hub_degree = ctx->params.hub_degree;
+  a.mark = ctx->mark.as<uint8_t>();
+  const DevStep d{cur, mark_min_F, fixed};
+  const uint32_t wg = ctx->params.llb_wg;
+  const unsigned grid = cap_grid(((uint64_t)ctx->n_cols + wg - 1) / wg, (unsigned)ctx->cu_count * 8);
+  hipStream_t s = ctx->stream;
+  switch (wg) {
+    case 64: hipLaunchKernelGGL((k_td_dev<PtrT, 64>), dim3(grid), dim3(64), 0, s, a, ctl, d); break;
+    case 128: hipLaunchKernelGGL((k_td_dev<PtrT, 128>), dim3(grid), dim3(128), 0, s, a, ctl, d); break;
+    case 256: hipLaunchKernelGGL((k_td_dev<PtrT, 256>), dim3(grid), dim3(256), 0, s, a, ctl, d); break;
+    case 512: hipLaunchKernelGGL((k_td_dev<PtrT, 512>), dim3(grid), dim3(512), 0, s, a, ctl, d); break;
+    case 1024: hipLaunchKernelGGL((k_td_dev<PtrT, 1024>), dim3(grid), dim3(1024), 0, s, a, ctl, d); break;
+    default: fail(BFSX_ERR_ARG, "llb_wg must be 64, 128, 256, 512 or 1024");
+  }
+  const uint32_t n_chunks = (ctx->n_cols + 1023) / 1024;
+  const unsigned cgrid = cap_grid(((uint64_t)n_chunks + 3) / 4, (unsigned)ctx->cu_count * 16);
+  hipLaunchKernelGGL(k_compact_dev<PtrT>, dim3(cgrid), dim3(256), 0, s,
+                     static_cast<const PtrT *>(ctx->col_ptr.p), ctx->n_rows,
+                     ctx->mark.as<uint8_t>(), n_chunks, ctx->vis.as<uint32_t>(),
+                     ctx->levels.as<int32_t>(), ctl, d, ctx->queue[cur ^ 1].as<uint32_t>(),
+                     track ? 1 : 0, (uint32_t *)nullptr);
+  BFSX_HIP(hipGetLastError());
+}
+
+void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, uint32_t mark_min_F) {
+  if (ctx->ptr_bytes == 8) dev_level_t<uint64_t>(ctx, cur, fixed, track, mark_min_F);
+  else dev_level_t<uint32_t>(ctx, cur, fixed, track, mark_min_F);
 }
 
 void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out) {

@@ -44,7 +44,10 @@ void release(DevBuf &b) {
 
 static void set_device(bfsx_ctx *ctx) { BFSX_HIP(hipSetDevice(ctx->device)); }
 
+void drop_loop_graph(bfsx_ctx *ctx);
+
 static void free_search_state(bfsx_ctx *ctx) {
+  drop_loop_graph(ctx);  // the captured loop holds pointers into this state
   for (DevBuf *b : {&ctx->levels, &ctx->parents, &ctx->vis, &ctx->fbm[0], &ctx->fbm[1],
                     &ctx->queue[0], &ctx->queue[1], &ctx->glb_scan, &ctx->glb_start,
                     &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark})
@@ -184,7 +187,7 @@ struct Walker {
       compact(ctx, a, a.mark, fbm_next);
       new_fbm = fbm_next != nullptr;
     }
-    level_end(ctx, cur, cur ^ 1, stamp);
+    level_end(ctx, cur, cur ^ 1, stamp, choice);
     return new_fbm;
   }
 
@@ -221,6 +224,87 @@ static Walker start_search(bfsx_ctx *ctx, uint32_t root, bool pull_possible, boo
   return w;
 }
 
+// ---------------------------------------------------------------------------
+// captured level loop: K device-driven levels per hipGraph launch
+// ---------------------------------------------------------------------------
+static uint32_t mark_min_frontier(const bfsx_ctx *ctx) {
+  const uint32_t r = ctx->params.mark_ratio_q10;
+  if (r == 0) return 0;
+  if (r == 0xFFFFFFFFu) return 0xFFFFFFFFu;
+  const double avg_deg = ctx->n_rows ? (double)ctx->nnz / ctx->n_rows : 0.0;
+  if (avg_deg <= 0) return 0xFFFFFFFFu;
+  const double f = std::ceil((double)ctx->n_cols * r / (1024.0 * avg_deg));
+  return f >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)f;
+}
+
+void drop_loop_graph(bfsx_ctx *ctx) {
+  if (ctx->loop_exec) (void)hipGraphExecDestroy(ctx->loop_exec);
+  if (ctx->loop_graph) (void)hipGraphDestroy(ctx->loop_graph);
+  ctx->loop_exec = nullptr;
+  ctx->loop_graph = nullptr;
+  ctx->loop_key = 0;
+}
+
+// fixed = BFSX_LLB / BFSX_SLB, or -1 for the scheduler's choice between them
+static hipGraphExec_t loop_graph(bfsx_ctx *ctx, int fixed, bool learned) {
+  const uint32_t K = (ctx->params.graph_levels + 1) & ~1u;  // even: buffers return to 0
+  const uint32_t mmf = mark_min_frontier(ctx);
+  const uint64_t key = 1 + ((uint64_t)(fixed + 1) | ((uint64_t)learned << 3) |
+                            ((uint64_t)K << 4) | ((uint64_t)ctx->params.llb_wg << 12) |
+                            ((uint64_t)ctx->params.hub_degree << 24)) ^
+                       ((uint64_t)mmf * 0x9E3779B97F4A7C15ull);
+  if (ctx->loop_exec && ctx->loop_key == key) return ctx->loop_exec;
+  drop_loop_graph(ctx);
+  hipStream_t s = ctx->stream;
+  BFSX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    for (uint32_t k = 0; k < K; k++) {
+      const uint32_t cur = k & 1u;
+      if (learned) level_begin(ctx);
+      dev_level(ctx, cur, fixed, learned, mmf);
+      level_end(ctx, cur, cur ^ 1, learned, fixed);
+      if (learned) decide(ctx, cur ^ 1, false, (1u << BFSX_LLB) | (1u << BFSX_SLB));
+    }
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  BFSX_HIP(hipStreamEndCapture(s, &ctx->loop_graph));
+  BFSX_HIP(hipGraphInstantiate(&ctx->loop_exec, ctx->loop_graph, nullptr, nullptr, 0));
+  ctx->loop_key = key;
+  return ctx->loop_exec;
+}
+
+static RunOut search_graph_loop(bfsx_ctx *ctx, uint32_t root, int fixed, bool learned) {
+  hipStream_t s = ctx->stream;
+  start_search(ctx, root, false, learned, learned, nullptr);
+  hipGraphExec_t exec = loop_graph(ctx, learned ? -1 : (fixed < 0 ? BFSX_LLB : fixed), learned);
+  BFSX_HIP(hipStreamSynchronize(s));
+  ctx->trace.clear();
+  RunOut out;
+  BFSX_HIP(hipEventRecord(ctx->ev0, s));
+  if (learned) decide(ctx, 0, true, (1u << BFSX_LLB) | (1u << BFSX_SLB));
+  const uint64_t max_launches = (uint64_t)ctx->n_cols / 2 + 2;
+  for (uint64_t launches = 0;; launches++) {
+    if (launches > max_launches) fail(BFSX_ERR_DEVICE, "captured level loop did not terminate");
+    BFSX_HIP(hipGraphLaunch(exec, s));
+    read_ctl(ctx);
+    if (ctx->h_ctl->done) break;
+  }
+  BFSX_HIP(hipEventRecord(ctx->ev1, s));
+  BFSX_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0;
+  BFSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  out.ms = ms;
+  for (int k = 0; k < BFSX_N_STEP_KINDS; k++) {
+    out.steps[k] = ctx->h_ctl->kind_count[k];
+    out.n_levels += out.steps[k];
+  }
+  return out;
+}
+
 static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
                      bool want_parents) {
   if (strategy < BFSX_LLB || strategy > BFSX_LEARNED) fail(BFSX_ERR_ARG, "unknown strategy");
@@ -232,6 +316,13 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   const bool trace = flags & BFSX_RUN_TRACE;
   const bool timed = trace || (flags & BFSX_RUN_STEP_TIMES);
   hipStream_t s = ctx->stream;
+  if (flags & BFSX_RUN_GRAPH_LOOP) {
+    if (fixed != BFSX_LLB && fixed != BFSX_SLB)
+      fail(BFSX_ERR_UNSUPPORTED, "the captured level loop runs LLB, SLB or the learned choice between them");
+    if (timed || want_parents)
+      fail(BFSX_ERR_UNSUPPORTED, "the captured level loop has no per-level trace or parents");
+    return search_graph_loop(ctx, root, strategy == BFSX_LEARNED ? -1 : fixed, learned);
+  }
   if (want_parents) ensure(ctx->parents, sizeof(uint32_t) * ctx->n_cols, s);
   uint32_t *par = want_parents ? ctx->parents.as<uint32_t>() : nullptr;
   // initial state outside the timer (host.cpp:113-121 / run_bfs_gpu.hpp:674-677)
@@ -879,6 +970,7 @@ int bfsx_set_params(bfsx_ctx *ctx, const bfsx_params *in) {
   return guarded(ctx, [&] {
     validate_params(*in);
     ctx->params = *in;
+    drop_loop_graph(ctx);
   });
 }
 

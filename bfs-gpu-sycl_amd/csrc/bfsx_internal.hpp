@@ -31,6 +31,7 @@ struct LevelCtl {
   uint32_t done;                   // frontier became empty
   float sched_out[4];              // scheduler outputs of the last decision
   float sched_in[8];               // scheduler inputs of the last decision
+  uint32_t kind_count[4];          // levels expanded by each step kind
 };
 
 // Bitmaps hold whole 1024-vertex chunks (32 words) so that a wave can own a
@@ -80,6 +81,11 @@ struct bfsx_ctx {
   bfsx_params params{};
   std::vector<bfsx_level_record> trace;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_step0 = nullptr, ev_step1 = nullptr;
+
+  // captured level loop (BFSX_RUN_GRAPH_LOOP), rebuilt when its key changes
+  hipGraph_t loop_graph = nullptr;
+  hipGraphExec_t loop_exec = nullptr;
+  uint64_t loop_key = 0;
 };
 
 namespace bfsx {
@@ -141,7 +147,8 @@ void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level,
                    uint32_t *fbm);
 void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next);
 void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
-               bool stamp);
+               bool stamp, int kind);
+void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, uint32_t mark_min_F);
 void level_begin(bfsx_ctx *ctx);
 void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out);
 void level_histogram(bfsx_ctx *ctx, uint32_t max_levels, unsigned long long *d_counts,

@@ -271,3 +271,66 @@ def test_level_histogram_and_step_times(engine):
 def test_stream_copy(engine):
     ms = engine.stream_copy(1 << 26, 4)
     assert ms > 0
+
+
+# ---------------------------------------------------------------------------
+# captured level loop (hipGraph, device-side termination)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("K", [1, 8, 64])
+def test_graph_loop_parity(engine, K):
+    old = engine.params
+    try:
+        engine.set_params(graph_levels=K)
+        cases = []
+        engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 111, 112, 4)
+        cases.append((engine.download_csc(), engine.pick_roots(4, 113)))
+        engine.generate_grid(200, 0.05, 3, 114, 4)
+        cases.append((engine.download_csc(), engine.pick_roots(3, 115, 1)))
+        cp, ri = path_graph(2500)
+        engine.upload_csc(cp, ri, symmetric=True)
+        cases.append(((cp, ri), np.array([3, 1250], np.uint32)))
+        for (cp, ri), roots in cases:
+            engine.upload_csc(cp, ri, symmetric=True)
+            for r in roots:
+                want, ec, _ = cpu.bfs_queue(cp, ri, int(r))
+                for s in ("llb", "slb"):
+                    lev, st = engine.run(int(r), s, graph_loop=True)
+                    assert np.array_equal(lev, want), (K, s, int(r))
+                    assert st["edges_count"] == ec
+                    assert st["n_levels"] == int(want.max()) + 1 == st["steps"][s]
+    finally:
+        engine.set_params(**old)
+
+
+def test_graph_loop_learned_and_mark_modes(engine):
+    from oracle import mlp_ref
+    engine.generate_rmat(14, 16, 0.57, 0.19, 0.19, 121, 122, 4)
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(4, 123)
+    old = engine.params
+    try:
+        for ratio in (0, 64, 0xFFFFFFFF):
+            engine.set_params(mark_ratio_q10=ratio)
+            for seed in range(3):
+                engine.set_scheduler_weights(mlp_ref.init_weights(200 + seed) * 3)
+                for r in roots:
+                    want, ec, _ = cpu.bfs_queue(cp, ri, int(r))
+                    lev, st = engine.run(int(r), "learned", graph_loop=True)
+                    assert np.array_equal(lev, want) and st["edges_count"] == ec
+                    assert st["steps"]["glb"] == st["steps"]["pull"] == 0
+                    assert st["steps"]["llb"] + st["steps"]["slb"] == int(want.max()) + 1
+    finally:
+        engine.set_scheduler_weights(None)
+        engine.set_params(**old)
+
+
+def test_graph_loop_unsupported_combinations(engine):
+    engine.generate_rmat(10, 8, 0.57, 0.19, 0.19, 1, 2, 4)
+    for s in ("glb", "pull"):
+        with pytest.raises(bfsx.BfsxError) as e:
+            engine.run(0, s, graph_loop=True)
+        assert e.value.status == 7
+    with pytest.raises(bfsx.BfsxError):
+        engine.run(0, "llb", graph_loop=True, parents=True)
+    with pytest.raises(bfsx.BfsxError):
+        engine.run(0, "llb", graph_loop=True, step_times=True)
