@@ -72,6 +72,14 @@ class Params(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("edges_count", C.c_uint64), ("n_levels", C.c_uint32),
+                ("push_levels", C.c_uint32), ("pull_levels", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 P = C.c_void_p
 _lib = None
 
@@ -98,6 +106,9 @@ SIGNATURES = {
     "bfsx_last_trace": (C.c_int, [P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "bfsx_level_histogram": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(C.c_uint32)]),
     "bfsx_stream_copy": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "bfsx_run_multi": (C.c_int, [P, P, P, C.c_uint32, C.c_uint32, P, C.POINTER(BatchStats)]),
+    "bfsx_generate_rmat_union": (C.c_int, [P, C.c_uint32, P, P, P, P, C.c_int, P,
+                                           C.POINTER(C.c_uint64)]),
     "bfsx_sched_set_weights": (C.c_int, [P, P, C.c_uint32]),
     "bfsx_mlp_forward": (C.c_int, [P, P, C.c_uint32, P, P]),
     "bfsx_mlp_loss_grad": (C.c_int, [P, P, P, P, C.c_uint32, P, P, P]),
@@ -268,6 +279,30 @@ class Engine:
         ms = C.c_double(0)
         self._check(lib().bfsx_stream_copy(self._h, nbytes, repeats, C.byref(ms)))
         return ms.value
+
+    # ---- batched searches -----------------------------------------------
+    def run_multi(self, sources, bits, n_bits, levels=True):
+        """Searches bit b from every sources[i] with bits[i] == b, all in one
+        batch; returns (uint8 levels [n_bits, n_cols] or None, stats)."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        bit = np.ascontiguousarray(bits, np.uint16)
+        g = self.info()
+        lev = np.empty((n_bits, g["n_cols"]), np.uint8) if levels else None
+        st = BatchStats()
+        self._check(lib().bfsx_run_multi(self._h, _ptr(src), _ptr(bit), len(src), n_bits,
+                                         _ptr(lev), C.byref(st)))
+        return lev, st.as_dict()
+
+    def generate_rmat_union(self, scales, edge_factors, seeds_graph, seeds_perm, ptr_bytes=4):
+        sc = np.ascontiguousarray(scales, np.int32)
+        ef = np.ascontiguousarray(edge_factors, np.int32)
+        sg = np.ascontiguousarray(seeds_graph, np.uint64)
+        sp = np.ascontiguousarray(seeds_perm, np.uint64)
+        off = np.zeros(len(sc) + 1, np.uint32)
+        nnz = C.c_uint64(0)
+        self._check(lib().bfsx_generate_rmat_union(self._h, len(sc), _ptr(sc), _ptr(ef), _ptr(sg),
+                                                   _ptr(sp), ptr_bytes, _ptr(off), C.byref(nnz)))
+        return off, nnz.value
 
     # ---- learned scheduler ----------------------------------------------
     def set_scheduler_weights(self, w):

@@ -50,7 +50,8 @@ static void free_search_state(bfsx_ctx *ctx) {
   drop_loop_graph(ctx);  // the captured loop holds pointers into this state
   for (DevBuf *b : {&ctx->levels, &ctx->parents, &ctx->vis, &ctx->fbm[0], &ctx->fbm[1],
                     &ctx->queue[0], &ctx->queue[1], &ctx->glb_scan, &ctx->glb_start,
-                    &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark})
+                    &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark, &ctx->ms_seen,
+                    &ctx->ms_cur, &ctx->ms_nxt, &ctx->ms_lv, &ctx->ms_hubs})
     release(*b);
 }
 
@@ -899,6 +900,69 @@ int bfsx_stream_copy(bfsx_ctx *ctx, uint64_t bytes, int repeats, double *ms) {
     if (ms) *ms = t / repeats;
     release(a);
     release(b);
+  });
+}
+
+int bfsx_run_multi(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits,
+                   uint32_t n_sources, uint32_t n_bits, uint8_t *levels_out,
+                   bfsx_batch_stats *stats) {
+  if (!ctx || (n_sources && (!sources || !bits))) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    // the queue buffers and control block are shared with single searches
+    const MsResult r = ms_run(ctx, sources, bits, n_sources, n_bits, levels_out, 14);
+    if (stats) {
+      stats->ms = r.ms;
+      stats->edges_count = r.edges;
+      stats->n_levels = r.n_levels;
+      stats->push_levels = r.push_levels;
+      stats->pull_levels = r.pull_levels;
+    }
+  });
+}
+
+int bfsx_generate_rmat_union(bfsx_ctx *ctx, uint32_t n_graphs, const int *scales,
+                             const int *edge_factors, const uint64_t *seeds_graph,
+                             const uint64_t *seeds_perm, int col_ptr_bytes, uint32_t *offsets,
+                             uint64_t *nnz_out) {
+  if (!ctx || n_graphs == 0 || !scales || !edge_factors || !seeds_graph || !seeds_perm)
+    return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (col_ptr_bytes != 4 && col_ptr_bytes != 8) fail(BFSX_ERR_ARG, "col_ptr_bytes must be 4 or 8");
+    uint64_t V = 0, M = 0;
+    std::vector<uint64_t> off(n_graphs + 1, 0), eoff(n_graphs + 1, 0);
+    for (uint32_t g = 0; g < n_graphs; g++) {
+      if (scales[g] < 1 || scales[g] > 30) fail(BFSX_ERR_ARG, "scale must be in [1,30]");
+      if (edge_factors[g] < 1 || edge_factors[g] > 1024) fail(BFSX_ERR_ARG, "bad edge factor");
+      off[g] = V;
+      eoff[g] = M;
+      V += 1ull << scales[g];
+      M += (uint64_t)edge_factors[g] << scales[g];
+    }
+    off[n_graphs] = V;
+    eoff[n_graphs] = M;
+    if (V >= 0xFFFFFFFFull) fail(BFSX_ERR_ARG, "packed graphs exceed 2^32-1 vertices");
+    ctx->has_graph = false;
+    free_search_state(ctx);
+    release(ctx->col_ptr);
+    release(ctx->row_idx);
+    DevBuf du, dv;
+    ensure(du, sizeof(uint32_t) * M, ctx->stream);
+    ensure(dv, sizeof(uint32_t) * M, ctx->stream);
+    for (uint32_t g = 0; g < n_graphs; g++)
+      gen_rmat_pairs_offset(ctx, scales[g], eoff[g + 1] - eoff[g], thresh32(0.57),
+                            thresh32(0.57 + 0.19), thresh32(0.57 + 0.19 + 0.19),
+                            seeds_graph[g], seeds_perm[g],
+                            (uint32_t)off[g], du.as<uint32_t>() + eoff[g],
+                            dv.as<uint32_t>() + eoff[g]);
+    build_symmetric_csc(ctx, (uint32_t)V, du.as<uint32_t>(), dv.as<uint32_t>(), M, col_ptr_bytes);
+    release(du);
+    release(dv);
+    after_graph_change(ctx);
+    if (offsets)
+      for (uint32_t g = 0; g <= n_graphs; g++) offsets[g] = (uint32_t)off[g];
+    if (nnz_out) *nnz_out = ctx->nnz;
   });
 }
 

@@ -92,12 +92,12 @@ __global__ void k_perm_scatter(uint32_t n, uint64_t mask, const uint64_t *sorted
     perm[sorted[i] & mask] = i;
 }
 
-__global__ void k_apply_perm(uint64_t m, const uint32_t *perm, uint32_t *u,
+__global__ void k_apply_perm(uint64_t m, const uint32_t *perm, uint32_t offset, uint32_t *u,
                              uint32_t *v) {
   for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m;
        k += (uint64_t)gridDim.x * blockDim.x) {
-    u[k] = perm[u[k]];
-    v[k] = perm[v[k]];
+    u[k] = perm[u[k]] + offset;
+    v[k] = perm[v[k]] + offset;
   }
 }
 
@@ -184,6 +184,14 @@ static unsigned grid_for(uint64_t n, unsigned block = 256) {
 void gen_rmat_pairs(bfsx_ctx *ctx, int scale, uint64_t m, uint32_t tA, uint32_t tAB,
                     uint32_t tABC, uint64_t seed_graph, uint64_t seed_perm,
                     uint32_t *d_u, uint32_t *d_v) {
+  gen_rmat_pairs_offset(ctx, scale, m, tA, tAB, tABC, seed_graph, seed_perm, 0, d_u, d_v);
+}
+
+// Same pairs with every vertex id shifted by `offset` (graphs packed into
+// one vertex range).
+void gen_rmat_pairs_offset(bfsx_ctx *ctx, int scale, uint64_t m, uint32_t tA, uint32_t tAB,
+                           uint32_t tABC, uint64_t seed_graph, uint64_t seed_perm,
+                           uint32_t offset, uint32_t *d_u, uint32_t *d_v) {
   hipStream_t s = ctx->stream;
   const uint32_t n = 1u << scale;
   hipLaunchKernelGGL(k_rmat_pairs, dim3(grid_for(m)), dim3(256), 0, s, scale, m, tA,
@@ -207,7 +215,7 @@ void gen_rmat_pairs(bfsx_ctx *ctx, int scale, uint64_t m, uint32_t tA, uint32_t 
   hipLaunchKernelGGL(k_perm_scatter, dim3(grid_for(n)), dim3(256), 0, s, n, mask,
                      sorted.as<uint64_t>(), perm.as<uint32_t>());
   hipLaunchKernelGGL(k_apply_perm, dim3(grid_for(m)), dim3(256), 0, s, m,
-                     perm.as<uint32_t>(), d_u, d_v);
+                     perm.as<uint32_t>(), offset, d_u, d_v);
   BFSX_HIP(hipGetLastError());
   BFSX_HIP(hipStreamSynchronize(s));
   release(keys);

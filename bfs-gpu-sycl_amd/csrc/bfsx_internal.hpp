@@ -82,6 +82,9 @@ struct bfsx_ctx {
   std::vector<bfsx_level_record> trace;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_step0 = nullptr, ev_step1 = nullptr;
 
+  // batched multi-source search state
+  bfsx::DevBuf ms_seen, ms_cur, ms_nxt, ms_lv, ms_hubs;
+
   // captured level loop (BFSX_RUN_GRAPH_LOOP), rebuilt when its key changes
   hipGraph_t loop_graph = nullptr;
   hipGraphExec_t loop_exec = nullptr;
@@ -154,6 +157,18 @@ void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out);
 void level_histogram(bfsx_ctx *ctx, uint32_t max_levels, unsigned long long *d_counts,
                      unsigned long long *d_edges);
 void stream_copy(bfsx_ctx *ctx, const void *src, void *dst, uint64_t bytes);
+
+// ---- batched multi-source search (bfsx_msbfs.hip) -------------------------
+struct MsResult {
+  double ms = 0;
+  uint32_t n_levels = 0, push_levels = 0, pull_levels = 0;
+  unsigned long long edges = 0;
+};
+MsResult ms_run(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits, uint32_t n_src,
+                uint32_t n_bits, uint8_t *levels_out, uint32_t push_divisor);
+void gen_rmat_pairs_offset(bfsx_ctx *ctx, int scale, uint64_t m, uint32_t tA, uint32_t tAB,
+                           uint32_t tABC, uint64_t seed_graph, uint64_t seed_perm,
+                           uint32_t offset, uint32_t *d_u, uint32_t *d_v);
 
 // ---- learned scheduler (bfsx_sched.hip) -----------------------------------
 // Features of the frontier in queue `idx` -> MLP -> ctl->choice.

@@ -39,3 +39,43 @@ def build(engine, name, **override):
 def roots(engine, w):
     mode = 1 if w["kind"] == "grid" else 0
     return engine.pick_roots(w["n_roots"], w["seed_roots"], mode)
+
+
+def small_graph_shapes(n_graphs=64):
+    """Scale ~ U{12..18} and edge factor ~ U{8..32} for packed graph i, seeded 4000+i."""
+    import numpy as np
+    out = []
+    for i in range(n_graphs):
+        rng = np.random.default_rng(4000 + i)
+        out.append((int(rng.integers(12, 19)), int(rng.integers(8, 33))))
+    return out
+
+
+def build_batch(engine, big="rmat22", n_small=64, n_big_roots=256):
+    """The batched workload: R-MAT scale 22 (256 roots, bits 0..255) packed in
+    one CSC with 64 small R-MAT graphs, graph i searched from one root in bit i.
+    Returns (sources, bits, offsets, n_bits)."""
+    import numpy as np
+    w = WORKLOADS[big]
+    a, b, c = RMAT_ABC
+    # the big graph alone first, to draw its roots (it sits at offset 0 below)
+    engine.generate_rmat(w["scale"], w["edge_factor"], a, b, c, w["seed_graph"], w["seed_perm"],
+                         w["ptr_bytes"])
+    big_roots = engine.pick_roots(n_big_roots, w["seed_roots"])
+    shapes = small_graph_shapes(n_small)
+    scales = [w["scale"]] + [s for s, _ in shapes]
+    efs = [w["edge_factor"]] + [e for _, e in shapes]
+    sg = [w["seed_graph"]] + [4000 + i for i in range(n_small)]
+    sp = [w["seed_perm"]] + [5000 + i for i in range(n_small)]
+    off, _ = engine.generate_rmat_union(scales, efs, sg, sp, w["ptr_bytes"])
+    cp, _ = engine.download_csc()
+    deg = np.diff(cp.astype(np.int64))
+    small_roots = []
+    for i in range(n_small):
+        lo, hi = int(off[i + 1]), int(off[i + 2])
+        rng = np.random.default_rng(6000 + i)
+        cand = lo + rng.permutation(hi - lo)
+        small_roots.append(int(cand[np.argmax(deg[cand] > 0)]))
+    sources = np.concatenate([big_roots, small_roots]).astype(np.uint32)
+    bits = np.concatenate([np.arange(len(big_roots)), np.arange(n_small)]).astype(np.uint16)
+    return sources, bits, off, max(len(big_roots), n_small)

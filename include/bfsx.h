@@ -180,6 +180,30 @@ int bfsx_level_histogram(bfsx_ctx *ctx, uint64_t *counts, uint64_t *edges,
  * its duration.  Used to calibrate HBM counters and bandwidth. */
 int bfsx_stream_copy(bfsx_ctx *ctx, uint64_t bytes, int repeats, double *ms);
 
+/* ---- batched searches (no counterpart in the reference) -----------------
+ * Up to 256 searches advance together, one bit each in a per-vertex bitset.
+ * sources[i] starts search bits[i] at level 0 (several sources may share a
+ * bit: the level is then the distance to the nearest one).  levels_out is
+ * n_bits x n_cols uint8, one row per search (0xFF = unreached), or NULL.
+ * Fails with BFSX_ERR_OVERFLOW if a level would exceed 254. */
+typedef struct bfsx_batch_stats {
+  double ms;              /* level loop of the whole batch */
+  uint64_t edges_count;   /* sum over the searches of their edgesCount */
+  uint32_t n_levels;
+  uint32_t push_levels;   /* levels run top-down */
+  uint32_t pull_levels;   /* levels run bottom-up (symmetric graphs) */
+} bfsx_batch_stats;
+int bfsx_run_multi(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits,
+                   uint32_t n_sources, uint32_t n_bits, uint8_t *levels_out,
+                   bfsx_batch_stats *stats);
+/* Builds n_graphs R-MAT graphs (quadrants 0.57/0.19/0.19/0.05) packed into
+ * one symmetric CSC: graph g holds vertices [offsets[g], offsets[g+1]).
+ * offsets has n_graphs + 1 entries. */
+int bfsx_generate_rmat_union(bfsx_ctx *ctx, uint32_t n_graphs, const int *scales,
+                             const int *edge_factors, const uint64_t *seeds_graph,
+                             const uint64_t *seeds_perm, int col_ptr_bytes,
+                             uint32_t *offsets, uint64_t *nnz);
+
 /* ---- learned per-level scheduler (no counterpart in the reference) ------
  * An FP32 MLP 8 -> 32 -> 32 -> 4 (ReLU) predicts log2(device microseconds)
  * of each step kind (LLB, GLB, SLB, PULL) from frontier statistics; with
