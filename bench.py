@@ -44,7 +44,7 @@ def per_root_times(runs):
 
 
 def sweep(eng, roots, strategy, n):
-    out = []
+    out, st = [], None
     for _ in range(n):
         _, st = eng.run_batch(roots, strategy)
         out.append([s["ms"] for s in st])
@@ -196,6 +196,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run the streaming copy probe over this many bytes")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -212,6 +214,15 @@ def main():
     log("device", dev)
     if args.workload == "batch22":
         print(json.dumps(bench_batch(eng, args)), flush=True)
+        eng.close()
+        return
+    if args.copy_probe:
+        # plain streaming copy of known size: calibrates the HBM byte counters
+        # (FETCH_SIZE / WRITE_SIZE) and gives the achievable copy bandwidth
+        ms = eng.stream_copy(args.copy_probe, repeats=10)
+        print(json.dumps({"probe": "k_stream_copy", "bytes": args.copy_probe, "ms": ms,
+                          "GBps_read_plus_write": 2 * args.copy_probe / (ms * 1e-3) / 1e9}),
+              flush=True)
         eng.close()
         return
     t0 = time.time()

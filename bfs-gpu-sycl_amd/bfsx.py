@@ -115,6 +115,8 @@ SIGNATURES = {
     "bfsx_mlp_train": (C.c_int, [P, P, P, P, C.c_uint32, P, C.c_uint32, C.c_float, P]),
     "bfsx_sched_record": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, P, P, P,
                                     C.c_uint32, C.POINTER(C.c_uint32)]),
+    "bfsx_autotune": (C.c_int, [P, P, C.c_uint32, C.c_int, C.c_uint32, C.POINTER(Params),
+                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bfsx_get_params": (C.c_int, [P, C.POINTER(Params)]),
     "bfsx_set_params": (C.c_int, [P, C.POINTER(Params)]),
 }
@@ -279,6 +281,16 @@ class Engine:
         ms = C.c_double(0)
         self._check(lib().bfsx_stream_copy(self._h, nbytes, repeats, C.byref(ms)))
         return ms.value
+
+    def autotune(self, roots, strategy="llb", passes=1):
+        """Tunes the launch shape for `strategy` on the resident graph; returns
+        (params dict, gteps_before, gteps_after).  The params stay installed."""
+        roots = np.ascontiguousarray(roots, np.uint32)
+        p = Params()
+        before, after = C.c_double(0), C.c_double(0)
+        self._check(lib().bfsx_autotune(self._h, _ptr(roots), len(roots), strategy_id(strategy),
+                                        passes, C.byref(p), C.byref(before), C.byref(after)))
+        return p.as_dict(), before.value, after.value
 
     # ---- batched searches -----------------------------------------------
     def run_multi(self, sources, bits, n_bits, levels=True):

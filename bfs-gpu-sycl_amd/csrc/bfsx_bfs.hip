@@ -467,6 +467,8 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
   const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nwaves = gridDim.x * (WG / 64);
+  uint32_t n_found = 0;               // wave totals (uniform)
+  unsigned long long found_deg = 0;
   for (uint32_t g = blockIdx.x * (WG / 64) + (threadIdx.x >> 6); g < n_groups; g += nwaves) {
     const uint32_t u = g * 64 + lane;
     const uint32_t vword = a.vis[u >> 5];
@@ -538,13 +540,55 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         a.levels[u] = a.next_level;
         if (a.parents) a.parents[u] = par;
       }
-      wave_append(found, u, a.q_out, a.q_out_count);
-      if (a.q_out_edges) {
-        const unsigned long long s = wave_sum(found ? deg : 0ull);
-        if ((int)lane == __ffsll((unsigned long long)m) - 1) atomicAdd(a.q_out_edges, s);
-      }
+      n_found += (uint32_t)__popcll(m);
+      if (a.q_out_edges) found_deg += wave_sum(found ? deg : 0ull);
     }
   }
+  // The next frontier is the bitmap fbm_next; only its size and degree sum
+  // are published, with one atomic per work-group (a queue is built from the
+  // bitmap only if a top-down step follows).
+  __shared__ uint32_t s_n[WG / 64];
+  __shared__ unsigned long long s_d[WG / 64];
+  if (lane == 0) {
+    s_n[threadIdx.x >> 6] = n_found;
+    s_d[threadIdx.x >> 6] = found_deg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t n = 0;
+    unsigned long long d = 0;
+    for (int w = 0; w < WG / 64; w++) {
+      n += s_n[w];
+      d += s_d[w];
+    }
+    if (n) atomicAdd(a.q_out_count, n);
+    if (a.q_out_edges && d) atomicAdd(a.q_out_edges, d);
+  }
+}
+
+// Frontier queue from a frontier bitmap (after a bottom-up step, when a
+// top-down step follows).  A work-group owns words_per_group consecutive
+// words (a multiple of WG) and each thread a contiguous run of them, so the
+// queue comes out in vertex order.  Pass 1 counts, one atomicAdd reserves
+// the group's slots, pass 2 writes.
+template <int WG>
+__global__ __launch_bounds__(WG) void k_bm_to_queue(const uint32_t *bm, uint32_t n_words,
+                                                   uint32_t words_per_group, uint32_t *q,
+                                                   uint32_t *counter) {
+  __shared__ uint32_t s_wave[WG / 64 + 1];
+  __shared__ uint32_t s_base;
+  const uint32_t run = words_per_group / WG;
+  const uint32_t w0 = blockIdx.x * words_per_group + threadIdx.x * run;
+  const uint32_t w1 = min(w0 + run, n_words);
+  uint32_t cnt = 0;
+  for (uint32_t w = w0; w < w1; w++) cnt += __popc(bm[w]);
+  uint32_t total;
+  const uint32_t ex = block_excl_scan<WG>(cnt, total, s_wave);
+  if (threadIdx.x == 0) s_base = total ? atomicAdd(counter, total) : 0u;
+  __syncthreads();
+  uint32_t pos = s_base + ex;
+  for (uint32_t w = w0; w < w1; w++)
+    for (uint32_t b = bm[w]; b; b &= b - 1) q[pos++] = w * 32 + (__ffs(b) - 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -556,45 +600,70 @@ __device__ __forceinline__ uint32_t nz_bytes(uint32_t w) {
   return ((t >> 7) & 1u) | ((t >> 14) & 2u) | ((t >> 21) & 4u) | ((t >> 28) & 8u);
 }
 
-// A wave owns a 1024-vertex chunk: each lane reads 16 mark bytes as one
-// uint4, so the whole mark array streams through once.  New vertices get
-// their level, are appended to the next queue with one atomicAdd per wave
-// (positions by a wave prefix sum of per-lane counts), their visited bits
-// are OR-ed into words this wave owns (no atomics), and the next frontier
-// bitmap words are written in full.  Marks are cleared where set.
+// A work-group owns a run of up to COMPACT_MAX_CHUNKS consecutive
+// 1024-vertex chunks; a wave works on one chunk at a time, each lane reading
+// 16 mark bytes as one uint4.  Pass 1 counts the marked vertices of every
+// chunk into LDS; one wave scans the counts and reserves the group's queue
+// slots with a single atomicAdd (so the queue stays in vertex order inside a
+// group).  Pass 2 re-reads the marks (now in L2), gives new vertices their
+// level, writes them to the queue, ORs their visited bits into words this
+// wave owns (no atomics), writes the next frontier bitmap words in full and
+// clears the marks.  The degree sum goes out with one atomicAdd per group.
+constexpr uint32_t COMPACT_MAX_CHUNKS = 64;
+constexpr uint32_t COMPACT_WG = 256;
+
+__device__ __forceinline__ uint32_t mark_bits(uint4 m) {
+  if (!(m.x | m.y | m.z | m.w)) return 0u;
+  return nz_bytes(m.x) | (nz_bytes(m.y) << 4) | (nz_bytes(m.z) << 8) | (nz_bytes(m.w) << 12);
+}
+
 template <typename PtrT>
 __device__ __forceinline__ void compact_body(const PtrT *col_ptr, uint32_t n_rows, uint8_t *mark,
-                                             uint32_t n_chunks, uint32_t *vis, int32_t *levels,
-                                             int32_t next_level, uint32_t *q_out,
-                                             uint32_t *q_count, unsigned long long *q_edges,
-                                             uint32_t *fbm_next) {
+                                             uint32_t n_chunks, uint32_t chunks_per_group,
+                                             uint32_t *vis, int32_t *levels, int32_t next_level,
+                                             uint32_t *q_out, uint32_t *q_count,
+                                             unsigned long long *q_edges, uint32_t *fbm_next) {
+  constexpr uint32_t NW = COMPACT_WG / 64;
+  __shared__ uint32_t s_num[COMPACT_MAX_CHUNKS];
+  __shared__ uint32_t s_pre[COMPACT_MAX_CHUNKS];
+  __shared__ uint32_t s_base;
+  __shared__ unsigned long long s_deg[NW];
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
-  for (uint32_t c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); c < n_chunks;
-       c += nwaves) {
-    uint4 *mp = reinterpret_cast<uint4 *>(mark) + (size_t)c * 64 + lane;
-    const uint4 m = *mp;
-    uint32_t bits = 0;
-    if (m.x | m.y | m.z | m.w) {
-      bits = nz_bytes(m.x) | (nz_bytes(m.y) << 4) | (nz_bytes(m.z) << 8) | (nz_bytes(m.w) << 12);
-      *mp = make_uint4(0, 0, 0, 0);
-    }
-    const uint32_t cnt = __popc(bits);
-    const uint32_t incl = wave_incl_scan(cnt);
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t c0 = blockIdx.x * chunks_per_group;
+  if (c0 >= n_chunks) return;  // uniform over the group
+  const uint32_t nc = min(chunks_per_group, n_chunks - c0);
+  uint4 *mk = reinterpret_cast<uint4 *>(mark);
+  for (uint32_t i = wv; i < nc; i += NW) {
+    const uint32_t t = wave_sum((uint32_t)__popc(mark_bits(mk[(size_t)(c0 + i) * 64 + lane])));
+    if (lane == 0) s_num[i] = t;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    const uint32_t n = lane < nc ? s_num[lane] : 0u;
+    const uint32_t incl = wave_incl_scan(n);
     const uint32_t total = __shfl(incl, 63, 64);
-    uint32_t base = 0;
-    if (total) {
-      if (lane == 63) base = atomicAdd(q_count, total);
-      base = __shfl(base, 63, 64);
-    }
-    uint32_t pos = base + incl - cnt;
-    const uint32_t v0 = c * 1024 + lane * 16;
-    unsigned long long dsum = 0;
-    for (uint32_t b = bits; b; b &= b - 1) {
-      const uint32_t v = v0 + (__ffs(b) - 1);
-      levels[v] = next_level;
-      q_out[pos++] = v;
-      if (q_edges && v < n_rows) dsum += (unsigned long long)(col_ptr[v + 1] - col_ptr[v]);
+    if (lane < nc) s_pre[lane] = incl - n;
+    if (lane == 0) s_base = total ? atomicAdd(q_count, total) : 0u;
+  }
+  __syncthreads();
+  unsigned long long dsum = 0;
+  for (uint32_t i = wv; i < nc; i += NW) {
+    const uint32_t c = c0 + i;
+    uint32_t bits = 0;
+    if (s_num[i]) {
+      uint4 *mp = mk + (size_t)c * 64 + lane;
+      bits = mark_bits(*mp);
+      if (bits) *mp = make_uint4(0, 0, 0, 0);
+      const uint32_t cnt = __popc(bits);
+      uint32_t pos = s_base + s_pre[i] + wave_incl_scan(cnt) - cnt;
+      const uint32_t v0 = c * 1024 + lane * 16;
+      for (uint32_t b = bits; b; b &= b - 1) {
+        const uint32_t v = v0 + (__ffs(b) - 1);
+        levels[v] = next_level;
+        q_out[pos++] = v;
+        if (q_edges && v < n_rows) dsum += (unsigned long long)(col_ptr[v + 1] - col_ptr[v]);
+      }
     }
     const uint32_t partner = __shfl_xor(bits, 1, 64);
     if ((lane & 1) == 0) {
@@ -603,38 +672,44 @@ __device__ __forceinline__ void compact_body(const PtrT *col_ptr, uint32_t n_row
       if (word) vis[wi] |= word;
       if (fbm_next) fbm_next[wi] = word;
     }
-    if (q_edges && total) {
-      dsum = wave_sum(dsum);
-      if (lane == 0 && dsum) atomicAdd(q_edges, dsum);
+  }
+  if (q_edges) {
+    dsum = wave_sum(dsum);
+    if (lane == 0) s_deg[wv] = dsum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long d = 0;
+      for (uint32_t w = 0; w < NW; w++) d += s_deg[w];
+      if (d) atomicAdd(q_edges, d);
     }
   }
 }
 
 template <typename PtrT>
-__global__ __launch_bounds__(256) void k_compact(const PtrT *col_ptr, uint32_t n_rows,
-                                                 uint8_t *mark, uint32_t n_chunks,
-                                                 uint32_t *vis, int32_t *levels,
-                                                 int32_t next_level, uint32_t *q_out,
-                                                 uint32_t *q_count,
-                                                 unsigned long long *q_edges,
-                                                 uint32_t *fbm_next) {
-  compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, vis, levels, next_level, q_out, q_count,
-                     q_edges, fbm_next);
+__global__ __launch_bounds__(COMPACT_WG) void k_compact(const PtrT *col_ptr, uint32_t n_rows,
+                                                        uint8_t *mark, uint32_t n_chunks,
+                                                        uint32_t chunks_per_group, uint32_t *vis,
+                                                        int32_t *levels, int32_t next_level,
+                                                        uint32_t *q_out, uint32_t *q_count,
+                                                        unsigned long long *q_edges,
+                                                        uint32_t *fbm_next) {
+  compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, chunks_per_group, vis, levels, next_level,
+                     q_out, q_count, q_edges, fbm_next);
 }
 
 // Compaction inside a captured level loop: runs only when the step of this
 // level marked (same rule as k_td_dev), level from the control block.
 template <typename PtrT>
-__global__ __launch_bounds__(256) void k_compact_dev(const PtrT *col_ptr, uint32_t n_rows,
-                                                     uint8_t *mark, uint32_t n_chunks,
-                                                     uint32_t *vis, int32_t *levels,
-                                                     LevelCtl *ctl, DevStep d, uint32_t *q_out,
-                                                     int track, uint32_t *fbm_next) {
+__global__ __launch_bounds__(COMPACT_WG) void k_compact_dev(
+    const PtrT *col_ptr, uint32_t n_rows, uint8_t *mark, uint32_t n_chunks,
+    uint32_t chunks_per_group, uint32_t *vis, int32_t *levels, LevelCtl *ctl, DevStep d,
+    uint32_t *q_out, int track, uint32_t *fbm_next) {
   const uint32_t F = ctl->q_count[d.cur];
   if (F == 0 || F < d.mark_min_F) return;
   const uint32_t out = d.cur ^ 1;
-  compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, vis, levels, (int32_t)ctl->level + 1,
-                     q_out, &ctl->q_count[out], track ? &ctl->q_edges[out] : nullptr, fbm_next);
+  compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, chunks_per_group, vis, levels,
+                     (int32_t)ctl->level + 1, q_out, &ctl->q_count[out],
+                     track ? &ctl->q_edges[out] : nullptr, fbm_next);
 }
 
 // frontier bitmap of one level from the levels array (a wave owns 64 vertices)
@@ -710,6 +785,7 @@ __global__ void k_level_end(LevelCtl *ctl, uint32_t consumed, uint32_t produced,
     ctl->q_count[consumed] = 0;
     ctl->q_edges[consumed] = 0;
     ctl->glb_cursor = 0;
+    ctl->conv = 0;
     ctl->level += 1;
     ctl->done = n == 0;
   }
@@ -923,19 +999,51 @@ void step_pull(bfsx_ctx *ctx, const StepArgs &s) {
   BFSX_HIP(hipGetLastError());
 }
 
+// Compaction launch shape: about eight groups per CU, each owning between
+// 4 (one chunk per wave) and COMPACT_MAX_CHUNKS consecutive chunks.
+struct CompactShape {
+  uint32_t n_chunks, per_group;
+  unsigned grid;
+};
+static CompactShape compact_shape(const bfsx_ctx *ctx, uint32_t n_cols) {
+  CompactShape c;
+  c.n_chunks = (uint32_t)(((uint64_t)n_cols + 1023) / 1024);
+  const uint64_t groups = (uint64_t)ctx->cu_count * 8;
+  uint64_t per = (c.n_chunks + groups - 1) / groups;
+  per = per < COMPACT_WG / 64 ? COMPACT_WG / 64 : per;
+  per = per > COMPACT_MAX_CHUNKS ? COMPACT_MAX_CHUNKS : per;
+  c.per_group = (uint32_t)per;
+  c.grid = (unsigned)((c.n_chunks + per - 1) / per);
+  if (c.grid == 0) c.grid = 1;
+  return c;
+}
+
 void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next) {
-  const uint32_t n_chunks = (s.n_cols + 1023) / 1024;
-  const unsigned grid = cap_grid(((uint64_t)n_chunks + 3) / 4, (unsigned)ctx->cu_count * 16);
+  const CompactShape c = compact_shape(ctx, s.n_cols);
   unsigned long long *qe = s.track_degrees ? &s.ctl->q_edges[s.q_out_idx] : nullptr;
   uint32_t *qc = &s.ctl->q_count[s.q_out_idx];
   if (ctx->ptr_bytes == 8)
-    hipLaunchKernelGGL(k_compact<uint64_t>, dim3(grid), dim3(256), 0, ctx->stream,
-                       static_cast<const uint64_t *>(s.col_ptr), s.n_rows, mark, n_chunks, s.vis,
-                       s.levels, s.next_level, s.q_out, qc, qe, fbm_next);
+    hipLaunchKernelGGL(k_compact<uint64_t>, dim3(c.grid), dim3(COMPACT_WG), 0, ctx->stream,
+                       static_cast<const uint64_t *>(s.col_ptr), s.n_rows, mark, c.n_chunks,
+                       c.per_group, s.vis, s.levels, s.next_level, s.q_out, qc, qe, fbm_next);
   else
-    hipLaunchKernelGGL(k_compact<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream,
-                       static_cast<const uint32_t *>(s.col_ptr), s.n_rows, mark, n_chunks, s.vis,
-                       s.levels, s.next_level, s.q_out, qc, qe, fbm_next);
+    hipLaunchKernelGGL(k_compact<uint32_t>, dim3(c.grid), dim3(COMPACT_WG), 0, ctx->stream,
+                       static_cast<const uint32_t *>(s.col_ptr), s.n_rows, mark, c.n_chunks,
+                       c.per_group, s.vis, s.levels, s.next_level, s.q_out, qc, qe, fbm_next);
+  BFSX_HIP(hipGetLastError());
+}
+
+// Frontier queue `q` of the bitmap `bm` (after a bottom-up step); the slot
+// cursor is the control block's conversion counter, zeroed by level_end.
+void bitmap_to_queue(bfsx_ctx *ctx, const uint32_t *bm, uint32_t *q) {
+  const uint32_t n_words = (uint32_t)bitmap_words(ctx->n_cols);
+  const uint64_t groups = (uint64_t)ctx->cu_count * 8;
+  uint32_t per = (uint32_t)((n_words + groups - 1) / groups);
+  per = (per + 255) & ~255u;  // whole 1 KiB runs of words per group
+  const unsigned grid = (unsigned)((n_words + per - 1) / per);
+  LevelCtl *ctl = ctx->ctl.as<LevelCtl>();
+  hipLaunchKernelGGL((k_bm_to_queue<256>), dim3(grid), dim3(256), 0, ctx->stream, bm, n_words,
+                     per, q, &ctl->conv);
   BFSX_HIP(hipGetLastError());
 }
 
@@ -1008,11 +1116,10 @@ static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, uint
     case 1024: hipLaunchKernelGGL((k_td_dev<PtrT, 1024>), dim3(grid), dim3(1024), 0, s, a, ctl, d); break;
     default: fail(BFSX_ERR_ARG, "llb_wg must be 64, 128, 256, 512 or 1024");
   }
-  const uint32_t n_chunks = (ctx->n_cols + 1023) / 1024;
-  const unsigned cgrid = cap_grid(((uint64_t)n_chunks + 3) / 4, (unsigned)ctx->cu_count * 16);
-  hipLaunchKernelGGL(k_compact_dev<PtrT>, dim3(cgrid), dim3(256), 0, s,
+  const CompactShape c = compact_shape(ctx, ctx->n_cols);
+  hipLaunchKernelGGL(k_compact_dev<PtrT>, dim3(c.grid), dim3(COMPACT_WG), 0, s,
                      static_cast<const PtrT *>(ctx->col_ptr.p), ctx->n_rows,
-                     ctx->mark.as<uint8_t>(), n_chunks, ctx->vis.as<uint32_t>(),
+                     ctx->mark.as<uint8_t>(), c.n_chunks, c.per_group, ctx->vis.as<uint32_t>(),
                      ctx->levels.as<int32_t>(), ctl, d, ctx->queue[cur ^ 1].as<uint32_t>(),
                      track ? 1 : 0, (uint32_t *)nullptr);
   BFSX_HIP(hipGetLastError());

@@ -138,10 +138,12 @@ struct Walker {
   uint32_t F = 1, cur = 0, level = 0;
   int fbm_cur = 0;         // buffer holding the current frontier's bitmap
   bool fbm_valid = false;  // whether that bitmap is up to date
+  bool queue_valid = true; // false after a bottom-up step (frontier only as bitmap)
   bool pull_possible = false;
   bool track = false;      // accumulate frontier degree sums
   bool stamp = false;      // device clock around each step
   uint32_t *par = nullptr;
+  int last = BFSX_LLB;     // step kind of the last launch
 
   // Launches the kernels of one level with step kind `choice`, then the
   // level bookkeeping kernel.  The caller reads the control block back and
@@ -166,6 +168,8 @@ struct Walker {
     const bool use_mark = choice != BFSX_PULL && mark_step(ctx, F);
     a.mark = use_mark ? ctx->mark.as<uint8_t>() : nullptr;
     if (stamp) level_begin(ctx);
+    if (choice != BFSX_PULL && !queue_valid)
+      bitmap_to_queue(ctx, ctx->fbm[fbm_cur].as<uint32_t>(), ctx->queue[cur].as<uint32_t>());
     bool new_fbm = false;
     switch (choice) {
       case BFSX_LLB: step_llb(ctx, a, false); break;
@@ -189,6 +193,7 @@ struct Walker {
       new_fbm = fbm_next != nullptr;
     }
     level_end(ctx, cur, cur ^ 1, stamp, choice);
+    last = choice;
     return new_fbm;
   }
 
@@ -197,6 +202,7 @@ struct Walker {
     ctx->mark_dirty = false;
     if (new_fbm) fbm_cur ^= 1;
     fbm_valid = new_fbm;
+    queue_valid = last != BFSX_PULL;
     F = ctx->h_ctl->q_count[cur ^ 1];
     cur ^= 1;
     level++;
@@ -1020,6 +1026,77 @@ int bfsx_sched_record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, ui
     set_device(ctx);
     const uint32_t r = record(ctx, roots, n_roots, repeats, epsilon, seed, X, T, M, max_rows);
     if (n_rows) *n_rows = r;
+  });
+}
+
+// Harmonic-mean GTEPS of searches from `roots` with the context's parameters.
+static double tune_score(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n, int strategy,
+                         const std::vector<double> &edges) {
+  double s = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const RunOut r = search(ctx, roots[i], strategy, 0, false);
+    s += r.ms * 1e-3 / std::max(edges[i], 1.0);
+  }
+  return s > 0 ? n / s / 1e9 : 0.0;
+}
+
+int bfsx_autotune(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, int strategy,
+                  uint32_t passes, bfsx_params *best, double *gteps_before, double *gteps_after) {
+  if (!ctx || !roots || n_roots == 0) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
+    if (strategy < BFSX_LLB || strategy > BFSX_LEARNED) fail(BFSX_ERR_ARG, "unknown strategy");
+    if (passes < 1 || passes > 4) fail(BFSX_ERR_ARG, "passes must be in [1,4]");
+    std::vector<double> edges(n_roots);
+    for (uint32_t i = 0; i < n_roots; i++) {
+      const RunOut r = search(ctx, roots[i], strategy, 0, false);
+      bfsx_run_stats st;
+      finish_run(ctx, r, nullptr, nullptr, &st);
+      edges[i] = (double)st.edges_count;
+    }
+    struct Knob {
+      uint32_t bfsx_params::*field;
+      std::vector<uint32_t> values;
+      uint32_t kinds;  // step kinds whose kernels read this parameter
+    };
+    const uint32_t L = 1u << BFSX_LLB, G = 1u << BFSX_GLB, S = 1u << BFSX_SLB, P = 1u << BFSX_PULL;
+    const std::vector<Knob> knobs = {
+        {&bfsx_params::llb_wg, {64, 128, 256, 512, 1024}, L | S},
+        {&bfsx_params::hub_degree, {0, 256, 1024, 4096, 16384}, L | S},
+        {&bfsx_params::mark_ratio_q10, {4, 16, 64, 256, 1024}, L | S | G},
+        {&bfsx_params::glb_wg, {128, 256, 512, 1024}, G},
+        {&bfsx_params::glb_blocks_per_cu, {2, 4, 8, 16, 32}, G},
+        {&bfsx_params::glb_chunk, {1024, 2048, 4096, 8192, 16384}, G},
+        {&bfsx_params::pull_wg, {64, 128, 256, 512}, P},
+    };
+    const uint32_t used = strategy == BFSX_LEARNED ? (L | G | S | P) : (1u << strategy);
+    bfsx_params best_p = ctx->params;
+    double best_s = tune_score(ctx, roots, n_roots, strategy, edges);
+    const double start_s = best_s;
+    for (uint32_t pass = 0; pass < passes; pass++) {
+      for (const Knob &k : knobs) {
+        if (!(k.kinds & used)) continue;
+        for (uint32_t v : k.values) {
+          if (best_p.*(k.field) == v) continue;
+          bfsx_params p = best_p;
+          p.*(k.field) = v;
+          validate_params(p);
+          ctx->params = p;
+          drop_loop_graph(ctx);
+          const double sc = tune_score(ctx, roots, n_roots, strategy, edges);
+          if (sc > best_s * 1.01) {  // ignore differences within timing noise
+            best_s = sc;
+            best_p = p;
+          }
+        }
+      }
+    }
+    ctx->params = best_p;
+    drop_loop_graph(ctx);
+    if (best) *best = best_p;
+    if (gteps_before) *gteps_before = start_s;
+    if (gteps_after) *gteps_after = best_s;
   });
 }
 

@@ -32,6 +32,7 @@ struct LevelCtl {
   float sched_out[4];              // scheduler outputs of the last decision
   float sched_in[8];               // scheduler inputs of the last decision
   uint32_t kind_count[4];          // levels expanded by each step kind
+  uint32_t conv;                   // slot cursor of a bitmap-to-queue conversion
 };
 
 // Bitmaps hold whole 1024-vertex chunks (32 words) so that a wave can own a
@@ -146,6 +147,7 @@ void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *par
 void step_llb(bfsx_ctx *ctx, const StepArgs &a, bool strided);
 void step_glb(bfsx_ctx *ctx, const StepArgs &a);
 void step_pull(bfsx_ctx *ctx, const StepArgs &a);
+void bitmap_to_queue(bfsx_ctx *ctx, const uint32_t *bm, uint32_t *q);
 void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level,
                    uint32_t *fbm);
 void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next);

@@ -237,6 +237,17 @@ int bfsx_sched_record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots,
 /* ---- launch shape ------------------------------------------------------- */
 int bfsx_get_params(bfsx_ctx *ctx, bfsx_params *out);
 int bfsx_set_params(bfsx_ctx *ctx, const bfsx_params *in);
+/* Autotuner (replaces the reference's hand-set SM_FACTOR and fixed
+ * work-group size): coordinate descent over the launch-shape parameters that
+ * `strategy` uses (BFSX_LEARNED tunes those of every step kind), scoring
+ * each candidate by the harmonic-mean throughput of searches from `roots`
+ * on the resident graph; `passes` sweeps over the parameters (1-4).  The
+ * best parameters are installed in the context and copied to *best;
+ * *gteps_before / *gteps_after (optional) report the score of the starting
+ * and the final parameters. */
+int bfsx_autotune(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, int strategy,
+                  uint32_t passes, bfsx_params *best, double *gteps_before,
+                  double *gteps_after);
 
 #ifdef __cplusplus
 }

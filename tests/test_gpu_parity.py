@@ -334,3 +334,21 @@ def test_graph_loop_unsupported_combinations(engine):
         engine.run(0, "llb", graph_loop=True, parents=True)
     with pytest.raises(bfsx.BfsxError):
         engine.run(0, "llb", graph_loop=True, step_times=True)
+
+
+# ---------------------------------------------------------------------------
+# autotuner
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("strategy", ["llb", "glb"])
+def test_autotune_installs_valid_parameters(engine, strategy):
+    engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 161, 162, 4)
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(6, 163)
+    old = engine.params
+    try:
+        p, before, after = engine.autotune(roots[:3], strategy, passes=1)
+        assert after >= before > 0
+        assert engine.params == p
+        assert_levels(engine, cp, ri, roots[3:], [strategy])
+    finally:
+        engine.set_params(**old)
