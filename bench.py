@@ -58,15 +58,27 @@ def algorithmic_bytes(F, E_F, D, w_ptr, L_b=4):
     return 5.0 * E_F + F * (2.0 * w_ptr + 4.0) + D * (4.0 + L_b)
 
 
+def pull_bytes(n, UV, S, D, w_ptr, L_b=4):
+    """Bottom-up useful bytes of one level: the visited bitmap read, the
+    current frontier bitmap read once and the next one written (n/8 each),
+    col_ptr twice per unvisited candidate, one row_idx read per adjacency
+    entry examined, a level write per discovered vertex."""
+    return 3.0 * n / 8.0 + UV * 2.0 * w_ptr + 4.0 * S + D * L_b
+
+
 KERNEL_NAMES = {"llb": "k_llb", "slb": "k_llb (strided)", "glb": "k_glb_expand + scan kernels",
                 "pull": "k_pull"}
 
 
-def roofline_leg(eng, roots, strategy, w_ptr):
+def roofline_leg(eng, roots, strategy, info):
     """Per-level step times (HIP events around each step's kernels on the
     engine stream) and level histograms -> algorithmic bytes / time of the
     dominant step kind."""
     import bfsx
+    w_ptr = info["col_ptr_bytes"]
+    n = float(info["n_cols"])
+    live = n - float(info["n_isolated"])  # vertices a bottom-up step may test
+    pull = bfsx.STEP_NAMES.index("pull")
     per_kind_bytes = np.zeros(4)
     per_kind_us = np.zeros(4)
     per_kind_launches = np.zeros(4)
@@ -74,13 +86,18 @@ def roofline_leg(eng, roots, strategy, w_ptr):
         eng.run(int(r), strategy, levels=False, step_times=True)
         tr = eng.last_trace()
         counts, edges = eng.level_histogram()
+        reached = np.cumsum(np.asarray(counts, np.float64))
         for rec in tr:
             L = rec["level"]
             F = float(counts[L])
             E = float(edges[L])
             D = float(counts[L + 1]) if L + 1 < len(counts) else 0.0
             k = rec["choice"]
-            per_kind_bytes[k] += algorithmic_bytes(F, E, D, w_ptr)
+            if k == pull:
+                per_kind_bytes[k] += pull_bytes(n, live - reached[L], float(rec["pull_scanned"]),
+                                                D, w_ptr)
+            else:
+                per_kind_bytes[k] += algorithmic_bytes(F, E, D, w_ptr)
             per_kind_us[k] += rec["us"]
             per_kind_launches[k] += 1
     k = int(np.argmax(per_kind_us))
@@ -284,9 +301,14 @@ def main():
                                    "rows": sched_meta["rows"],
                                    "loss_final": round(sched_meta["loss_final"], 4)}
     if not args.no_roofline:
-        rl = roofline_leg(eng, roots, strategy, info["col_ptr_bytes"])
+        rl = roofline_leg(eng, roots, strategy, info)
         rl["traffic"] = traffic_from_profiles(args.workload, rl["step"])
         result["roofline"] = rl
+        if strategy != "llb":
+            # the top-down explore kernel on its own (fixed LLB run)
+            ex = roofline_leg(eng, roots, "llb", info)
+            ex["traffic"] = traffic_from_profiles(args.workload, ex["step"])
+            result["explore_roofline_llb"] = ex
         result["graph500_gteps"] = round(float(value) / 2, 3)
     if not args.no_cpu:
         lev_cache = {}

@@ -59,7 +59,8 @@ class LevelRecord(C.Structure):
     _fields_ = [("level", C.c_uint32), ("choice", C.c_uint32),
                 ("frontier_vertices", C.c_uint64), ("frontier_edges", C.c_uint64),
                 ("unvisited_vertices", C.c_uint64), ("unvisited_edges", C.c_uint64),
-                ("discovered", C.c_uint64), ("us", C.c_double)]
+                ("discovered", C.c_uint64), ("us", C.c_double),
+                ("pull_scanned", C.c_uint64)]
 
 
 class Params(C.Structure):

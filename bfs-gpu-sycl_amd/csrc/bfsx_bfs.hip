@@ -98,17 +98,27 @@ __device__ __forceinline__ void wave_append(bool isnew, uint32_t n, uint32_t *q_
   if (isnew) q_out[base + lanes_below(m)] = n;
 }
 
-// Degree sum of the discovered vertices of a wave, one atomic per wave.
+// Adds a per-thread value over the work-group to *dst with one atomic.
+// Every thread of the group must call it.
+template <int WG>
+__device__ __forceinline__ void block_atomic_add(unsigned long long v, unsigned long long *dst) {
+  __shared__ unsigned long long s_part[WG / 64];
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < WG / 64; w++) t += s_part[w];
+    if (t) atomicAdd(dst, t);
+  }
+}
+
+// Out-degree of a discovered vertex (for the next frontier's edge count).
 template <typename PtrT>
-__device__ __forceinline__ void wave_track_degree(bool isnew, uint32_t n, uint32_t n_rows,
-                                                  const PtrT *col_ptr,
-                                                  unsigned long long *acc) {
-  const unsigned long long m = __ballot(isnew);
-  if (m == 0) return;
-  unsigned long long d = 0;
-  if (isnew && n < n_rows) d = (unsigned long long)(col_ptr[n + 1] - col_ptr[n]);
-  d = wave_sum(d);
-  if ((int)__lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(acc, d);
+__device__ __forceinline__ unsigned long long out_degree(const PtrT *col_ptr, uint32_t n,
+                                                         uint32_t n_rows) {
+  return n < n_rows ? (unsigned long long)(col_ptr[n + 1] - col_ptr[n]) : 0ull;
 }
 
 struct KArgs {
@@ -126,6 +136,7 @@ struct KArgs {
   int32_t next_level;
   uint32_t hub_degree;
   uint8_t *mark;      // non-null: mark-then-compact mode
+  unsigned long long *examined;  // nullable: bottom-up adjacency entries read
 };
 
 // Two ways to discover a neighbour n of a frontier vertex:
@@ -138,7 +149,7 @@ struct KArgs {
 //    where per-edge device-scope atomics would dominate).
 template <typename PtrT>
 __device__ __forceinline__ void discover(const KArgs &a, const PtrT *col_ptr, uint32_t n,
-                                         uint32_t from) {
+                                         uint32_t from, unsigned long long &dacc) {
   if (a.mark) {
     if (!((a.vis[n >> 5] >> (n & 31)) & 1u)) {
       a.mark[n] = 1;
@@ -150,9 +161,9 @@ __device__ __forceinline__ void discover(const KArgs &a, const PtrT *col_ptr, ui
   if (isnew) {
     a.levels[n] = a.next_level;
     if (a.parents) a.parents[n] = from;
+    if (a.q_out_edges) dacc += out_degree(col_ptr, n, a.n_rows);
   }
   wave_append(isnew, n, a.q_out, a.q_out_count);
-  if (a.q_out_edges) wave_track_degree(isnew, n, a.n_rows, col_ptr, a.q_out_edges);
 }
 
 // Expands the edge range [lo, hi) of a work-group whose frontier entries are
@@ -167,7 +178,8 @@ constexpr int EPT = 4;
 template <typename PtrT, int WG>
 __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr,
                                              const PtrT *s_scan, const PtrT *s_start,
-                                             const uint32_t *s_v, PtrT lo, PtrT hi) {
+                                             const uint32_t *s_v, PtrT lo, PtrT hi,
+                                             unsigned long long &dacc) {
   for (PtrT base = lo + (PtrT)threadIdx.x * EPT; base < hi; base += (PtrT)WG * EPT) {
     uint32_t j = 0;
 #pragma unroll
@@ -200,9 +212,9 @@ __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr
         if (isnew) {
           a.levels[n[k]] = a.next_level;
           if (a.parents) a.parents[n[k]] = from[k];
+          if (a.q_out_edges) dacc += out_degree(col_ptr, n[k], a.n_rows);
         }
         wave_append(isnew, n[k], a.q_out, a.q_out_count);
-        if (a.q_out_edges) wave_track_degree(isnew, n[k], a.n_rows, col_ptr, a.q_out_edges);
       }
     }
   }
@@ -225,7 +237,8 @@ struct LlbShared {
 // SLB), block scan of their degrees, cooperative expansion of their edges.
 template <typename PtrT, int WG>
 __device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh, uint32_t tile,
-                                         uint32_t n_tiles, bool strided) {
+                                         uint32_t n_tiles, bool strided,
+                                         unsigned long long &dacc) {
   constexpr int NW = WG / 64;
   PtrT *s_start = sh.start;
   PtrT *s_scan = sh.scan;
@@ -261,7 +274,7 @@ __device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh
   __syncthreads();
   // owner of edge i = last slot j with s_scan[j] <= i (empty slots tie with
   // the next one and are skipped by taking the last)
-  expand_edges<PtrT, WG>(a, col_ptr, s_scan, s_start, s_v, PtrT(0), total);
+  expand_edges<PtrT, WG>(a, col_ptr, s_scan, s_start, s_v, PtrT(0), total, dacc);
   if (a.hub_degree) {
     const uint32_t nh = s_hub_n;
     const uint32_t w = t >> 6, l = t & 63;
@@ -269,25 +282,29 @@ __device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh
       const uint32_t slot = s_hub[h];
       const PtrT b = s_start[slot];
       const PtrT e1 = col_ptr[s_v[slot] + 1];
-      for (PtrT e = b + l; e < e1; e += 64) discover<PtrT>(a, col_ptr, a.row_idx[e], s_v[slot]);
+      for (PtrT e = b + l; e < e1; e += 64)
+        discover<PtrT>(a, col_ptr, a.row_idx[e], s_v[slot], dacc);
     }
   }
 }
 
-// Host-sized launch: one tile per work-group, grid = ceil(F / WG).
+// Host-sized launch: one tile per work-group, grid = ceil(F / WG).  The
+// degree sum of exact-once discoveries leaves with one atomic per group.
 template <typename PtrT, int WG, bool STRIDED>
 __global__ __launch_bounds__(WG) void k_llb(KArgs a) {
   __shared__ LlbShared<PtrT, WG> sh;
-  llb_tile<PtrT, WG>(a, sh, blockIdx.x, gridDim.x, STRIDED);
+  unsigned long long dacc = 0;
+  llb_tile<PtrT, WG>(a, sh, blockIdx.x, gridDim.x, STRIDED, dacc);
+  if (a.q_out_edges && !a.mark) block_atomic_add<WG>(dacc, a.q_out_edges);
 }
 
 // Device-driven top-down step for captured level loops: the frontier size,
 // level, discovery mode and LLB/SLB choice come from the control block, and
 // a fixed grid walks the tiles.  Kernels after the search ended see F = 0.
 struct DevStep {
-  uint32_t cur;          // queue buffer holding the frontier
-  uint32_t mark_min_F;   // frontiers at least this large use mark + compact
-  int fixed;             // BFSX_LLB or BFSX_SLB, or -1: read ctl->choice
+  uint32_t cur;                  // queue buffer holding the frontier
+  int fixed;                     // BFSX_LLB or BFSX_SLB, or -1: read ctl->choice
+  unsigned long long mark_min_E; // frontiers with at least this many edges mark + compact
 };
 
 template <typename PtrT, int WG>
@@ -299,10 +316,12 @@ __global__ __launch_bounds__(WG) void k_td_dev(KArgs a, LevelCtl *ctl, DevStep d
   if (choice != BFSX_LLB && choice != BFSX_SLB) return;
   a.F = F;
   a.next_level = (int32_t)ctl->level + 1;
-  if (F < d.mark_min_F) a.mark = nullptr;
+  if (ctl->q_edges[d.cur] < d.mark_min_E) a.mark = nullptr;
   const uint32_t n_tiles = (F + WG - 1) / WG;
+  unsigned long long dacc = 0;
   for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
-    llb_tile<PtrT, WG>(a, sh, tile, n_tiles, choice == BFSX_SLB);
+    llb_tile<PtrT, WG>(a, sh, tile, n_tiles, choice == BFSX_SLB, dacc);
+  if (a.q_out_edges && !a.mark) block_atomic_add<WG>(dacc, a.q_out_edges);
 }
 
 // ---------------------------------------------------------------------------
@@ -419,6 +438,7 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
   const uint32_t t = threadIdx.x;
   const uint32_t n_chunks = ctl->glb_chunks;
   const PtrT total = (PtrT)ctl->glb_total;
+  unsigned long long dacc = 0;
   for (;;) {
     if (t == 0) s_c = atomicAdd(&ctl->glb_cursor, 1u);
     __syncthreads();
@@ -445,10 +465,11 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
       __syncthreads();
       const PtrT lo = e0 > s_win_lo ? e0 : s_win_lo;
       const PtrT hi = e1 < s_win_hi ? e1 : s_win_hi;
-      expand_edges<PtrT, WG>(a, col_ptr, s_scan, s_start, s_v, lo, hi);
+      expand_edges<PtrT, WG>(a, col_ptr, s_scan, s_start, s_v, lo, hi, dacc);
       __syncthreads();
     }
   }
+  if (a.q_out_edges && !a.mark) block_atomic_add<WG>(dacc, a.q_out_edges);
 }
 
 // ---------------------------------------------------------------------------
@@ -456,113 +477,218 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
 // ---------------------------------------------------------------------------
 // Candidates with more neighbours than this are scanned by the whole wave.
 constexpr uint32_t PULL_WAVE_DEG = 64;
+// A wave takes 2048 vertices (64 visited-bitmap words) at a time.
+constexpr uint32_t PULL_CHUNK = 2048;
+// Frontier summary: one bit per 2^shift vertices, at most
+// PULL_SUM_WORDS_MAX words, held in dynamic LDS by every work-group of the
+// bottom-up step.
 
-__device__ __forceinline__ bool in_frontier(const uint32_t *fbm, uint32_t w) {
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Whether w is in the current frontier; a clear summary bit answers without
+// touching the (L2-missing) frontier bitmap.
+__device__ __forceinline__ bool in_frontier(const uint32_t *fbm, const uint32_t *s_sum,
+                                            uint32_t shift, uint32_t w) {
+  if (s_sum) {
+    const uint32_t b = w >> shift;
+    if (!((s_sum[b >> 5] >> (b & 31)) & 1u)) return false;
+  }
   return (fbm[w >> 5] >> (w & 31)) & 1u;
 }
 
+// Summary of a frontier bitmap: bit b = any frontier vertex in
+// [b << shift, (b + 1) << shift); shift >= 5, so a bit covers whole words.
+__global__ void k_fbm_summary(const uint32_t *fbm, uint32_t n_words, uint32_t shift,
+                              uint32_t *sum, uint32_t n_bits) {
+  const uint32_t per = 1u << (shift - 5);  // bitmap words per summary bit
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n_bits; b0 += stride) {
+    const uint32_t b = b0 + threadIdx.x;
+    uint32_t any = 0;
+    if (b < n_bits) {
+      const uint32_t w0 = b * per;
+      const uint32_t w1 = min(w0 + per, n_words);
+      for (uint32_t w = w0; w < w1; w++) any |= fbm[w];
+    }
+    const unsigned long long m = __ballot(any != 0);
+    const uint32_t lane = threadIdx.x & 63;
+    if (lane == 0 && b < n_bits) sum[b >> 5] = (uint32_t)m;
+    if (lane == 32 && b < n_bits) sum[b >> 5] = (uint32_t)(m >> 32);
+  }
+}
+
+// Index of the k-th (0-based) set bit of m, which has more than k set bits:
+// the largest p with popc(m & ((1 << p) - 1)) <= k.
+__device__ __forceinline__ uint32_t kth_set_bit(uint32_t m, uint32_t k) {
+  uint32_t p = 0;
+#pragma unroll
+  for (uint32_t sh = 16; sh > 0; sh >>= 1)
+    if ((uint32_t)__popc(m & ((1u << (p + sh)) - 1u)) <= k) p += sh;
+  return p;
+}
+
+// Bottom-up step.  A wave owns 2048-vertex chunks (grid-stride): lane l
+// reads visited word l of the chunk (one coalesced load) and holds its
+// unvisited bits.  The unvisited vertices are then checked 64 at a time,
+// one per lane: a wave prefix sum of the per-word counts and a shuffle
+// binary search give each lane its word, a popcount search its bit (no
+// LDS list).  Lists longer than PULL_WAVE_DEG are scanned by the whole
+// wave.  Frontier probes go through the LDS summary when the frontier is
+// sparse.  Found vertices set bits in 64 LDS words that the wave then writes
+// as the next frontier bitmap and ORs into the visited bitmap (the wave owns
+// both word ranges: no global atomics).  The next frontier's size and degree
+// sum leave with one atomic per work-group; no queue is written.
 template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
-                                             uint32_t *fbm_next, uint32_t n_groups) {
+                                             uint32_t *fbm_next, const uint32_t *g_sum,
+                                             uint32_t sum_words, uint32_t shift) {
+  constexpr int NW = WG / 64;
+  extern __shared__ uint32_t s_sum_mem[];  // sum_words entries when g_sum is set
+  __shared__ uint32_t s_found[NW][64];
   const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t nwaves = gridDim.x * (WG / 64);
-  uint32_t n_found = 0;               // wave totals (uniform)
-  unsigned long long found_deg = 0;
-  for (uint32_t g = blockIdx.x * (WG / 64) + (threadIdx.x >> 6); g < n_groups; g += nwaves) {
-    const uint32_t u = g * 64 + lane;
-    const uint32_t vword = a.vis[u >> 5];
-    const bool cand = u < a.n_rows && !((vword >> (u & 31)) & 1u);
-    bool found = false;
-    uint32_t par = 0;
-    PtrT b = 0, e = 0;
-    if (cand) {
-      b = col_ptr[u];
-      e = col_ptr[u + 1];
-    }
-    const unsigned long long deg = (unsigned long long)(e - b);
-    // small lists: each lane scans its own, four independent loads per round
-    // (indices past the end repeat the last neighbour, which is harmless)
-    if (cand && deg <= PULL_WAVE_DEG) {
-      for (PtrT k = b; k < e; k += 4) {
-        const PtrT last = e - 1;
-        const uint32_t w0 = a.row_idx[k];
-        const uint32_t w1 = a.row_idx[k + 1 < e ? k + 1 : last];
-        const uint32_t w2 = a.row_idx[k + 2 < e ? k + 2 : last];
-        const uint32_t w3 = a.row_idx[k + 3 < e ? k + 3 : last];
-        const bool h0 = in_frontier(fbm_cur, w0), h1 = in_frontier(fbm_cur, w1);
-        const bool h2 = in_frontier(fbm_cur, w2), h3 = in_frontier(fbm_cur, w3);
-        if (h0 | h1 | h2 | h3) {
-          found = true;
-          par = h0 ? w0 : h1 ? w1 : h2 ? w2 : w3;
-          break;
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t *s_sum = nullptr;
+  if (g_sum) {
+    for (uint32_t i = threadIdx.x; i < sum_words; i += WG) s_sum_mem[i] = g_sum[i];
+    __syncthreads();
+    s_sum = s_sum_mem;
+  }
+  const uint32_t n_words = (uint32_t)bitmap_words(a.n_cols);
+  const uint32_t n_chunks = (a.n_rows + PULL_CHUNK - 1) / PULL_CHUNK;
+  uint32_t n_found = 0;
+  unsigned long long found_deg = 0, scanned = 0;
+  uint32_t *fw = s_found[wv];
+  for (uint32_t c = blockIdx.x * NW + wv; c < n_chunks; c += gridDim.x * NW) {
+    const uint32_t wi = c * 64 + lane;
+    const uint32_t vbase = wi * 32;
+    const uint32_t vword = wi < n_words ? a.vis[wi] : ~0u;
+    uint32_t cm = ~vword;
+    if (vbase >= a.n_rows) cm = 0;
+    else if (a.n_rows - vbase < 32) cm &= (1u << (a.n_rows - vbase)) - 1u;
+    const uint32_t cnt = __popc(cm);
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t excl = incl - cnt;
+    fw[lane] = 0;
+    wave_sync_lds();
+    for (uint32_t r = 0; r < total; r += 64) {
+      const uint32_t idx = r + lane;
+      const bool cand = idx < total;
+      // owner word: the last lane whose exclusive count is <= idx (it has
+      // a non-zero count, since a zero-count lane ties with the next one)
+      uint32_t o = 0;
+#pragma unroll
+      for (uint32_t st = 32; st > 0; st >>= 1) {
+        const uint32_t e = __shfl(excl, (int)(o + st), 64);
+        if (e <= idx) o += st;
+      }
+      const uint32_t om = __shfl(cm, (int)o, 64);
+      const uint32_t oe = __shfl(excl, (int)o, 64);
+      const uint32_t off = cand ? o * 32 + kth_set_bit(om, idx - oe) : 0u;
+      const uint32_t u = c * PULL_CHUNK + off;
+      bool found = false;
+      uint32_t par = 0;
+      PtrT b = 0, e = 0;
+      if (cand) {
+        b = col_ptr[u];
+        e = col_ptr[u + 1];
+      }
+      const PtrT deg = e - b;
+      // short lists: four independent probes per round (indices past the
+      // end repeat the last neighbour, which is harmless)
+      if (cand && deg <= PULL_WAVE_DEG) {
+        PtrT k = b;
+        for (; k < e; k += 4) {
+          const PtrT last = e - 1;
+          const uint32_t w0 = a.row_idx[k];
+          const uint32_t w1 = a.row_idx[k + 1 < e ? k + 1 : last];
+          const uint32_t w2 = a.row_idx[k + 2 < e ? k + 2 : last];
+          const uint32_t w3 = a.row_idx[k + 3 < e ? k + 3 : last];
+          const bool h0 = in_frontier(fbm_cur, s_sum, shift, w0);
+          const bool h1 = in_frontier(fbm_cur, s_sum, shift, w1);
+          const bool h2 = in_frontier(fbm_cur, s_sum, shift, w2);
+          const bool h3 = in_frontier(fbm_cur, s_sum, shift, w3);
+          if (h0 | h1 | h2 | h3) {
+            found = true;
+            par = h0 ? w0 : h1 ? w1 : h2 ? w2 : w3;
+            break;
+          }
+        }
+        if (a.examined) scanned += (unsigned long long)((k + 4 < e ? k + 4 : e) - b);
+      }
+      // long lists: the wave scans one at a time, 64 neighbours per round
+      unsigned long long big = __ballot(cand && deg > PULL_WAVE_DEG);
+      while (big) {
+        const int src = __ffsll(big) - 1;
+        big &= big - 1;
+        const PtrT bb = __shfl(b, src, 64), ee = __shfl(e, src, 64);
+        bool hit = false;
+        uint32_t hp = 0;
+        for (PtrT base = bb; base < ee; base += 64) {
+          const PtrT k = base + lane;
+          uint32_t w = 0;
+          bool h = false;
+          if (k < ee) {
+            w = a.row_idx[k];
+            h = in_frontier(fbm_cur, s_sum, shift, w);
+          }
+          const unsigned long long hm = __ballot(h);
+          if (a.examined && lane == 0) scanned += (unsigned long long)(ee - base < 64 ? ee - base : 64);
+          if (hm) {
+            hp = __shfl(w, __ffsll(hm) - 1, 64);
+            hit = true;
+            break;
+          }
+        }
+        if ((int)lane == src) {
+          found = hit;
+          par = hp;
         }
       }
-    }
-    // long lists: the wave scans one list at a time, 64 neighbours per round
-    unsigned long long big = __ballot(cand && deg > PULL_WAVE_DEG);
-    while (big) {
-      const int src = __ffsll(big) - 1;
-      big &= big - 1;
-      const PtrT bb = __shfl(b, src, 64), ee = __shfl(e, src, 64);
-      bool hit = false;
-      uint32_t hp = 0;
-      for (PtrT base = bb; base < ee; base += 64) {
-        const PtrT k = base + lane;
-        uint32_t w = 0;
-        bool h = false;
-        if (k < ee) {
-          w = a.row_idx[k];
-          h = in_frontier(fbm_cur, w);
-        }
-        const unsigned long long hm = __ballot(h);
-        if (hm) {
-          hp = __shfl(w, __ffsll(hm) - 1, 64);
-          hit = true;
-          break;
-        }
-      }
-      if ((int)lane == src) {
-        found = hit;
-        par = hp;
-      }
-    }
-    const unsigned long long m = __ballot(found);
-    if (lane == 0) {
-      fbm_next[2 * g] = (uint32_t)m;
-      if ((uint32_t)m) a.vis[2 * g] = vword | (uint32_t)m;
-    } else if (lane == 32) {
-      fbm_next[2 * g + 1] = (uint32_t)(m >> 32);
-      if ((uint32_t)(m >> 32)) a.vis[2 * g + 1] = vword | (uint32_t)(m >> 32);
-    }
-    if (m) {
       if (found) {
         a.levels[u] = a.next_level;
         if (a.parents) a.parents[u] = par;
+        atomicOr(&fw[off >> 5], 1u << (off & 31));
+        found_deg += (unsigned long long)deg;
       }
-      n_found += (uint32_t)__popcll(m);
-      if (a.q_out_edges) found_deg += wave_sum(found ? deg : 0ull);
     }
+    wave_sync_lds();
+    const uint32_t f = fw[lane];
+    if (wi < n_words) {
+      fbm_next[wi] = f;
+      if (f) a.vis[wi] = vword | f;
+    }
+    n_found += __popc(f);
+    wave_sync_lds();  // fw is cleared by the next chunk
   }
-  // The next frontier is the bitmap fbm_next; only its size and degree sum
-  // are published, with one atomic per work-group (a queue is built from the
-  // bitmap only if a top-down step follows).
-  __shared__ uint32_t s_n[WG / 64];
-  __shared__ unsigned long long s_d[WG / 64];
+  // one atomic per work-group for each total
+  __shared__ uint32_t s_n[NW];
+  __shared__ unsigned long long s_d[NW], s_x[NW];
+  const uint32_t nf = wave_sum(n_found);
+  const unsigned long long fd = wave_sum(found_deg);
+  const unsigned long long sx = a.examined ? wave_sum(scanned) : 0ull;
   if (lane == 0) {
-    s_n[threadIdx.x >> 6] = n_found;
-    s_d[threadIdx.x >> 6] = found_deg;
+    s_n[wv] = nf;
+    s_d[wv] = fd;
+    s_x[wv] = sx;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t n = 0;
-    unsigned long long d = 0;
-    for (int w = 0; w < WG / 64; w++) {
+    unsigned long long d = 0, x = 0;
+    for (int w = 0; w < NW; w++) {
       n += s_n[w];
       d += s_d[w];
+      x += s_x[w];
     }
     if (n) atomicAdd(a.q_out_count, n);
     if (a.q_out_edges && d) atomicAdd(a.q_out_edges, d);
+    if (a.examined && x) atomicAdd(a.examined, x);
   }
 }
 
@@ -705,7 +831,7 @@ __global__ __launch_bounds__(COMPACT_WG) void k_compact_dev(
     uint32_t chunks_per_group, uint32_t *vis, int32_t *levels, LevelCtl *ctl, DevStep d,
     uint32_t *q_out, int track, uint32_t *fbm_next) {
   const uint32_t F = ctl->q_count[d.cur];
-  if (F == 0 || F < d.mark_min_F) return;
+  if (F == 0 || ctl->q_edges[d.cur] < d.mark_min_E) return;
   const uint32_t out = d.cur ^ 1;
   compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, chunks_per_group, vis, levels,
                      (int32_t)ctl->level + 1, q_out, &ctl->q_count[out],
@@ -890,6 +1016,7 @@ static KArgs kargs(const StepArgs &s, uint32_t hub) {
   a.next_level = s.next_level;
   a.hub_degree = hub;
   a.mark = s.mark;
+  a.examined = s.count_examined ? &s.ctl->examined : nullptr;
   return a;
 }
 
@@ -976,19 +1103,52 @@ void step_glb(bfsx_ctx *ctx, const StepArgs &s) {
   BFSX_HIP(hipGetLastError());
 }
 
+// Summary resolution for n vertices: the smallest shift >= 5 with at most
+// max_words * 32 summary bits.
+static uint32_t pull_summary_shift(uint64_t n, uint32_t max_words) {
+  uint32_t shift = 5;
+  while (((n + (1ull << shift) - 1) >> shift) > (uint64_t)max_words * 32) shift++;
+  return shift;
+}
+
 template <typename PtrT>
 static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
   const KArgs a = kargs(s, 0);
-  const uint32_t n_groups = (s.n_cols + 63) / 64;
   const uint32_t wg = ctx->params.pull_wg;
-  const unsigned grid = cap_grid(((uint64_t)n_groups * 64 + wg - 1) / wg,
-                                 (unsigned)ctx->cu_count * 32);
+  const uint32_t n_chunks = (s.n_rows + PULL_CHUNK - 1) / PULL_CHUNK;
   hipStream_t st = ctx->stream;
+  // The summary fills the LDS that the kernel's own arrays leave free (at
+  // most PULL_SUM_WORDS_MAX words), and pays when at most a quarter of its
+  // bits can be set.
+  const uint32_t static_lds = (wg / 64) * 64 * 4 + 1024;
+  uint32_t max_words = ctx->lds_per_block > static_lds ? (ctx->lds_per_block - static_lds) / 4 : 0;
+  max_words = max_words > PULL_SUM_WORDS_MAX ? PULL_SUM_WORDS_MAX : max_words;
+  const uint32_t shift = pull_summary_shift(s.n_cols, max_words ? max_words : 1);
+  const uint32_t sum_bits = (uint32_t)(((uint64_t)s.n_cols + (1ull << shift) - 1) >> shift);
+  const uint32_t sum_words = (sum_bits + 31) / 32;
+  const bool use_sum = max_words >= 64 && ((uint64_t)s.F << shift) * 4 <= (uint64_t)s.n_cols;
+  const uint32_t *g_sum = nullptr;
+  if (use_sum) {
+    ensure(ctx->fsum, sizeof(uint32_t) * PULL_SUM_WORDS_MAX, st);
+    const uint32_t n_words = (uint32_t)bitmap_words(s.n_cols);
+    const unsigned sgrid = cap_grid(((uint64_t)sum_bits + 255) / 256, (unsigned)ctx->cu_count * 4);
+    hipLaunchKernelGGL(k_fbm_summary, dim3(sgrid), dim3(256), 0, st, s.fbm_cur, n_words, shift,
+                       ctx->fsum.as<uint32_t>(), sum_bits);
+    g_sum = ctx->fsum.as<uint32_t>();
+  }
+  const size_t lds = use_sum ? sizeof(uint32_t) * sum_words : 0;
+  // resident work-groups per CU: at most 32 waves, and what LDS allows
+  const uint32_t nw = wg / 64;
+  uint32_t per_cu = 32 / nw;
+  const uint32_t by_lds = ctx->lds_per_cu / (uint32_t)(static_lds + lds);
+  per_cu = by_lds < per_cu ? (by_lds ? by_lds : 1) : per_cu;
+  const unsigned grid = cap_grid(((uint64_t)n_chunks + nw - 1) / nw,
+                                 (unsigned)ctx->cu_count * per_cu);
   switch (wg) {
-    case 64: hipLaunchKernelGGL((k_pull<PtrT, 64>), dim3(grid), dim3(64), 0, st, a, s.fbm_cur, s.fbm_next, n_groups); break;
-    case 128: hipLaunchKernelGGL((k_pull<PtrT, 128>), dim3(grid), dim3(128), 0, st, a, s.fbm_cur, s.fbm_next, n_groups); break;
-    case 256: hipLaunchKernelGGL((k_pull<PtrT, 256>), dim3(grid), dim3(256), 0, st, a, s.fbm_cur, s.fbm_next, n_groups); break;
-    case 512: hipLaunchKernelGGL((k_pull<PtrT, 512>), dim3(grid), dim3(512), 0, st, a, s.fbm_cur, s.fbm_next, n_groups); break;
+    case 64: hipLaunchKernelGGL((k_pull<PtrT, 64>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
+    case 128: hipLaunchKernelGGL((k_pull<PtrT, 128>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
+    case 256: hipLaunchKernelGGL((k_pull<PtrT, 256>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
+    case 512: hipLaunchKernelGGL((k_pull<PtrT, 512>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
     default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
   }
 }
@@ -1088,7 +1248,8 @@ void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, 
 // One device-driven top-down level (LLB or SLB, or the choice in the control
 // block when fixed < 0) for a captured loop: step, compaction, bookkeeping.
 template <typename PtrT>
-static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, uint32_t mark_min_F) {
+static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
+                        unsigned long long mark_min_E) {
   KArgs a{};
   a.col_ptr = ctx->col_ptr.p;
   a.row_idx = ctx->row_idx.as<uint32_t>();
@@ -1104,7 +1265,7 @@ static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, uint
   a.parents = nullptr;
   a.hub_degree = ctx->params.hub_degree;
   a.mark = ctx->mark.as<uint8_t>();
-  const DevStep d{cur, mark_min_F, fixed};
+  const DevStep d{cur, fixed, mark_min_E};
   const uint32_t wg = ctx->params.llb_wg;
   const unsigned grid = cap_grid(((uint64_t)ctx->n_cols + wg - 1) / wg, (unsigned)ctx->cu_count * 8);
   hipStream_t s = ctx->stream;
@@ -1125,9 +1286,10 @@ static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, uint
   BFSX_HIP(hipGetLastError());
 }
 
-void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, uint32_t mark_min_F) {
-  if (ctx->ptr_bytes == 8) dev_level_t<uint64_t>(ctx, cur, fixed, track, mark_min_F);
-  else dev_level_t<uint32_t>(ctx, cur, fixed, track, mark_min_F);
+void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
+               unsigned long long mark_min_E) {
+  if (ctx->ptr_bytes == 8) dev_level_t<uint64_t>(ctx, cur, fixed, track, mark_min_E);
+  else dev_level_t<uint32_t>(ctx, cur, fixed, track, mark_min_E);
 }
 
 void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out) {

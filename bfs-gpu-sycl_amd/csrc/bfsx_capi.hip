@@ -50,7 +50,7 @@ static void free_search_state(bfsx_ctx *ctx) {
   drop_loop_graph(ctx);  // the captured loop holds pointers into this state
   for (DevBuf *b : {&ctx->levels, &ctx->parents, &ctx->vis, &ctx->fbm[0], &ctx->fbm[1],
                     &ctx->queue[0], &ctx->queue[1], &ctx->glb_scan, &ctx->glb_start,
-                    &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark, &ctx->ms_seen,
+                    &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark, &ctx->fsum, &ctx->ms_seen,
                     &ctx->ms_cur, &ctx->ms_nxt, &ctx->ms_lv, &ctx->ms_hubs})
     release(*b);
 }
@@ -71,6 +71,7 @@ static void ensure_pull_state(bfsx_ctx *ctx) {
   const uint64_t w = bitmap_words(ctx->n_cols);
   ensure(ctx->fbm[0], sizeof(uint32_t) * w, ctx->stream);
   ensure(ctx->fbm[1], sizeof(uint32_t) * w, ctx->stream);
+  ensure(ctx->fsum, sizeof(uint32_t) * bfsx::PULL_SUM_WORDS_MAX, ctx->stream);
 }
 
 static void after_graph_change(bfsx_ctx *ctx) {
@@ -86,7 +87,7 @@ static bfsx_params default_params() {
   p.glb_blocks_per_cu = 8;
   p.glb_chunk = 4096;
   p.hub_degree = 0;
-  p.pull_wg = 256;
+  p.pull_wg = 512;
   p.graph_levels = 8;
   p.mark_ratio_q10 = 64;
   return p;
@@ -119,12 +120,13 @@ static void read_ctl(bfsx_ctx *ctx) {
   BFSX_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-static bool mark_step(const bfsx_ctx *ctx, uint32_t F) {
+// Top-down levels whose frontier has at least this many edges mark and
+// compact; thinner ones discover exact-once with atomics.
+static unsigned long long mark_min_edges(const bfsx_ctx *ctx) {
   const uint32_t r = ctx->params.mark_ratio_q10;
-  if (r == 0) return true;
-  if (r == 0xFFFFFFFFu) return false;
-  const double avg_deg = ctx->n_rows ? (double)ctx->nnz / ctx->n_rows : 0.0;
-  return (double)F * avg_deg * 1024.0 >= (double)ctx->n_cols * r;
+  if (r == 0) return 0;
+  if (r == 0xFFFFFFFFu) return ~0ull;
+  return ((unsigned long long)ctx->n_cols * r + 1023) / 1024;
 }
 
 static bool graph_symmetric(const bfsx_ctx *ctx) {
@@ -136,12 +138,14 @@ static bool graph_symmetric(const bfsx_ctx *ctx) {
 struct Walker {
   bfsx_ctx *ctx;
   uint32_t F = 1, cur = 0, level = 0;
+  unsigned long long EF = 0;  // frontier edges (kept by the kernels: track is on)
   int fbm_cur = 0;         // buffer holding the current frontier's bitmap
   bool fbm_valid = false;  // whether that bitmap is up to date
   bool queue_valid = true; // false after a bottom-up step (frontier only as bitmap)
   bool pull_possible = false;
   bool track = false;      // accumulate frontier degree sums
   bool stamp = false;      // device clock around each step
+  bool count = false;      // count adjacency entries read by bottom-up steps
   uint32_t *par = nullptr;
   int last = BFSX_LLB;     // step kind of the last launch
 
@@ -165,7 +169,8 @@ struct Walker {
     a.ctl = ctx->ctl.as<LevelCtl>();
     a.next_level = (int32_t)level + 1;
     a.track_degrees = track;
-    const bool use_mark = choice != BFSX_PULL && mark_step(ctx, F);
+    a.count_examined = count;
+    const bool use_mark = choice != BFSX_PULL && EF >= mark_min_edges(ctx);
     a.mark = use_mark ? ctx->mark.as<uint8_t>() : nullptr;
     if (stamp) level_begin(ctx);
     if (choice != BFSX_PULL && !queue_valid)
@@ -204,6 +209,7 @@ struct Walker {
     fbm_valid = new_fbm;
     queue_valid = last != BFSX_PULL;
     F = ctx->h_ctl->q_count[cur ^ 1];
+    EF = ctx->h_ctl->q_edges[cur ^ 1];
     cur ^= 1;
     level++;
   }
@@ -211,8 +217,11 @@ struct Walker {
 
 static uint32_t allow_mask(const Walker &w) { return 0x7u | (w.pull_possible ? 0x8u : 0u); }
 
-static Walker start_search(bfsx_ctx *ctx, uint32_t root, bool pull_possible, bool track,
-                           bool stamp, uint32_t *par) {
+// Initial state of a search (levels, visited bitmap, queue, control block),
+// read back so that the root's degree is known; frontier degree sums are
+// always kept, since they decide between mark and exact-once discovery.
+static Walker start_search(bfsx_ctx *ctx, uint32_t root, bool pull_possible, bool stamp,
+                           uint32_t *par) {
   if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
   if (root >= ctx->n_cols) fail(BFSX_ERR_ARG, "root out of range");
   hipStream_t s = ctx->stream;
@@ -222,28 +231,20 @@ static Walker start_search(bfsx_ctx *ctx, uint32_t root, bool pull_possible, boo
     ctx->mark_dirty = false;
   }
   init_search(ctx, root, graph_symmetric(ctx), par);
+  read_ctl(ctx);
   Walker w;
   w.ctx = ctx;
   w.pull_possible = pull_possible;
-  w.track = track;
+  w.track = true;
   w.stamp = stamp;
   w.par = par;
+  w.EF = ctx->h_ctl->q_edges[0];
   return w;
 }
 
 // ---------------------------------------------------------------------------
 // captured level loop: K device-driven levels per hipGraph launch
 // ---------------------------------------------------------------------------
-static uint32_t mark_min_frontier(const bfsx_ctx *ctx) {
-  const uint32_t r = ctx->params.mark_ratio_q10;
-  if (r == 0) return 0;
-  if (r == 0xFFFFFFFFu) return 0xFFFFFFFFu;
-  const double avg_deg = ctx->n_rows ? (double)ctx->nnz / ctx->n_rows : 0.0;
-  if (avg_deg <= 0) return 0xFFFFFFFFu;
-  const double f = std::ceil((double)ctx->n_cols * r / (1024.0 * avg_deg));
-  return f >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)f;
-}
-
 void drop_loop_graph(bfsx_ctx *ctx) {
   if (ctx->loop_exec) (void)hipGraphExecDestroy(ctx->loop_exec);
   if (ctx->loop_graph) (void)hipGraphDestroy(ctx->loop_graph);
@@ -255,11 +256,11 @@ void drop_loop_graph(bfsx_ctx *ctx) {
 // fixed = BFSX_LLB / BFSX_SLB, or -1 for the scheduler's choice between them
 static hipGraphExec_t loop_graph(bfsx_ctx *ctx, int fixed, bool learned) {
   const uint32_t K = (ctx->params.graph_levels + 1) & ~1u;  // even: buffers return to 0
-  const uint32_t mmf = mark_min_frontier(ctx);
+  const unsigned long long mme = mark_min_edges(ctx);
   const uint64_t key = 1 + ((uint64_t)(fixed + 1) | ((uint64_t)learned << 3) |
                             ((uint64_t)K << 4) | ((uint64_t)ctx->params.llb_wg << 12) |
                             ((uint64_t)ctx->params.hub_degree << 24)) ^
-                       ((uint64_t)mmf * 0x9E3779B97F4A7C15ull);
+                       ((uint64_t)mme * 0x9E3779B97F4A7C15ull);
   if (ctx->loop_exec && ctx->loop_key == key) return ctx->loop_exec;
   drop_loop_graph(ctx);
   hipStream_t s = ctx->stream;
@@ -268,7 +269,7 @@ static hipGraphExec_t loop_graph(bfsx_ctx *ctx, int fixed, bool learned) {
     for (uint32_t k = 0; k < K; k++) {
       const uint32_t cur = k & 1u;
       if (learned) level_begin(ctx);
-      dev_level(ctx, cur, fixed, learned, mmf);
+      dev_level(ctx, cur, fixed, true, mme);
       level_end(ctx, cur, cur ^ 1, learned, fixed);
       if (learned) decide(ctx, cur ^ 1, false, (1u << BFSX_LLB) | (1u << BFSX_SLB));
     }
@@ -286,7 +287,7 @@ static hipGraphExec_t loop_graph(bfsx_ctx *ctx, int fixed, bool learned) {
 
 static RunOut search_graph_loop(bfsx_ctx *ctx, uint32_t root, int fixed, bool learned) {
   hipStream_t s = ctx->stream;
-  start_search(ctx, root, false, learned, learned, nullptr);
+  start_search(ctx, root, false, learned, nullptr);
   hipGraphExec_t exec = loop_graph(ctx, learned ? -1 : (fixed < 0 ? BFSX_LLB : fixed), learned);
   BFSX_HIP(hipStreamSynchronize(s));
   ctx->trace.clear();
@@ -333,14 +334,14 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   if (want_parents) ensure(ctx->parents, sizeof(uint32_t) * ctx->n_cols, s);
   uint32_t *par = want_parents ? ctx->parents.as<uint32_t>() : nullptr;
   // initial state outside the timer (host.cpp:113-121 / run_bfs_gpu.hpp:674-677)
-  Walker w = start_search(ctx, root, fixed == BFSX_PULL || (learned && symmetric),
-                          trace || learned, learned, par);
+  Walker w = start_search(ctx, root, fixed == BFSX_PULL || (learned && symmetric), false, par);
+  w.count = timed;
   BFSX_HIP(hipStreamSynchronize(s));
   ctx->trace.clear();
 
   RunOut out;
   BFSX_HIP(hipEventRecord(ctx->ev0, s));
-  uint64_t f_edges = 0, visited_v = 1, visited_e = 0;
+  uint64_t f_edges = 0, visited_v = 1, visited_e = 0, examined = 0;
   if (learned) decide(ctx, 0, true, allow_mask(w));
   if (trace || learned) {
     read_ctl(ctx);
@@ -362,6 +363,8 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
       r.choice = (uint32_t)choice;
       r.frontier_vertices = F;
       r.discovered = w.F;
+      r.pull_scanned = ctx->h_ctl->examined - examined;
+      examined = ctx->h_ctl->examined;
       if (trace) {
         r.frontier_edges = f_edges;
         r.unvisited_vertices = ctx->n_cols - visited_v;
@@ -434,7 +437,7 @@ static uint32_t record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, u
   };
   uint32_t rows = 0;
   for (uint32_t ri = 0; ri < n_roots && rows < max_rows; ri++) {
-    Walker w = start_search(ctx, roots[ri], symmetric, true, true, nullptr);
+    Walker w = start_search(ctx, roots[ri], symmetric, true, nullptr);
     const uint32_t allow = allow_mask(w);
     decide(ctx, 0, true, allow);
     read_ctl(ctx);
@@ -576,6 +579,9 @@ int bfsx_create(int device, bfsx_ctx **out) {
     hipDeviceProp_t prop;
     BFSX_HIP(hipGetDeviceProperties(&prop, device));
     ctx->cu_count = prop.multiProcessorCount;
+    ctx->lds_per_block = (uint32_t)prop.sharedMemPerBlock;
+    if (prop.maxSharedMemoryPerMultiProcessor > 0)
+      ctx->lds_per_cu = (uint32_t)prop.maxSharedMemoryPerMultiProcessor;
     ctx->name = prop.gcnArchName;
     ctx->hbm_bytes = prop.totalGlobalMem;
     int khz = 0;

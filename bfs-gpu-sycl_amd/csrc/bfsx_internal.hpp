@@ -33,11 +33,15 @@ struct LevelCtl {
   float sched_in[8];               // scheduler inputs of the last decision
   uint32_t kind_count[4];          // levels expanded by each step kind
   uint32_t conv;                   // slot cursor of a bitmap-to-queue conversion
+  unsigned long long examined;     // adjacency entries read by bottom-up steps (running)
 };
 
 // Bitmaps hold whole 1024-vertex chunks (32 words) so that a wave can own a
 // chunk; mark arrays hold one byte per vertex, padded the same way.
 __host__ __device__ inline uint64_t bitmap_words(uint64_t n) { return ((n + 1023) / 1024) * 32; }
+
+// Largest frontier summary of the bottom-up step (words; 32 KiB).
+constexpr uint32_t PULL_SUM_WORDS_MAX = 8192;
 __host__ __device__ inline uint64_t mark_bytes(uint64_t n) { return ((n + 1023) / 1024) * 1024; }
 
 // Growable device allocation.
@@ -53,6 +57,8 @@ struct bfsx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int cu_count = 0;
+  uint32_t lds_per_block = 65536;  // LDS bytes one work-group may allocate
+  uint32_t lds_per_cu = 65536;     // LDS bytes of one CU
   int wall_clock_khz = 100000;
   std::string name;
   uint64_t hbm_bytes = 0;
@@ -70,6 +76,7 @@ struct bfsx_ctx {
   // per-search state, sized for the resident graph
   bfsx::DevBuf levels, parents, vis, fbm[2], queue[2], ctl, mark;
   bfsx::DevBuf glb_scan, glb_start, glb_tile, glb_chunk_lo;
+  bfsx::DevBuf fsum;     // bottom-up step: frontier summary bitmap (PULL_SUM_WORDS_MAX)
   bfsx::DevBuf scratch;  // generic scratch (reductions, root picking)
   bfsx::LevelCtl *h_ctl = nullptr;  // pinned mirror
   bool mark_dirty = false;          // a mark step did not reach its compaction
@@ -141,6 +148,7 @@ struct StepArgs {
   int32_t next_level;
   bool track_degrees;      // accumulate degree sum of discovered vertices
   uint8_t *mark;           // non-null: mark-then-compact step
+  bool count_examined;     // bottom-up: add adjacency entries read to ctl->examined
 };
 
 void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *parents);
@@ -153,7 +161,10 @@ void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level,
 void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next);
 void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
                bool stamp, int kind);
-void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, uint32_t mark_min_F);
+// mark_min_E: top-down levels whose frontier has at least this many edges
+// mark and compact (needs track: the edge counts are kept in the control block)
+void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
+               unsigned long long mark_min_E);
 void level_begin(bfsx_ctx *ctx);
 void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out);
 void level_histogram(bfsx_ctx *ctx, uint32_t max_levels, unsigned long long *d_counts,

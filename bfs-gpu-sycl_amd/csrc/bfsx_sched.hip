@@ -30,13 +30,17 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + H1 * IN, OFF_W2 = OFF_B1 + H1,
 static_assert(NPARAM == BFSX_SCHED_PARAMS, "parameter count");
 
 __device__ __forceinline__ float flog2p1(double v) { return (float)log2(1.0 + (v > 0 ? v : 0.0)); }
+// log2((1 + a) / (1 + b)) for a, b >= 0
+__device__ __forceinline__ float flog2ratio(double a, double b) {
+  return (float)(log2(1.0 + (a > 0 ? a : 0.0)) - log2(1.0 + (b > 0 ? b : 0.0)));
+}
 
 // ---------------------------------------------------------------------------
 // inference inside the level loop (one wave)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_decide(LevelCtl *ctl, const float *Wg, uint32_t idx,
                                                uint64_t n_cols, uint64_t nnz, uint64_t n_iso,
-                                               uint32_t khz, uint32_t allow_mask, int first) {
+                                               uint32_t allow_mask, int first) {
   __shared__ float W[NPARAM];
   __shared__ float x[IN], h1[H1], h2[H2], y[OUT];
   const int l = threadIdx.x;
@@ -49,18 +53,18 @@ __global__ __launch_bounds__(64) void k_decide(LevelCtl *ctl, const float *Wg, u
     const double UV = vv < (double)n_cols ? (double)n_cols - vv : 0.0;
     const double ve = (double)ctl->visited_edges;
     const double UE = ve < (double)nnz ? (double)nnz - ve : 0.0;
-    const uint32_t pc = first ? 0u : ctl->choice;
-    double us = 0.0;
-    if (!first && ctl->t_end > ctl->t_begin)
-      us = (double)(ctl->t_end - ctl->t_begin) * 1000.0 / (double)khz;
-    x[0] = flog2p1(F) / 32.f;
-    x[1] = flog2p1(EF) / 32.f;
-    x[2] = flog2p1(UV) / 32.f;
-    x[3] = flog2p1(UE) / 32.f;
-    x[4] = (float)ctl->level / 64.f;
-    x[5] = (float)(pc & 1u);
-    x[6] = (float)(pc >> 1);
-    x[7] = flog2p1(us) / 16.f;
+    const double n = (double)n_cols, m = (double)nnz;
+    // size-free inputs: fractions of the graph on a log scale, mean degrees,
+    // plus the graph size for fixed per-launch costs
+    x[0] = flog2ratio(F, n) / 32.f;
+    x[1] = flog2ratio(EF, m) / 32.f;
+    x[2] = flog2ratio(UV, n) / 32.f;
+    x[3] = flog2ratio(UE, m) / 32.f;
+    x[4] = flog2p1(F > 0 ? EF / F : 0.0) / 16.f;
+    x[5] = flog2p1(n > 0 ? m / n : 0.0) / 16.f;
+    x[6] = flog2p1(n) / 32.f;
+    // the frontier exists only as a bitmap after a bottom-up step
+    x[7] = (!first && ctl->choice == BFSX_PULL) ? 1.f : 0.f;
   }
   __syncthreads();
   if (l < H1) {
@@ -105,7 +109,7 @@ void decide(bfsx_ctx *ctx, uint32_t idx, bool first, uint32_t allow_mask) {
                              : 0;
   hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, ctx->stream, ctx->ctl.as<LevelCtl>(),
                      ctx->sched_w.as<float>(), idx, (uint64_t)ctx->n_cols, (uint64_t)ctx->nnz,
-                     n_iso, (uint32_t)ctx->wall_clock_khz, allow_mask, first ? 1 : 0);
+                     n_iso, allow_mask, first ? 1 : 0);
   BFSX_HIP(hipGetLastError());
 }
 

@@ -1,15 +1,19 @@
 #!/usr/bin/env python3
 """Trains the per-level scheduler MLP on the GPU and writes sched_mlp.json.
 
-1. For each training workload (R-MAT scale 19, the 4096 x 4096 grid, R-MAT
-   scale 22), generate the graph in HBM and record, for every level of
-   searches from training roots, the scheduler inputs and the device time of
-   every step kind from the same saved state (bfsx_sched_record).  Training
-   roots use a different seed from the workloads' evaluation roots.
-2. Fit the 8 -> 32 -> 32 -> 4 network to log2(step microseconds) with the
-   engine's GPU forward/backward/Adam kernels (bfsx_mlp_train).
-3. Evaluate learned vs fixed LLB (harmonic GTEPS) on each workload's own
-   roots, and save weights plus metadata.
+1. For each training graph (R-MAT scales 16..24 with edge factors 8..32 and
+   two grids, none of them an evaluation workload), generate it in HBM and
+   record, for every level of searches from its roots, the scheduler inputs
+   and the device time of every step kind from the same saved state
+   (bfsx_sched_record).
+2. Fit the 8 -> 32 -> 32 -> 4 network with the engine's GPU
+   forward/backward/Adam kernels (bfsx_mlp_train).  Targets are log2 of each
+   kind's step time relative to LLB's on the same level (the scheduler runs
+   the argmin, which only needs the order); rows are weighted by how much
+   time the level can cost, so that the slow levels of large graphs decide
+   the fit.
+3. Evaluate learned vs fixed LLB (harmonic GTEPS) on evaluation workloads
+   with their own roots, and save weights plus metadata.
 """
 import argparse
 import json
@@ -28,9 +32,11 @@ import numpy as np  # noqa: E402
 import bfsx  # noqa: E402
 import workloads  # noqa: E402
 
-FEATURES = ["log2(1+frontier vertices)/32", "log2(1+frontier edges)/32",
-            "log2(1+unvisited vertices)/32", "log2(1+unvisited edges)/32", "level/64",
-            "previous kind & 1", "previous kind >> 1", "log2(1+previous step us)/16"]
+FEATURES = ["log2((1+frontier vertices)/(1+n))/32", "log2((1+frontier edges)/(1+nnz))/32",
+            "log2((1+unvisited vertices)/(1+n))/32", "log2((1+unvisited edges)/(1+nnz))/32",
+            "log2(1+frontier edges/frontier vertices)/16", "log2(1+nnz/n)/16", "log2(1+n)/32",
+            "1 if the last step was bottom-up"]
+TRAIN_SETS = ",".join(workloads.TRAIN_WORKLOADS)
 
 
 def log(*a):
@@ -57,12 +63,23 @@ def evaluate(eng, roots, reps=3):
     return out
 
 
+def targets_and_weights(T, M):
+    """Relative log2 targets (LLB = 0) and per-row weights.
+
+    A row's weight is log2(1 + best step time / 8 us), at least 1/8: levels
+    that take milliseconds count more than levels that take microseconds.
+    """
+    Tr = np.where(M > 0, T - T[:, :1], 0.0).astype(np.float32)
+    best = np.where(M > 0, T, np.inf).min(axis=1)
+    w = np.maximum(np.log2(1.0 + 2.0 ** best / 8.0), 0.125)
+    return Tr, (M * w[:, None]).astype(np.float32)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "sched_mlp.json"))
-    ap.add_argument("--datasets", default="rmat19,grid4096,rmat22")
-    ap.add_argument("--roots", type=int, default=32, help="training roots per R-MAT workload")
-    ap.add_argument("--grid-roots", type=int, default=6)
+    ap.add_argument("--datasets", default=TRAIN_SETS)
+    ap.add_argument("--eval", default="rmat19,grid4096,rmat22")
     ap.add_argument("--repeats", type=int, default=2)
     ap.add_argument("--epsilon", type=float, default=0.25)
     ap.add_argument("--epochs", type=int, default=4000)
@@ -78,8 +95,7 @@ def main():
     for k, name in enumerate(names):
         t0 = time.time()
         w, nnz = workloads.build(eng, name)
-        n_train = args.grid_roots if w["kind"] == "grid" else args.roots
-        roots = eng.pick_roots(n_train, w["seed_roots"] + 100, 1 if w["kind"] == "grid" else 0)
+        roots = workloads.roots(eng, w)
         X, T, M = eng.sched_record(roots, repeats=args.repeats, epsilon=args.epsilon,
                                    seed=args.seed + k)
         Xs.append(X); Ts.append(T); Ms.append(M); tags += [name] * len(X)
@@ -90,19 +106,26 @@ def main():
     X = np.concatenate(Xs); T = np.concatenate(Ts); M = np.concatenate(Ms)
     if args.save_data:
         np.savez(args.save_data, X=X, T=T, M=M, tags=np.array(tags))
+    Tr, Mw = targets_and_weights(T, M)
     W0 = init_weights(args.seed)
     t0 = time.time()
-    W, curve = eng.mlp_train(X, T, M, W0, args.epochs, args.lr)
+    W, curve = eng.mlp_train(X, Tr, Mw, W0, args.epochs, args.lr)
     log(f"trained {len(X)} rows, {args.epochs} epochs in {time.time() - t0:.1f}s: "
         f"loss {curve[0]:.3f} -> {curve[-1]:.4f}")
     pred = eng.mlp_forward(X, W)
     big = np.where(M > 0, pred, np.inf).argmin(axis=1)
     best = np.where(M > 0, T, np.inf).argmin(axis=1)
     log(f"picks the fastest kind on {np.mean(big == best):.3f} of recorded levels")
+    us = np.where(M > 0, 2.0 ** T, np.inf)
+    rows = np.arange(len(X))
+    for name in names:
+        sel = np.array(tags) == name
+        log(f"  {name}: recorded us  llb {us[sel, 0].sum():.0f}  picked "
+            f"{us[rows[sel], big[sel]].sum():.0f}  best {us[rows[sel], best[sel]].sum():.0f}")
 
     eng.set_scheduler_weights(W)
     evals = {}
-    for name in names:
+    for name in [e for e in args.eval.split(",") if e]:
         w, _ = workloads.build(eng, name)
         roots = workloads.roots(eng, w)[:args.eval_roots]
         evals[name] = evaluate(eng, roots)
@@ -110,7 +133,8 @@ def main():
     meta = {
         "format": "bfsx scheduler MLP 8-32-32-4 ReLU, float32, row-major W1,b1,W2,b2,W3,b3",
         "inputs": FEATURES,
-        "outputs": "log2(device microseconds) of LLB, GLB, SLB, PULL; the smallest is run",
+        "outputs": "log2 of the step time of LLB, GLB, SLB, PULL relative to LLB; the smallest "
+                   "allowed one is run",
         "train_workloads": names,
         "rows": int(len(X)),
         "epochs": args.epochs,

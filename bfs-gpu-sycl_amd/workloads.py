@@ -23,9 +23,23 @@ WORKLOADS = {
 }
 
 
+# Graphs the per-level scheduler is trained on (seeds 7000+k / 7100+k /
+# 7200+k); none of them is an evaluation workload above.
+TRAIN_WORKLOADS = {}
+for _k, (_scale, _ef) in enumerate([(16, 16), (18, 16), (20, 8), (20, 16), (20, 32), (22, 16),
+                                     (23, 16), (24, 16)]):
+    TRAIN_WORKLOADS[f"train_rmat{_scale}_ef{_ef}"] = dict(
+        kind="rmat", scale=_scale, edge_factor=_ef, seed_graph=7000 + _k, seed_perm=7100 + _k,
+        seed_roots=7200 + _k, n_roots=16, ptr_bytes=4)
+for _k, _side in enumerate([1024, 2048, 3072, 4096]):
+    TRAIN_WORKLOADS[f"train_grid{_side}"] = dict(
+        kind="grid", side=_side, drop_prob=0.05, n_long=round(1e-4 * 2 * _side * (_side - 1)),
+        seed_graph=7050 + _k, seed_roots=7250 + _k, n_roots=4, ptr_bytes=4)
+
+
 def build(engine, name, **override):
     """Generates workload `name` in HBM; returns (config dict, nnz)."""
-    w = dict(WORKLOADS[name], **override)
+    w = dict(WORKLOADS[name] if name in WORKLOADS else TRAIN_WORKLOADS[name], **override)
     if w["kind"] == "rmat":
         a, b, c = RMAT_ABC
         nnz = engine.generate_rmat(w["scale"], w["edge_factor"], a, b, c, w["seed_graph"],

@@ -100,6 +100,7 @@ typedef struct bfsx_level_record {
   uint64_t unvisited_edges;
   uint64_t discovered;      /* vertices found by this step */
   double us;                /* device time of this step */
+  uint64_t pull_scanned;    /* adjacency entries read by a bottom-up step */
 } bfsx_level_record;
 
 /* launch-shape parameters (the SM_FACTOR / work-group knobs) */
@@ -112,8 +113,9 @@ typedef struct bfsx_params {
                                degree are expanded by a whole wave (0 = off) */
   uint32_t pull_wg;         /* work-group size of the bottom-up step */
   uint32_t graph_levels;    /* levels per captured hipGraph launch */
-  uint32_t mark_ratio_q10;  /* top-down steps whose estimated frontier edges
-                               reach n_cols * mark_ratio_q10 / 1024 mark
+  uint32_t mark_ratio_q10;  /* top-down steps whose frontier edges (exact
+                               out-degree sum, kept on the device) reach
+                               n_cols * mark_ratio_q10 / 1024 mark
                                neighbours with plain byte stores and compact
                                afterwards; smaller ones discover exact-once
                                with atomics (0 = always mark,
@@ -167,7 +169,7 @@ int bfsx_run_batch(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots,
                    bfsx_run_stats *stats);
 /* Per-level records of the last run made with BFSX_RUN_TRACE or
  * BFSX_RUN_STEP_TIMES (the latter fills level, choice, frontier_vertices,
- * discovered and us). */
+ * discovered, us and pull_scanned). */
 int bfsx_last_trace(bfsx_ctx *ctx, bfsx_level_record *out, uint32_t max_records,
                     uint32_t *n_records);
 /* Level histogram of the last run, computed on the device: counts[l] =

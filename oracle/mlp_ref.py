@@ -81,8 +81,14 @@ def init_weights(seed):
     return pack(p).astype(np.float32)
 
 
-def features(F, EF, UV, UE, level, prev_choice, prev_us):
-    """The scheduler inputs as the engine computes them on the device."""
+def features(F, EF, UV, UE, n, nnz, after_pull):
+    """The scheduler inputs as the engine computes them on the device.
+
+    F/EF: frontier vertices/edges, UV/UE: unvisited vertices/edges, n/nnz:
+    graph vertices/edges, after_pull: the last step was bottom-up.
+    """
     lg = lambda v: np.log2(1.0 + max(float(v), 0.0))
-    return np.array([lg(F) / 32, lg(EF) / 32, lg(UV) / 32, lg(UE) / 32, level / 64,
-                     prev_choice & 1, prev_choice >> 1, lg(prev_us) / 16], np.float64)
+    return np.array([(lg(F) - lg(n)) / 32, (lg(EF) - lg(nnz)) / 32, (lg(UV) - lg(n)) / 32,
+                     (lg(UE) - lg(nnz)) / 32, lg(EF / F if F > 0 else 0.0) / 16,
+                     lg(nnz / n if n > 0 else 0.0) / 16, lg(n) / 32,
+                     1.0 if after_pull else 0.0], np.float64)

@@ -266,6 +266,19 @@ def test_level_histogram_and_step_times(engine):
         assert (edges == np.array([deg[want == k].sum() for k in range(len(hist))])).all()
         assert [t["frontier_vertices"] for t in tr] == list(hist)
         assert all(t["us"] > 0 for t in tr)
+        if s != "pull":
+            assert all(t["pull_scanned"] == 0 for t in tr)
+            continue
+        # a bottom-up level reads every list of a candidate that stays
+        # undiscovered, at least one entry of each discovered one, and never
+        # more than the candidates' lists
+        later = lambda k: (want > k) | (want < 0)
+        for t in tr:
+            k = t["level"]
+            d = int(hist[k + 1]) if k + 1 < len(hist) else 0
+            lo = int(deg[later(k + 1)].sum()) + d
+            hi = int(deg[later(k)].sum())
+            assert lo <= t["pull_scanned"] <= hi, (k, lo, t["pull_scanned"], hi)
 
 
 def test_stream_copy(engine):

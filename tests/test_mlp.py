@@ -75,8 +75,11 @@ def test_planted_gradient_mistake_is_caught():
 
 
 def test_feature_formulas():
-    x = mlp_ref.features(1, 255, 1023, 0, 3, 2, 0)
-    assert np.allclose(x, [1 / 32, 8 / 32, 10 / 32, 0, 3 / 64, 0, 1, 0])
+    # n = 1021, nnz = 3063 (mean degree 3), frontier 3 vertices of degree 255
+    x = mlp_ref.features(3, 765, 1021, 765, 1021, 3063, True)
+    l1022 = np.log2(1022.0)
+    assert np.allclose(x, [(2 - l1022) / 32, -2 / 32, 0, -2 / 32, 8 / 16, 2 / 16, l1022 / 32, 1])
+    assert mlp_ref.features(0, 0, 0, 0, 0, 0, False).tolist() == [0] * 8
 
 
 # ---------------------------------------------------------------------------
@@ -169,5 +172,5 @@ def test_record_rows(engine):
     deg = np.diff(cp.astype(np.int64))
     r0 = int(roots[0])
     want = mlp_ref.features(1, deg[r0], info["n_cols"] - info["n_isolated"] - 1,
-                            info["nnz"] - deg[r0], 0, 0, 0)
+                            info["nnz"] - deg[r0], info["n_cols"], info["nnz"], False)
     np.testing.assert_allclose(X[0], want, rtol=1e-6, atol=1e-6)
