@@ -71,9 +71,9 @@ KERNEL_NAMES = {"llb": "k_llb", "slb": "k_llb (strided)", "glb": "k_glb_expand +
 
 
 def roofline_leg(eng, roots, strategy, info):
-    """Per-level step times (HIP events around each step's kernels on the
-    engine stream) and level histograms -> algorithmic bytes / time of the
-    dominant step kind."""
+    """Per-level kernel times (HIP events around each step's main kernel on
+    the engine stream) and level histograms -> algorithmic bytes / time of
+    the dominant step kind's kernel."""
     import bfsx
     w_ptr = info["col_ptr_bytes"]
     n = float(info["n_cols"])
@@ -81,6 +81,7 @@ def roofline_leg(eng, roots, strategy, info):
     pull = bfsx.STEP_NAMES.index("pull")
     per_kind_bytes = np.zeros(4)
     per_kind_us = np.zeros(4)
+    per_kind_step_us = np.zeros(4)
     per_kind_launches = np.zeros(4)
     for r in roots:
         eng.run(int(r), strategy, levels=False, step_times=True)
@@ -98,7 +99,8 @@ def roofline_leg(eng, roots, strategy, info):
                                                 D, w_ptr)
             else:
                 per_kind_bytes[k] += algorithmic_bytes(F, E, D, w_ptr)
-            per_kind_us[k] += rec["us"]
+            per_kind_us[k] += rec["kernel_us"]
+            per_kind_step_us[k] += rec["us"]
             per_kind_launches[k] += 1
     k = int(np.argmax(per_kind_us))
     achieved = per_kind_bytes[k] / (per_kind_us[k] * 1e-6) / 1e9
@@ -114,22 +116,26 @@ def roofline_leg(eng, roots, strategy, info):
         "traffic": None,
         "launches": int(per_kind_launches[k]),
         "avg_launch_us": round(per_kind_us[k] / max(per_kind_launches[k], 1), 2),
+        "avg_step_us": round(per_kind_step_us[k] / max(per_kind_launches[k], 1), 2),
         "alg_bytes_per_launch": round(per_kind_bytes[k] / max(per_kind_launches[k], 1), 1),
     }
 
 
 def traffic_from_profiles(workload, step):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    scaled by the copy-kernel calibration, if a summary for this workload and
-    step kind is committed under profiles/."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{step}_{workload}.json")
+    scaled by the copy-kernel calibration: the `hbm_bytes_per_launch:` line of
+    profiles/traffic_<step>_<workload>.md, if that summary is committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{step}_{workload}.md")
     if not os.path.exists(path):
         return None
-    try:
-        d = json.load(open(path))
-        return d.get("hbm_bytes_per_launch")
-    except Exception:
-        return None
+    for line in open(path):
+        line = line.strip().strip("`")
+        if line.startswith("hbm_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(eng, roots, gpu_levels_of, budget_s=30.0):

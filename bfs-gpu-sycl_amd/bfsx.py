@@ -60,7 +60,7 @@ class LevelRecord(C.Structure):
                 ("frontier_vertices", C.c_uint64), ("frontier_edges", C.c_uint64),
                 ("unvisited_vertices", C.c_uint64), ("unvisited_edges", C.c_uint64),
                 ("discovered", C.c_uint64), ("us", C.c_double),
-                ("pull_scanned", C.c_uint64)]
+                ("pull_scanned", C.c_uint64), ("kernel_us", C.c_double)]
 
 
 class Params(C.Structure):

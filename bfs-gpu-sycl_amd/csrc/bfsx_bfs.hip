@@ -838,6 +838,20 @@ __global__ __launch_bounds__(COMPACT_WG) void k_compact_dev(
                      track ? &ctl->q_edges[out] : nullptr, fbm_next);
 }
 
+// frontier bitmap from a frontier queue: clear, then set one bit per entry
+__global__ void k_fill_words(uint32_t *p, uint32_t n) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
+__global__ void k_queue_bits(const uint32_t *q, uint32_t F, uint32_t *fbm) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F; i += stride) {
+    const uint32_t v = q[i];
+    atomicOr(&fbm[v >> 5], 1u << (v & 31));
+  }
+}
+
 // frontier bitmap of one level from the levels array (a wave owns 64 vertices)
 __global__ void k_levels_to_fbm(const int32_t *levels, uint32_t n_cols, int32_t level,
                                 uint32_t *fbm, uint32_t n_groups) {
@@ -994,6 +1008,18 @@ __global__ void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+// Event pair around a step's main kernel (timed runs only).
+struct KernelTimer {
+  const StepArgs &s;
+  hipStream_t st;
+  KernelTimer(const StepArgs &a, hipStream_t stream) : s(a), st(stream) {
+    if (s.ev_k0) BFSX_HIP(hipEventRecord(s.ev_k0, st));
+  }
+  ~KernelTimer() {
+    if (s.ev_k1) (void)hipEventRecord(s.ev_k1, st);
+  }
+};
+
 static unsigned cap_grid(uint64_t blocks, unsigned cap) {
   if (blocks == 0) return 1;
   return blocks > cap ? cap : (unsigned)blocks;
@@ -1038,6 +1064,7 @@ static void launch_llb_t(bfsx_ctx *ctx, const KArgs &a) {
 void step_llb(bfsx_ctx *ctx, const StepArgs &s, bool strided) {
   if (s.F == 0) return;
   const KArgs a = kargs(s, ctx->params.hub_degree);
+  KernelTimer kt(s, ctx->stream);
   if (ctx->ptr_bytes == 8) {
     if (strided) launch_llb_t<uint64_t, true>(ctx, a);
     else launch_llb_t<uint64_t, false>(ctx, a);
@@ -1068,6 +1095,7 @@ static void step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
                      st, g_loc, tile, g_end, F, g_scan, chunk, ctx->glb_chunk_lo.as<uint32_t>());
   const KArgs a = kargs(s, 0);
   const unsigned grid = (unsigned)ctx->cu_count * ctx->params.glb_blocks_per_cu;
+  KernelTimer kt(s, st);
   switch (ctx->params.glb_wg) {
     case 128:
       hipLaunchKernelGGL((k_glb_expand<PtrT, 128>), dim3(grid), dim3(128), 0, st, a, g_scan,
@@ -1144,6 +1172,7 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
   per_cu = by_lds < per_cu ? (by_lds ? by_lds : 1) : per_cu;
   const unsigned grid = cap_grid(((uint64_t)n_chunks + nw - 1) / nw,
                                  (unsigned)ctx->cu_count * per_cu);
+  KernelTimer kt(s, st);
   switch (wg) {
     case 64: hipLaunchKernelGGL((k_pull<PtrT, 64>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
     case 128: hipLaunchKernelGGL((k_pull<PtrT, 128>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
@@ -1204,6 +1233,16 @@ void bitmap_to_queue(bfsx_ctx *ctx, const uint32_t *bm, uint32_t *q) {
   LevelCtl *ctl = ctx->ctl.as<LevelCtl>();
   hipLaunchKernelGGL((k_bm_to_queue<256>), dim3(grid), dim3(256), 0, ctx->stream, bm, n_words,
                      per, q, &ctl->conv);
+  BFSX_HIP(hipGetLastError());
+}
+
+void queue_to_fbm(bfsx_ctx *ctx, const uint32_t *q, uint32_t F, uint32_t *fbm) {
+  const uint32_t n_words = (uint32_t)bitmap_words(ctx->n_cols);
+  hipLaunchKernelGGL(k_fill_words, dim3(cap_grid(((uint64_t)n_words + 255) / 256, (unsigned)ctx->cu_count * 8)),
+                     dim3(256), 0, ctx->stream, fbm, n_words);
+  if (F)
+    hipLaunchKernelGGL(k_queue_bits, dim3(cap_grid(((uint64_t)F + 255) / 256, (unsigned)ctx->cu_count * 8)),
+                       dim3(256), 0, ctx->stream, q, F, fbm);
   BFSX_HIP(hipGetLastError());
 }
 

@@ -170,6 +170,8 @@ struct Walker {
     a.next_level = (int32_t)level + 1;
     a.track_degrees = track;
     a.count_examined = count;
+    a.ev_k0 = count ? ctx->ev_k0 : nullptr;
+    a.ev_k1 = count ? ctx->ev_k1 : nullptr;
     const bool use_mark = choice != BFSX_PULL && EF >= mark_min_edges(ctx);
     a.mark = use_mark ? ctx->mark.as<uint8_t>() : nullptr;
     if (stamp) level_begin(ctx);
@@ -182,8 +184,9 @@ struct Walker {
       case BFSX_GLB: step_glb(ctx, a); break;
       case BFSX_PULL:
         if (!pull_possible) fail(BFSX_ERR_UNSUPPORTED, "bottom-up step on a non-symmetric graph");
-        if (!fbm_valid)
-          levels_to_fbm(ctx, a.levels, (int32_t)level, ctx->fbm[fbm_cur].as<uint32_t>());
+        // (the queue is valid whenever the bitmap is not: only a bottom-up
+        // step leaves the queue stale, and it writes the bitmap)
+        if (!fbm_valid) queue_to_fbm(ctx, a.q_in, F, ctx->fbm[fbm_cur].as<uint32_t>());
         a.fbm_cur = ctx->fbm[fbm_cur].as<uint32_t>();
         a.fbm_next = ctx->fbm[fbm_cur ^ 1].as<uint32_t>();
         step_pull(ctx, a);
@@ -373,9 +376,11 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
         visited_v = ctx->h_ctl->visited_vertices;
         visited_e = ctx->h_ctl->visited_edges;
       }
-      float us = 0;
+      float us = 0, kus = 0;
       BFSX_HIP(hipEventElapsedTime(&us, ctx->ev_step0, ctx->ev_step1));
+      BFSX_HIP(hipEventElapsedTime(&kus, ctx->ev_k0, ctx->ev_k1));
       r.us = us * 1000.0;
+      r.kernel_us = kus * 1000.0;
       ctx->trace.push_back(r);
     }
     out.steps[choice]++;
@@ -592,6 +597,8 @@ int bfsx_create(int device, bfsx_ctx **out) {
     BFSX_HIP(hipEventCreate(&ctx->ev1));
     BFSX_HIP(hipEventCreate(&ctx->ev_step0));
     BFSX_HIP(hipEventCreate(&ctx->ev_step1));
+    BFSX_HIP(hipEventCreate(&ctx->ev_k0));
+    BFSX_HIP(hipEventCreate(&ctx->ev_k1));
     ensure(ctx->ctl, sizeof(LevelCtl), ctx->stream);
     BFSX_HIP(hipHostMalloc((void **)&ctx->h_ctl, sizeof(LevelCtl), hipHostMallocDefault));
     ctx->params = default_params();
@@ -617,7 +624,7 @@ void bfsx_destroy(bfsx_ctx *ctx) {
   for (DevBuf *b : {&ctx->sched_w, &ctx->snap_levels, &ctx->snap_vis, &ctx->snap_ctl})
     release(*b);
   if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
-  for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_step0, ctx->ev_step1})
+  for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_step0, ctx->ev_step1, ctx->ev_k0, ctx->ev_k1})
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -89,6 +89,7 @@ struct bfsx_ctx {
   bfsx_params params{};
   std::vector<bfsx_level_record> trace;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_step0 = nullptr, ev_step1 = nullptr;
+  hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr;  // main kernel of a timed step
 
   // batched multi-source search state
   bfsx::DevBuf ms_seen, ms_cur, ms_nxt, ms_lv, ms_hubs;
@@ -149,6 +150,7 @@ struct StepArgs {
   bool track_degrees;      // accumulate degree sum of discovered vertices
   uint8_t *mark;           // non-null: mark-then-compact step
   bool count_examined;     // bottom-up: add adjacency entries read to ctl->examined
+  hipEvent_t ev_k0, ev_k1; // non-null: recorded around the step's main kernel
 };
 
 void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *parents);
@@ -156,6 +158,7 @@ void step_llb(bfsx_ctx *ctx, const StepArgs &a, bool strided);
 void step_glb(bfsx_ctx *ctx, const StepArgs &a);
 void step_pull(bfsx_ctx *ctx, const StepArgs &a);
 void bitmap_to_queue(bfsx_ctx *ctx, const uint32_t *bm, uint32_t *q);
+void queue_to_fbm(bfsx_ctx *ctx, const uint32_t *q, uint32_t F, uint32_t *fbm);
 void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level,
                    uint32_t *fbm);
 void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next);

@@ -101,6 +101,8 @@ typedef struct bfsx_level_record {
   uint64_t discovered;      /* vertices found by this step */
   double us;                /* device time of this step */
   uint64_t pull_scanned;    /* adjacency entries read by a bottom-up step */
+  double kernel_us;         /* device time of the step's main kernel
+                               (k_llb / k_glb_expand / k_pull) */
 } bfsx_level_record;
 
 /* launch-shape parameters (the SM_FACTOR / work-group knobs) */
@@ -169,7 +171,7 @@ int bfsx_run_batch(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots,
                    bfsx_run_stats *stats);
 /* Per-level records of the last run made with BFSX_RUN_TRACE or
  * BFSX_RUN_STEP_TIMES (the latter fills level, choice, frontier_vertices,
- * discovered, us and pull_scanned). */
+ * discovered, us, pull_scanned and kernel_us). */
 int bfsx_last_trace(bfsx_ctx *ctx, bfsx_level_record *out, uint32_t max_records,
                     uint32_t *n_records);
 /* Level histogram of the last run, computed on the device: counts[l] =

@@ -266,6 +266,8 @@ def test_level_histogram_and_step_times(engine):
         assert (edges == np.array([deg[want == k].sum() for k in range(len(hist))])).all()
         assert [t["frontier_vertices"] for t in tr] == list(hist)
         assert all(t["us"] > 0 for t in tr)
+        # the main kernel's events sit inside the step's
+        assert all(0 < t["kernel_us"] <= t["us"] + 1.0 for t in tr), tr
         if s != "pull":
             assert all(t["pull_scanned"] == 0 for t in tr)
             continue
