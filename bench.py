@@ -291,7 +291,7 @@ def main():
                       if w["kind"] == "rmat" else {"side": w["side"]}),
                    "vertices": info["n_cols"], "nnz": info["nnz"],
                    "col_ptr_bytes": info["col_ptr_bytes"], "roots": len(roots),
-                   "strategy": strategy,
+                   "strategy": strategy, "adjacency_order": w["order"],
                    "mean_edges_per_root": float(edges.mean())},
     }
     if strategy != "llb":

@@ -23,6 +23,7 @@ STRATEGIES = {"llb": LLB, "local": LLB, "glb": GLB, "global": GLB, "slb": SLB,
               "local_strided": SLB, "pull": PULL, "learned": LEARNED}
 STEP_NAMES = ["llb", "glb", "slb", "pull"]
 GRAPH_SYMMETRIC = 1
+GRAPH_HUB_FIRST = 2
 RUN_GRAPH_LOOP = 1
 RUN_TRACE = 2
 RUN_STEP_TIMES = 4
@@ -100,6 +101,7 @@ SIGNATURES = {
                                      C.POINTER(C.c_uint64)]),
     "bfsx_graph_info_get": (C.c_int, [P, C.POINTER(GraphInfo)]),
     "bfsx_download_csc": (C.c_int, [P, P, P]),
+    "bfsx_order_adjacency": (C.c_int, [P, C.c_int]),
     "bfsx_pick_roots": (C.c_int, [P, C.c_uint32, C.c_uint64, C.c_int, P,
                                   C.POINTER(C.c_uint32)]),
     "bfsx_run": (C.c_int, [P, C.c_uint32, C.c_int, C.c_uint32, P, P, C.POINTER(RunStats)]),
@@ -222,6 +224,12 @@ class Engine:
         ri = np.empty(max(g["nnz"], 1), np.uint32)
         self._check(lib().bfsx_download_csc(self._h, _ptr(cp), _ptr(ri)))
         return cp, ri[:g["nnz"]]
+
+    def order_adjacency(self, order="hub_first"):
+        """Re-sorts the resident adjacency lists: "id" (ascending neighbour
+        ids) or "hub_first" (neighbour degree descending, ties by id)."""
+        code = {"id": 0, "hub_first": 1}[order]
+        self._check(lib().bfsx_order_adjacency(self._h, code))
 
     def pick_roots(self, n, seed, mode=0):
         out = np.zeros(n, np.uint32)

@@ -621,33 +621,72 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         }
         if (a.examined) scanned += (unsigned long long)((k + 4 < e ? k + 4 : e) - b);
       }
-      // long lists: the wave scans one at a time, 64 neighbours per round
+      // long lists: four at a time, each by a quarter wave (16 lanes x 4
+      // loads = 64 consecutive neighbours per round, all loads in flight)
       unsigned long long big = __ballot(cand && deg > PULL_WAVE_DEG);
+      const uint32_t q = lane >> 4, ql = lane & 15;
+      const unsigned long long qmask = 0xFFFFull << (16 * q);
       while (big) {
-        const int src = __ffsll(big) - 1;
-        big &= big - 1;
-        const PtrT bb = __shfl(b, src, 64), ee = __shfl(e, src, 64);
-        bool hit = false;
-        uint32_t hp = 0;
-        for (PtrT base = bb; base < ee; base += 64) {
-          const PtrT k = base + lane;
-          uint32_t w = 0;
-          bool h = false;
-          if (k < ee) {
-            w = a.row_idx[k];
-            h = in_frontier(fbm_cur, s_sum, shift, w);
-          }
-          const unsigned long long hm = __ballot(h);
-          if (a.examined && lane == 0) scanned += (unsigned long long)(ee - base < 64 ? ee - base : 64);
-          if (hm) {
-            hp = __shfl(w, __ffsll(hm) - 1, 64);
-            hit = true;
-            break;
+        int my_src = -1;  // the candidate lane my quarter serves
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+          if (big) {
+            if (i == q) my_src = __ffsll(big) - 1;
+            big &= big - 1;
           }
         }
-        if ((int)lane == src) {
-          found = hit;
-          par = hp;
+        const PtrT sb = __shfl(b, my_src < 0 ? 0 : my_src, 64);
+        const PtrT se = __shfl(e, my_src < 0 ? 0 : my_src, 64);
+        PtrT base = my_src < 0 ? se : sb;
+        bool done = base >= se, hit = false;
+        uint32_t hp = 0;
+        while (__ballot(!done)) {
+          uint32_t w[4];
+          bool h[4];
+#pragma unroll
+          for (uint32_t j = 0; j < 4; j++) {
+            const PtrT k = base + j * 16 + ql;
+            h[j] = false;
+            w[j] = 0;
+            if (!done && k < se) w[j] = a.row_idx[k];
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < 4; j++) {
+            const PtrT k = base + j * 16 + ql;
+            if (!done && k < se) h[j] = in_frontier(fbm_cur, s_sum, shift, w[j]);
+          }
+          bool got = false;
+          uint32_t gw = 0;
+#pragma unroll
+          for (uint32_t j = 0; j < 4; j++) {
+            const unsigned long long hb = __ballot(h[j]) & qmask;
+            const uint32_t pw = __shfl(w[j], hb ? __ffsll(hb) - 1 : (int)lane, 64);
+            if (!got && hb) {
+              got = true;
+              gw = pw;
+            }
+          }
+          if (!done) {
+            if (a.examined && ql == 0) scanned += (unsigned long long)(se - base < 64 ? se - base : 64);
+            if (got) {
+              hit = true;
+              hp = gw;
+              done = true;
+            }
+            base += 64;
+            if (base >= se) done = true;
+          }
+        }
+        // hand each result to its candidate lane
+#pragma unroll
+        for (uint32_t qq = 0; qq < 4; qq++) {
+          const int s_q = __shfl(my_src, (int)(16 * qq), 64);
+          const bool h_q = __shfl((int)hit, (int)(16 * qq), 64) != 0;
+          const uint32_t p_q = __shfl(hp, (int)(16 * qq), 64);
+          if (s_q == (int)lane) {
+            found = h_q;
+            par = p_q;
+          }
         }
       }
       if (found) {

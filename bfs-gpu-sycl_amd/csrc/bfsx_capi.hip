@@ -760,6 +760,17 @@ int bfsx_graph_info_get(bfsx_ctx *ctx, bfsx_graph_info *out) {
   });
 }
 
+int bfsx_order_adjacency(bfsx_ctx *ctx, int order) {
+  if (!ctx) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
+    if (order != BFSX_ORDER_ID && order != BFSX_ORDER_HUB_FIRST)
+      fail(BFSX_ERR_ARG, "order must be BFSX_ORDER_ID or BFSX_ORDER_HUB_FIRST");
+    order_adjacency(ctx, order == BFSX_ORDER_HUB_FIRST);
+  });
+}
+
 int bfsx_download_csc(bfsx_ctx *ctx, void *col_ptr, uint32_t *row_idx) {
   if (!ctx) return BFSX_ERR_ARG;
   return guarded(ctx, [&] {

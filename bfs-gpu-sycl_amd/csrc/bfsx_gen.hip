@@ -360,6 +360,127 @@ void graph_summary(bfsx_ctx *ctx) {
   ctx->n_isolated = h[1];
 }
 
+// ---- adjacency order --------------------------------------------------------
+// Each column's list is re-sorted in place by a vertex rank: rank(v) = v
+// (ascending ids, the builder's order) or the position of v when vertices
+// are sorted by (degree descending, id ascending) ("hub first").  A
+// bottom-up step stops at the first neighbour in the frontier, and on
+// power-law graphs the frontier holds the hubs early on, so hub-first lists
+// end most scans after one probe.  Levels do not depend on the order.
+template <typename PtrT>
+__global__ void k_degree_keys(const PtrT *col_ptr, uint32_t n_rows, uint32_t n_cols,
+                              uint32_t max_deg, int B, uint64_t *keys) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n_cols;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t d = v < n_rows ? (uint64_t)(col_ptr[v + 1] - col_ptr[v]) : 0ull;
+    const uint64_t dc = d > max_deg ? max_deg : d;
+    keys[v] = ((uint64_t)(max_deg - dc) << B) | v;
+  }
+}
+
+// order[i] = vertex of rank i, rank[v] = rank of v
+__global__ void k_ranks(const uint64_t *sorted, uint32_t n, uint64_t mask, uint32_t *order,
+                        uint32_t *rank) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = (uint32_t)(sorted[i] & mask);
+    order[i] = v;
+    rank[v] = (uint32_t)i;
+  }
+}
+
+// key of edge e in column c: (c << B) | rank(row_idx[e]); a wave per column
+template <typename PtrT>
+__global__ void k_edge_rank_keys(const PtrT *col_ptr, uint32_t n_rows, const uint32_t *row_idx,
+                                 const uint32_t *rank, int B, uint64_t *keys) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t c = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6); c < n_rows;
+       c += nwaves) {
+    const PtrT b = col_ptr[c], e = col_ptr[c + 1];
+    for (PtrT k = b + lane; k < e; k += 64)
+      keys[k] = (c << B) | (rank ? rank[row_idx[k]] : row_idx[k]);
+  }
+}
+
+__global__ void k_rows_from_keys(const uint64_t *sorted, uint64_t nnz, uint64_t mask,
+                                 const uint32_t *order, uint32_t *row_idx) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nnz;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = (uint32_t)(sorted[k] & mask);
+    row_idx[k] = order ? order[r] : r;
+  }
+}
+
+template <typename PtrT>
+static void order_adjacency_t(bfsx_ctx *ctx, bool hub_first) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = ctx->n_cols;
+  const uint64_t nnz = ctx->nnz;
+  const int B = bits_for(n) > 0 ? bits_for(n) : 1;
+  const uint64_t mask = (1ull << B) - 1;
+  const PtrT *cp = ctx->col_ptr.as<PtrT>();
+  DevBuf keys, sorted, tmp, rank, order;
+  struct Releaser {  // temporaries go on every exit path
+    DevBuf *b[5];
+    ~Releaser() {
+      for (DevBuf *x : b) release(*x);
+    }
+  } releaser{{&keys, &sorted, &tmp, &rank, &order}};
+  size_t tbytes = 0;
+  if (hub_first) {
+    ensure(keys, sizeof(uint64_t) * n, s);
+    ensure(sorted, sizeof(uint64_t) * n, s);
+    ensure(rank, sizeof(uint32_t) * n, s);
+    ensure(order, sizeof(uint32_t) * n, s);
+    hipLaunchKernelGGL(k_degree_keys<PtrT>, dim3(grid_for(n)), dim3(256), 0, s, cp,
+                       ctx->n_rows, n, ctx->max_degree, B, keys.as<uint64_t>());
+    BFSX_HIP(hipGetLastError());
+    const int end_bit = B + bits_for((uint64_t)ctx->max_degree + 1);
+    BFSX_HIP(rocprim::radix_sort_keys(nullptr, tbytes, keys.as<uint64_t>(),
+                                      sorted.as<uint64_t>(), (size_t)n, 0, end_bit, s));
+    ensure(tmp, tbytes, s);
+    BFSX_HIP(rocprim::radix_sort_keys(tmp.p, tbytes, keys.as<uint64_t>(),
+                                      sorted.as<uint64_t>(), (size_t)n, 0, end_bit, s));
+    hipLaunchKernelGGL(k_ranks, dim3(grid_for(n)), dim3(256), 0, s, sorted.as<uint64_t>(), n,
+                       mask, order.as<uint32_t>(), rank.as<uint32_t>());
+    BFSX_HIP(hipGetLastError());
+    BFSX_HIP(hipStreamSynchronize(s));
+    release(keys);
+    release(sorted);
+    release(tmp);
+  }
+  if (nnz) {
+    ensure(keys, sizeof(uint64_t) * nnz, s);
+    hipLaunchKernelGGL(k_edge_rank_keys<PtrT>, dim3(grid_for((uint64_t)ctx->n_rows * 64)),
+                       dim3(256), 0, s, cp, ctx->n_rows, ctx->row_idx.as<uint32_t>(),
+                       hub_first ? rank.as<uint32_t>() : (const uint32_t *)nullptr, B,
+                       keys.as<uint64_t>());
+    BFSX_HIP(hipGetLastError());
+    ensure(sorted, sizeof(uint64_t) * nnz, s);
+    const int end_bit = B + bits_for((uint64_t)ctx->n_rows + 1);
+    tbytes = 0;
+    BFSX_HIP(rocprim::radix_sort_keys(nullptr, tbytes, keys.as<uint64_t>(),
+                                      sorted.as<uint64_t>(), (size_t)nnz, 0, end_bit, s));
+    ensure(tmp, tbytes, s);
+    BFSX_HIP(rocprim::radix_sort_keys(tmp.p, tbytes, keys.as<uint64_t>(),
+                                      sorted.as<uint64_t>(), (size_t)nnz, 0, end_bit, s));
+    hipLaunchKernelGGL(k_rows_from_keys, dim3(grid_for(nnz)), dim3(256), 0, s,
+                       sorted.as<uint64_t>(), nnz, mask,
+                       hub_first ? order.as<uint32_t>() : (const uint32_t *)nullptr,
+                       ctx->row_idx.as<uint32_t>());
+    BFSX_HIP(hipGetLastError());
+  }
+  BFSX_HIP(hipStreamSynchronize(s));
+}
+
+void order_adjacency(bfsx_ctx *ctx, bool hub_first) {
+  if (ctx->ptr_bytes == 8) order_adjacency_t<uint64_t>(ctx, hub_first);
+  else order_adjacency_t<uint32_t>(ctx, hub_first);
+  if (hub_first) ctx->gflags |= BFSX_GRAPH_HUB_FIRST;
+  else ctx->gflags &= ~BFSX_GRAPH_HUB_FIRST;
+}
+
 // ---- seeded root candidates ------------------------------------------------
 // Candidate k is x = (w0 * V) >> 32 of its tag-2 draw; ok[k] says whether it
 // qualifies (degree >= 1, or reached in component_levels when given).

@@ -126,6 +126,8 @@ uint64_t gen_grid_pairs(bfsx_ctx *ctx, uint32_t side, uint32_t t_drop,
                         uint32_t n_long, uint64_t seed, uint32_t *d_u,
                         uint32_t *d_v);
 void graph_summary(bfsx_ctx *ctx);  // max degree, isolated bitmap/count
+// re-sorts every adjacency list by neighbour id or hub first (needs graph_summary)
+void order_adjacency(bfsx_ctx *ctx, bool hub_first);
 void root_candidates(bfsx_ctx *ctx, uint64_t seed, uint64_t first,
                      uint32_t count, const int32_t *d_component_levels,
                      uint32_t *d_out_vertex, uint8_t *d_out_ok);

@@ -37,8 +37,14 @@ for _k, _side in enumerate([1024, 2048, 3072, 4096]):
         seed_graph=7050 + _k, seed_roots=7250 + _k, n_roots=4, ptr_bytes=4)
 
 
-def build(engine, name, **override):
-    """Generates workload `name` in HBM; returns (config dict, nnz)."""
+# Adjacency order the workloads are searched in (bfsx_order_adjacency).
+ORDER = "id"
+
+
+def build(engine, name, order=ORDER, **override):
+    """Generates workload `name` in HBM and orders its adjacency lists
+    (`order` = "hub_first", "id" or None to keep the builder's id order);
+    returns (config dict, nnz)."""
     w = dict(WORKLOADS[name] if name in WORKLOADS else TRAIN_WORKLOADS[name], **override)
     if w["kind"] == "rmat":
         a, b, c = RMAT_ABC
@@ -47,6 +53,9 @@ def build(engine, name, **override):
     else:
         nnz = engine.generate_grid(w["side"], w["drop_prob"], w["n_long"], w["seed_graph"],
                                    w["ptr_bytes"])
+    if order and order != "id":
+        engine.order_adjacency(order)
+    w["order"] = order or "id"
     return w, nnz
 
 

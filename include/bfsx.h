@@ -57,6 +57,7 @@ extern "C" {
 
 /* graph flags */
 #define BFSX_GRAPH_SYMMETRIC 1u /* every stored edge u->v has v->u */
+#define BFSX_GRAPH_HUB_FIRST 2u /* adjacency lists ordered hub first (below) */
 
 /* run flags */
 #define BFSX_RUN_GRAPH_LOOP 1u  /* capture the level loop in a hipGraph with a
@@ -151,6 +152,16 @@ int bfsx_generate_grid(bfsx_ctx *ctx, uint32_t side, double drop_prob,
                        uint32_t n_long, uint64_t seed, int col_ptr_bytes,
                        uint64_t *nnz);
 int bfsx_graph_info_get(bfsx_ctx *ctx, bfsx_graph_info *out);
+/* Re-sorts every adjacency list of the resident graph in place (col_ptr is
+ * unchanged): BFSX_ORDER_ID = ascending neighbour ids (the builder's and
+ * scipy's CSC order), BFSX_ORDER_HUB_FIRST = neighbours by degree
+ * descending, ties by id (sets BFSX_GRAPH_HUB_FIRST).  Levels do not change;
+ * bottom-up steps end their scans earlier on power-law graphs.  A layout
+ * step of this engine: the reference keeps lists in file order
+ * (src/graph_load.hpp:23-131). */
+#define BFSX_ORDER_ID 0
+#define BFSX_ORDER_HUB_FIRST 1
+int bfsx_order_adjacency(bfsx_ctx *ctx, int order);
 /* Copies the resident graph to host buffers of n_cols+1 / nnz entries. */
 int bfsx_download_csc(bfsx_ctx *ctx, void *col_ptr, uint32_t *row_idx);
 /* Seeded roots: mode 0 = vertices of degree >= 1, mode 1 = vertices of the

@@ -166,6 +166,34 @@ def test_glb_shapes(engine, glb_wg, chunk, bpc):
         engine.set_params(**old)
 
 
+@pytest.mark.parametrize("ptr", [4, 8])
+def test_hub_first_adjacency_order(engine, ptr):
+    """Lists re-sorted by (neighbour degree desc, id asc) hold the same
+    neighbours; every step kind still gives the oracle's levels and valid
+    parents; ordering by id restores the builder's arrays bit-exact."""
+    engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 81, 82, ptr)
+    cp, ri = engine.download_csc()
+    engine.order_adjacency("hub_first")
+    assert engine.info()["flags"] & bfsx.GRAPH_HUB_FIRST
+    cp2, ri2 = engine.download_csc()
+    assert (cp2 == cp).all()
+    deg = np.diff(cp.astype(np.int64))
+    col = np.repeat(np.arange(len(deg)), deg)
+    assert (ri2[np.lexsort((ri2, col))] == ri).all()  # same multiset per list
+    same = col[1:] == col[:-1]
+    a, b = ri2[:-1][same], ri2[1:][same]
+    assert ((deg[a] > deg[b]) | ((deg[a] == deg[b]) & (a < b))).all()
+    roots = engine.pick_roots(6, 83)
+    assert_levels(engine, cp, ri, roots, STEPS, parents=True)
+    engine.load_scheduler()
+    assert_levels(engine, cp, ri, roots, ["learned"])
+    engine.set_scheduler_weights(None)
+    engine.order_adjacency("id")
+    assert not engine.info()["flags"] & bfsx.GRAPH_HUB_FIRST
+    cp3, ri3 = engine.download_csc()
+    assert (cp3 == cp).all() and (ri3 == ri).all()
+
+
 def test_directed_graph_with_fewer_rows(engine):
     """Directed input, duplicates, self loops, n_rows < n_cols (reference quirk)."""
     rng = np.random.default_rng(9)
