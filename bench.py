@@ -316,6 +316,15 @@ def main():
             ex["traffic"] = traffic_from_profiles(args.workload, ex["step"])
             result["explore_roofline_llb"] = ex
         result["graph500_gteps"] = round(float(value) / 2, 3)
+        # whole searches against the top-down algorithmic bytes (the same
+        # denominator for every step kind, like TEPS)
+        reached = np.array([s["reached"] for s in stats], np.float64)
+        b_alg = 5.0 * edges + reached * (2.0 * info["col_ptr_bytes"] + 8.0 + 4.0)
+        ach = float(b_alg.sum() / (t_root.sum() * 1e-3) / 1e9)
+        result["search_roofline"] = {"bound": "hbm", "achieved": round(ach, 1),
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                                     "bytes_per_root": round(float(b_alg.mean()), 1)}
     if not args.no_cpu:
         lev_cache = {}
 
