@@ -68,7 +68,8 @@ class Params(C.Structure):
     _fields_ = [("llb_wg", C.c_uint32), ("glb_wg", C.c_uint32),
                 ("glb_blocks_per_cu", C.c_uint32), ("glb_chunk", C.c_uint32),
                 ("hub_degree", C.c_uint32), ("pull_wg", C.c_uint32),
-                ("graph_levels", C.c_uint32), ("mark_ratio_q10", C.c_uint32)]
+                ("graph_levels", C.c_uint32), ("mark_ratio_q10", C.c_uint32),
+                ("bucket_ratio_q10", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -473,6 +473,222 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
 }
 
 // ---------------------------------------------------------------------------
+// GLB with bucketed discovery (heavy levels)
+// ---------------------------------------------------------------------------
+// Random byte stores into `mark` (one per edge whose target is unvisited)
+// cost GLB three quarters of its time on heavy levels, and atomics on a
+// bitmap cost more.  Bucketed discovery makes every global access sequential
+// instead: targets are split by destination slice (2^BKT_SHIFT vertices), so
+//   pass 1  each work-group counts its edges per slice in LDS and writes its
+//           row of the (group x slice) count matrix;
+//   scan    column-wise exclusive scans give each group's slot range inside
+//           each slice's segment, and a scan of the slice totals the
+//           segments' bases;
+//   pass 2  the groups walk the same chunks again (static assignment) and
+//           append each target to its slice's segment (LDS cursors);
+//   apply   a work-group per slice holds the slice's visited words in LDS,
+//           ORs the targets into an LDS found-bitmap, and emits the new
+//           vertices: levels, visited and next-frontier words, the queue in
+//           vertex order with one atomic per group.
+// Every slice fits in LDS (2^16 vertices: 8 KiB visited + 8 KiB found).
+constexpr uint32_t BKT_SHIFT = 16;
+constexpr uint32_t BKT_MAX = 2048;  // slices: graphs of up to 2^27 vertices
+
+// Calls act(target) for every edge of [lo, hi) of the staged window (as
+// expand_edges: four consecutive edges per lane, all loads first).
+template <typename PtrT, int WG, class Act>
+__device__ __forceinline__ void window_edges(const KArgs &a, const PtrT *s_scan,
+                                             const PtrT *s_start, PtrT lo, PtrT hi, Act act) {
+  for (PtrT base = lo + (PtrT)threadIdx.x * EPT; base < hi; base += (PtrT)WG * EPT) {
+    uint32_t j = 0;
+#pragma unroll
+    for (uint32_t step = WG / 2; step > 0; step >>= 1)
+      if (s_scan[j + step] <= base) j += step;
+    uint32_t n[EPT];
+    bool valid[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; k++) {
+      const PtrT e = base + k;
+      valid[k] = e < hi;
+      while (valid[k] && j + 1 < WG && s_scan[j + 1] <= e) j++;
+      n[k] = valid[k] ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; k++)
+      if (valid[k]) act(n[k]);
+  }
+}
+
+// PASS 1: per-slice counts of this group's edges -> cnt[group][slice].
+// PASS 2: slots from pos[group][slice] + base[slice]; targets -> bkt.
+// Chunks are assigned statically (group g takes c = g, g + grid, ...), so
+// both passes see the same edges in every group.
+template <typename PtrT, int WG, int PASS>
+__global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
+                                                   const PtrT *g_start,
+                                                   const uint32_t *chunk_lo, LevelCtl *ctl,
+                                                   uint32_t chunk, uint32_t n_slices,
+                                                   uint32_t *cnt, const uint64_t *base,
+                                                   uint32_t *bkt, uint64_t cap) {
+  __shared__ PtrT s_scan[WG];
+  __shared__ PtrT s_start[WG];
+  __shared__ uint32_t s_v[WG];
+  __shared__ PtrT s_win_lo, s_win_hi;
+  __shared__ uint32_t s_cnt[BKT_MAX];
+  __shared__ uint64_t s_base[PASS == 2 ? BKT_MAX : 1];
+  const uint32_t t = threadIdx.x;
+  uint32_t *row = cnt + (size_t)blockIdx.x * n_slices;
+  for (uint32_t i = t; i < n_slices; i += WG) {
+    s_cnt[i] = PASS == 1 ? 0u : row[i];
+    if (PASS == 2) s_base[i] = base[i];
+  }
+  __syncthreads();
+  const uint32_t n_chunks = ctl->glb_chunks;
+  const PtrT total = (PtrT)ctl->glb_total;
+  for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const PtrT e0 = (PtrT)c * chunk;
+    const PtrT e1 = (e0 + chunk < total) ? e0 + chunk : total;
+    const uint32_t ilo = chunk_lo[c];
+    const uint32_t ihi = (c + 1 < n_chunks) ? chunk_lo[c + 1] : a.F - 1;
+    for (uint32_t wb = ilo; wb <= ihi; wb += WG) {
+      const uint32_t nwin = (ihi + 1 - wb < WG) ? ihi + 1 - wb : WG;
+      if (t < nwin) {
+        s_scan[t] = g_scan[wb + t];
+        s_start[t] = g_start[wb + t];
+      } else {
+        s_scan[t] = ~PtrT(0);
+      }
+      if (t == 0) {
+        s_win_lo = g_scan[wb];
+        s_win_hi = (wb + nwin < a.F) ? g_scan[wb + nwin] : total;
+      }
+      __syncthreads();
+      const PtrT lo = e0 > s_win_lo ? e0 : s_win_lo;
+      const PtrT hi = e1 < s_win_hi ? e1 : s_win_hi;
+      if (PASS == 1) {
+        window_edges<PtrT, WG>(a, s_scan, s_start, lo, hi,
+                               [&](uint32_t n) { atomicAdd(&s_cnt[n >> BKT_SHIFT], 1u); });
+      } else {
+        window_edges<PtrT, WG>(a, s_scan, s_start, lo, hi, [&](uint32_t n) {
+          const uint32_t k = n >> BKT_SHIFT;
+          const uint64_t p = s_base[k] + atomicAdd(&s_cnt[k], 1u);
+          if (p < cap) bkt[p] = n;
+        });
+      }
+      __syncthreads();
+    }
+  }
+  (void)s_v;
+  if (PASS == 1)
+    for (uint32_t i = t; i < n_slices; i += WG) row[i] = s_cnt[i];
+}
+
+// Column k of cnt[groups][slices]: exclusive scan down the groups in place;
+// the column total goes to tot[k].  A work-group per slice; thread t owns a
+// contiguous run of groups.
+template <int WG>
+__global__ __launch_bounds__(WG) void k_bucket_colscan(uint32_t *cnt, uint32_t n_groups,
+                                                       uint32_t n_slices, uint64_t *tot) {
+  __shared__ uint64_t s_wave[WG / 64 + 1];
+  const uint32_t k = blockIdx.x;
+  const uint32_t run = (n_groups + WG - 1) / WG;
+  const uint32_t g0 = threadIdx.x * run;
+  const uint32_t g1 = min(g0 + run, n_groups);
+  uint64_t sum = 0;
+  for (uint32_t g = g0; g < g1; g++) sum += cnt[(size_t)g * n_slices + k];
+  uint64_t total;
+  uint64_t pos = block_excl_scan<WG>(sum, total, s_wave);
+  for (uint32_t g = g0; g < g1; g++) {
+    const uint32_t c = cnt[(size_t)g * n_slices + k];
+    cnt[(size_t)g * n_slices + k] = (uint32_t)pos;  // slice segments hold < 2^32 targets
+    pos += c;
+  }
+  if (threadIdx.x == 0) tot[k] = total;
+}
+
+// base[k] = sum of tot[0..k), base[n_slices] = sum of all (one work-group).
+__global__ __launch_bounds__(1024) void k_bucket_base(const uint64_t *tot, uint32_t n_slices,
+                                                      uint64_t *base) {
+  __shared__ uint64_t s_wave[1024 / 64 + 1];
+  const uint32_t t = threadIdx.x;
+  const uint32_t run = (n_slices + 1023) / 1024;
+  const uint32_t k0 = t * run, k1 = min(k0 + run, n_slices);
+  uint64_t sum = 0;
+  for (uint32_t k = k0; k < k1; k++) sum += tot[k];
+  uint64_t total;
+  uint64_t pos = block_excl_scan<1024>(sum, total, s_wave);
+  for (uint32_t k = k0; k < k1; k++) {
+    base[k] = pos;
+    pos += tot[k];
+  }
+  if (t == 0) base[n_slices] = total;
+}
+
+// A work-group per slice: new vertices of the slice from its targets.
+template <typename PtrT, int WG>
+__global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32_t n_rows,
+                                                     uint32_t n_cols, const uint32_t *bkt,
+                                                     const uint64_t *base, uint64_t cap,
+                                                     uint32_t *vis,
+                                                     int32_t *levels, int32_t next_level,
+                                                     uint32_t *q_out, uint32_t *q_count,
+                                                     unsigned long long *q_edges,
+                                                     uint32_t *fbm_next) {
+  constexpr uint32_t WPS = 1u << (BKT_SHIFT - 5);  // visited words per slice
+  static_assert(WPS % WG == 0, "a thread owns whole words");
+  __shared__ uint32_t s_vis[WPS];
+  __shared__ uint32_t s_found[WPS];
+  __shared__ uint32_t s_wave[WG / 64 + 1];
+  __shared__ uint32_t s_qbase;
+  const uint32_t k = blockIdx.x;
+  const uint32_t n_words = (uint32_t)bitmap_words(n_cols);
+  const uint32_t w0 = k * WPS;
+  for (uint32_t i = threadIdx.x; i < WPS; i += WG) {
+    s_vis[i] = (w0 + i < n_words) ? vis[w0 + i] : ~0u;
+    s_found[i] = 0;
+  }
+  __syncthreads();
+  const uint64_t p0 = base[k], p1 = base[k + 1] < cap ? base[k + 1] : cap;
+  const uint32_t slice0 = k << BKT_SHIFT;
+  for (uint64_t p = p0 + threadIdx.x; p < p1; p += WG) {
+    const uint32_t l = bkt[p] - slice0;
+    const uint32_t bit = 1u << (l & 31);
+    if (!(s_vis[l >> 5] & bit)) atomicOr(&s_found[l >> 5], bit);
+  }
+  __syncthreads();
+  // thread t owns words [t * RUN, (t + 1) * RUN) of the slice: queue order
+  constexpr uint32_t RUN = WPS / WG;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < RUN; r++) {
+    const uint32_t i = threadIdx.x * RUN + r;
+    cnt += __popc(s_found[i] & ~s_vis[i]);
+  }
+  uint32_t total;
+  const uint32_t ex = block_excl_scan<WG>(cnt, total, s_wave);
+  if (threadIdx.x == 0) s_qbase = total ? atomicAdd(q_count, total) : 0u;
+  __syncthreads();
+  uint32_t pos = s_qbase + ex;
+  unsigned long long dsum = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < RUN; r++) {
+    const uint32_t i = threadIdx.x * RUN + r;
+    const uint32_t w = w0 + i;
+    if (w >= n_words) continue;
+    const uint32_t nw = s_found[i] & ~s_vis[i];
+    if (nw) vis[w] = s_vis[i] | nw;
+    if (fbm_next) fbm_next[w] = nw;
+    for (uint32_t b = nw; b; b &= b - 1) {
+      const uint32_t v = w * 32 + (__ffs(b) - 1);
+      levels[v] = next_level;
+      q_out[pos++] = v;
+      if (q_edges) dsum += out_degree(col_ptr, v, n_rows);
+    }
+  }
+  if (q_edges) block_atomic_add<WG>(dsum, q_edges);
+}
+
+// ---------------------------------------------------------------------------
 // PULL (bottom-up)
 // ---------------------------------------------------------------------------
 // Candidates with more neighbours than this are scanned by the whole wave.
@@ -1114,8 +1330,40 @@ void step_llb(bfsx_ctx *ctx, const StepArgs &s, bool strided) {
   BFSX_HIP(hipGetLastError());
 }
 
+// Bucketed discovery of a heavy GLB level (see k_glb_bucket).
 template <typename PtrT>
-static void step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
+static void glb_bucketed(bfsx_ctx *ctx, const StepArgs &s, const KArgs &a, const PtrT *g_scan,
+                         const PtrT *g_start, uint32_t chunk, unsigned grid, uint32_t n_slices) {
+  hipStream_t st = ctx->stream;
+  // the frontier's edge count (kept exactly by every step) bounds the targets
+  const uint64_t cap = s.EF ? s.EF : 1;
+  ensure(ctx->bkt, sizeof(uint32_t) * cap, st);
+  ensure(ctx->bkt_cnt, sizeof(uint32_t) * (size_t)grid * n_slices, st);
+  ensure(ctx->bkt_tot, sizeof(uint64_t) * n_slices, st);
+  ensure(ctx->bkt_base, sizeof(uint64_t) * (n_slices + 1), st);
+  uint32_t *cnt = ctx->bkt_cnt.as<uint32_t>();
+  uint64_t *tot = ctx->bkt_tot.as<uint64_t>();
+  uint64_t *base = ctx->bkt_base.as<uint64_t>();
+  uint32_t *bkt = ctx->bkt.as<uint32_t>();
+  const uint32_t *clo = ctx->glb_chunk_lo.as<uint32_t>();
+  KernelTimer kt(s, st);
+  hipLaunchKernelGGL((k_glb_bucket<PtrT, 256, 1>), dim3(grid), dim3(256), 0, st, a, g_scan,
+                     g_start, clo, s.ctl, chunk, n_slices, cnt, (const uint64_t *)nullptr,
+                     (uint32_t *)nullptr, (uint64_t)0);
+  hipLaunchKernelGGL((k_bucket_colscan<256>), dim3(n_slices), dim3(256), 0, st, cnt,
+                     (uint32_t)grid, n_slices, tot);
+  hipLaunchKernelGGL(k_bucket_base, dim3(1), dim3(1024), 0, st, tot, n_slices, base);
+  hipLaunchKernelGGL((k_glb_bucket<PtrT, 256, 2>), dim3(grid), dim3(256), 0, st, a, g_scan,
+                     g_start, clo, s.ctl, chunk, n_slices, cnt, base, bkt, cap);
+  hipLaunchKernelGGL((k_bucket_apply<PtrT, 512>), dim3(n_slices), dim3(512), 0, st,
+                     static_cast<const PtrT *>(s.col_ptr), s.n_rows, s.n_cols, bkt, base, cap, s.vis,
+                     s.levels, s.next_level, s.q_out, &s.ctl->q_count[s.q_out_idx],
+                     s.track_degrees ? &s.ctl->q_edges[s.q_out_idx] : nullptr, s.fbm_next);
+  BFSX_HIP(hipGetLastError());
+}
+
+template <typename PtrT>
+static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   hipStream_t st = ctx->stream;
   const uint32_t F = s.F;
   const uint32_t chunk = ctx->params.glb_chunk;
@@ -1134,6 +1382,11 @@ static void step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
                      st, g_loc, tile, g_end, F, g_scan, chunk, ctx->glb_chunk_lo.as<uint32_t>());
   const KArgs a = kargs(s, 0);
   const unsigned grid = (unsigned)ctx->cu_count * ctx->params.glb_blocks_per_cu;
+  const uint32_t n_slices = (uint32_t)(((uint64_t)ctx->n_cols + (1u << BKT_SHIFT) - 1) >> BKT_SHIFT);
+  if (s.mark && s.bucket_ok && !s.parents && n_slices <= BKT_MAX) {
+    glb_bucketed<PtrT>(ctx, s, a, g_scan, g_start, chunk, grid, n_slices);
+    return true;
+  }
   KernelTimer kt(s, st);
   switch (ctx->params.glb_wg) {
     case 128:
@@ -1154,10 +1407,11 @@ static void step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
       break;
     default: fail(BFSX_ERR_ARG, "glb_wg must be 128, 256, 512 or 1024");
   }
+  return false;
 }
 
-void step_glb(bfsx_ctx *ctx, const StepArgs &s) {
-  if (s.F == 0) return;
+bool step_glb(bfsx_ctx *ctx, const StepArgs &s) {
+  if (s.F == 0) return false;
   const uint64_t pw = ctx->ptr_bytes;
   // scratch: local + global scans, starts, tile sums, chunk table
   ensure(ctx->glb_scan, pw * 2 * ((uint64_t)ctx->n_cols + 1), ctx->stream);
@@ -1165,9 +1419,10 @@ void step_glb(bfsx_ctx *ctx, const StepArgs &s) {
   ensure(ctx->glb_tile, pw * ((uint64_t)ctx->n_cols / GLB_TILE + 4), ctx->stream);
   ensure(ctx->glb_chunk_lo, sizeof(uint32_t) * (ctx->nnz / ctx->params.glb_chunk + 2),
          ctx->stream);
-  if (ctx->ptr_bytes == 8) step_glb_t<uint64_t>(ctx, s);
-  else step_glb_t<uint32_t>(ctx, s);
+  const bool bucketed =
+      ctx->ptr_bytes == 8 ? step_glb_t<uint64_t>(ctx, s) : step_glb_t<uint32_t>(ctx, s);
   BFSX_HIP(hipGetLastError());
+  return bucketed;
 }
 
 // Summary resolution for n vertices: the smallest shift >= 5 with at most

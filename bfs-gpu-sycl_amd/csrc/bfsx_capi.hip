@@ -50,7 +50,8 @@ static void free_search_state(bfsx_ctx *ctx) {
   drop_loop_graph(ctx);  // the captured loop holds pointers into this state
   for (DevBuf *b : {&ctx->levels, &ctx->parents, &ctx->vis, &ctx->fbm[0], &ctx->fbm[1],
                     &ctx->queue[0], &ctx->queue[1], &ctx->glb_scan, &ctx->glb_start,
-                    &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark, &ctx->fsum, &ctx->ms_seen,
+                    &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark, &ctx->fsum, &ctx->bkt,
+                    &ctx->bkt_cnt, &ctx->bkt_tot, &ctx->bkt_base, &ctx->ms_seen,
                     &ctx->ms_cur, &ctx->ms_nxt, &ctx->ms_lv, &ctx->ms_hubs})
     release(*b);
 }
@@ -90,6 +91,7 @@ static bfsx_params default_params() {
   p.pull_wg = 512;
   p.graph_levels = 8;
   p.mark_ratio_q10 = 64;
+  p.bucket_ratio_q10 = 512;
   return p;
 }
 
@@ -139,6 +141,7 @@ struct Walker {
   bfsx_ctx *ctx;
   uint32_t F = 1, cur = 0, level = 0;
   unsigned long long EF = 0;  // frontier edges (kept by the kernels: track is on)
+  unsigned long long VE = 0;  // edges of the visited vertices
   int fbm_cur = 0;         // buffer holding the current frontier's bitmap
   bool fbm_valid = false;  // whether that bitmap is up to date
   bool queue_valid = true; // false after a bottom-up step (frontier only as bitmap)
@@ -172,8 +175,15 @@ struct Walker {
     a.count_examined = count;
     a.ev_k0 = count ? ctx->ev_k0 : nullptr;
     a.ev_k1 = count ? ctx->ev_k1 : nullptr;
+    a.EF = EF;
+    const uint32_t br = ctx->params.bucket_ratio_q10;
+    const unsigned long long UE = ctx->nnz > VE ? ctx->nnz - VE : 0ull;
+    a.bucket_ok = br != 0xFFFFFFFFu && (double)UE * 1024.0 >= (double)ctx->nnz * br;
     const bool use_mark = choice != BFSX_PULL && EF >= mark_min_edges(ctx);
     a.mark = use_mark ? ctx->mark.as<uint8_t>() : nullptr;
+    // next frontier bitmap of a top-down step (kept for bottom-up steps)
+    uint32_t *td_fbm_next = pull_possible ? ctx->fbm[fbm_cur ^ 1].as<uint32_t>() : nullptr;
+    bool bucketed = false;
     if (stamp) level_begin(ctx);
     if (choice != BFSX_PULL && !queue_valid)
       bitmap_to_queue(ctx, ctx->fbm[fbm_cur].as<uint32_t>(), ctx->queue[cur].as<uint32_t>());
@@ -181,7 +191,10 @@ struct Walker {
     switch (choice) {
       case BFSX_LLB: step_llb(ctx, a, false); break;
       case BFSX_SLB: step_llb(ctx, a, true); break;
-      case BFSX_GLB: step_glb(ctx, a); break;
+      case BFSX_GLB:
+        a.fbm_next = td_fbm_next;
+        bucketed = step_glb(ctx, a);
+        break;
       case BFSX_PULL:
         if (!pull_possible) fail(BFSX_ERR_UNSUPPORTED, "bottom-up step on a non-symmetric graph");
         // (the queue is valid whenever the bitmap is not: only a bottom-up
@@ -194,11 +207,12 @@ struct Walker {
         break;
       default: fail(BFSX_ERR_ARG, "unknown step kind");
     }
-    if (use_mark) {
+    if (bucketed) {
+      new_fbm = td_fbm_next != nullptr;
+    } else if (use_mark) {
       ctx->mark_dirty = true;
-      uint32_t *fbm_next = pull_possible ? ctx->fbm[fbm_cur ^ 1].as<uint32_t>() : nullptr;
-      compact(ctx, a, a.mark, fbm_next);
-      new_fbm = fbm_next != nullptr;
+      compact(ctx, a, a.mark, td_fbm_next);
+      new_fbm = td_fbm_next != nullptr;
     }
     level_end(ctx, cur, cur ^ 1, stamp, choice);
     last = choice;
@@ -213,6 +227,7 @@ struct Walker {
     queue_valid = last != BFSX_PULL;
     F = ctx->h_ctl->q_count[cur ^ 1];
     EF = ctx->h_ctl->q_edges[cur ^ 1];
+    VE = ctx->h_ctl->visited_edges;
     cur ^= 1;
     level++;
   }
@@ -242,6 +257,7 @@ static Walker start_search(bfsx_ctx *ctx, uint32_t root, bool pull_possible, boo
   w.stamp = stamp;
   w.par = par;
   w.EF = ctx->h_ctl->q_edges[0];
+  w.VE = ctx->h_ctl->visited_edges;
   return w;
 }
 
@@ -352,6 +368,8 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     visited_e = ctx->h_ctl->visited_edges;
   }
   while (w.F > 0) {
+    // a search has at most n_cols levels; more means a broken invariant
+    if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
     const int choice = learned ? (int)ctx->h_ctl->choice : fixed;
     const uint32_t level = w.level, F = w.F, cur = w.cur;
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
@@ -447,6 +465,7 @@ static uint32_t record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, u
     decide(ctx, 0, true, allow);
     read_ctl(ctx);
     while (w.F > 0 && rows < max_rows) {
+      if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
       for (int k = 0; k < 8; k++) X[(size_t)rows * 8 + k] = ctx->h_ctl->sched_in[k];
       snapshot(ctx, true);
       const Walker saved = w;
@@ -1089,6 +1108,7 @@ int bfsx_autotune(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, int st
         {&bfsx_params::llb_wg, {64, 128, 256, 512, 1024}, L | S},
         {&bfsx_params::hub_degree, {0, 256, 1024, 4096, 16384}, L | S},
         {&bfsx_params::mark_ratio_q10, {4, 16, 64, 256, 1024}, L | S | G},
+        {&bfsx_params::bucket_ratio_q10, {0, 256, 512, 768, 0xFFFFFFFFu}, G},
         {&bfsx_params::glb_wg, {128, 256, 512, 1024}, G},
         {&bfsx_params::glb_blocks_per_cu, {2, 4, 8, 16, 32}, G},
         {&bfsx_params::glb_chunk, {1024, 2048, 4096, 8192, 16384}, G},

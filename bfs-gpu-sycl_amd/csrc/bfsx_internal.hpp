@@ -77,6 +77,7 @@ struct bfsx_ctx {
   bfsx::DevBuf levels, parents, vis, fbm[2], queue[2], ctl, mark;
   bfsx::DevBuf glb_scan, glb_start, glb_tile, glb_chunk_lo;
   bfsx::DevBuf fsum;     // bottom-up step: frontier summary bitmap (PULL_SUM_WORDS_MAX)
+  bfsx::DevBuf bkt, bkt_cnt, bkt_tot, bkt_base;  // GLB bucketed discovery
   bfsx::DevBuf scratch;  // generic scratch (reductions, root picking)
   bfsx::LevelCtl *h_ctl = nullptr;  // pinned mirror
   bool mark_dirty = false;          // a mark step did not reach its compaction
@@ -153,11 +154,16 @@ struct StepArgs {
   uint8_t *mark;           // non-null: mark-then-compact step
   bool count_examined;     // bottom-up: add adjacency entries read to ctl->examined
   hipEvent_t ev_k0, ev_k1; // non-null: recorded around the step's main kernel
+  unsigned long long EF;   // frontier edges (out-degree sum of q_in)
+  bool bucket_ok;          // a marking GLB step may use bucketed discovery
 };
 
 void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *parents);
 void step_llb(bfsx_ctx *ctx, const StepArgs &a, bool strided);
-void step_glb(bfsx_ctx *ctx, const StepArgs &a);
+// Returns true when the step discovered with buckets (heavy level, no
+// parents): the next queue, levels, visited bits and (if a.fbm_next) the
+// next frontier bitmap are then final and no compaction follows.
+bool step_glb(bfsx_ctx *ctx, const StepArgs &a);
 void step_pull(bfsx_ctx *ctx, const StepArgs &a);
 void bitmap_to_queue(bfsx_ctx *ctx, const uint32_t *bm, uint32_t *q);
 void queue_to_fbm(bfsx_ctx *ctx, const uint32_t *q, uint32_t F, uint32_t *fbm);

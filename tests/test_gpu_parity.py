@@ -153,6 +153,27 @@ def test_mark_and_atomic_discovery_modes(engine, ratio):
         engine.set_params(**old)
 
 
+@pytest.mark.parametrize("bucket", [0, 512, 0xFFFFFFFF])
+def test_glb_bucketed_discovery(engine, bucket):
+    """GLB levels that mark use bucketed discovery always (0), when most
+    edges lead to unvisited vertices (512) or never; the next frontier
+    bitmap it writes feeds bottom-up steps of the learned schedule."""
+    engine.generate_rmat(16, 16, 0.57, 0.19, 0.19, 91, 92, 8)
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(8, 93)
+    old = engine.params
+    try:
+        engine.set_params(mark_ratio_q10=0, bucket_ratio_q10=bucket)
+        assert_levels(engine, cp, ri, roots, ["glb"])
+        engine.load_scheduler()
+        assert_levels(engine, cp, ri, roots, ["learned"])
+        engine.set_params(mark_ratio_q10=16)
+        assert_levels(engine, cp, ri, roots, ["glb", "learned"])
+    finally:
+        engine.set_scheduler_weights(None)
+        engine.set_params(**old)
+
+
 @pytest.mark.parametrize("glb_wg,chunk,bpc", [(128, 256, 1), (256, 512, 4), (1024, 65536, 2)])
 def test_glb_shapes(engine, glb_wg, chunk, bpc):
     engine.generate_rmat(13, 16, 0.57, 0.19, 0.19, 41, 42, 4)
