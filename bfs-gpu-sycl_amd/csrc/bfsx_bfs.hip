@@ -493,12 +493,28 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
 // Every slice fits in LDS (2^16 vertices: 8 KiB visited + 8 KiB found).
 constexpr uint32_t BKT_SHIFT = 16;
 constexpr uint32_t BKT_MAX = 2048;  // slices: graphs of up to 2^27 vertices
+constexpr int BKT_WG = 1024;        // work-group size of the two passes
 
 // Calls act(target) for every edge of [lo, hi) of the staged window (as
-// expand_edges: four consecutive edges per lane, all loads first).
+// expand_edges: four consecutive edges per lane, all loads first); a range
+// inside one list (a hub's chunk) needs no owner search.  In pass 2 the
+// lanes that hit the same slice in one instruction get consecutive slots
+// from the LDS cursor, so the stores of a hub's sorted list coalesce.
 template <typename PtrT, int WG, class Act>
 __device__ __forceinline__ void window_edges(const KArgs &a, const PtrT *s_scan,
                                              const PtrT *s_start, PtrT lo, PtrT hi, Act act) {
+  if (lo < hi && s_scan[1] >= hi) {
+    const PtrT off = s_start[0] - s_scan[0];
+    for (PtrT base = lo + (PtrT)threadIdx.x * EPT; base < hi; base += (PtrT)WG * EPT) {
+      uint32_t n[EPT];
+#pragma unroll
+      for (int k = 0; k < EPT; k++) n[k] = base + k < hi ? a.row_idx[off + base + k] : 0u;
+#pragma unroll
+      for (int k = 0; k < EPT; k++)
+        if (base + k < hi) act(n[k]);
+    }
+    return;
+  }
   for (PtrT base = lo + (PtrT)threadIdx.x * EPT; base < hi; base += (PtrT)WG * EPT) {
     uint32_t j = 0;
 #pragma unroll
@@ -532,7 +548,6 @@ __global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
                                                    uint32_t *bkt, uint64_t cap) {
   __shared__ PtrT s_scan[WG];
   __shared__ PtrT s_start[WG];
-  __shared__ uint32_t s_v[WG];
   __shared__ PtrT s_win_lo, s_win_hi;
   __shared__ uint32_t s_cnt[BKT_MAX];
   __shared__ uint64_t s_base[PASS == 2 ? BKT_MAX : 1];
@@ -578,7 +593,6 @@ __global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
       __syncthreads();
     }
   }
-  (void)s_v;
   if (PASS == 1)
     for (uint32_t i = t; i < n_slices; i += WG) row[i] = s_cnt[i];
 }
@@ -635,10 +649,9 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
                                                      unsigned long long *q_edges,
                                                      uint32_t *fbm_next) {
   constexpr uint32_t WPS = 1u << (BKT_SHIFT - 5);  // visited words per slice
-  static_assert(WPS % WG == 0, "a thread owns whole words");
+  static_assert(WPS % (WG / 64) == 0 && (WPS / (WG / 64)) % 64 == 0, "waves own whole rounds");
   __shared__ uint32_t s_vis[WPS];
   __shared__ uint32_t s_found[WPS];
-  __shared__ uint32_t s_wave[WG / 64 + 1];
   __shared__ uint32_t s_qbase;
   const uint32_t k = blockIdx.x;
   const uint32_t n_words = (uint32_t)bitmap_words(n_cols);
@@ -650,40 +663,71 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
   __syncthreads();
   const uint64_t p0 = base[k], p1 = base[k + 1] < cap ? base[k + 1] : cap;
   const uint32_t slice0 = k << BKT_SHIFT;
-  for (uint64_t p = p0 + threadIdx.x; p < p1; p += WG) {
-    const uint32_t l = bkt[p] - slice0;
-    const uint32_t bit = 1u << (l & 31);
-    if (!(s_vis[l >> 5] & bit)) atomicOr(&s_found[l >> 5], bit);
+  // eight loads in flight per thread, then the LDS updates
+  constexpr uint32_t U = 8;
+  for (uint64_t p = p0 + threadIdx.x; p < p1; p += (uint64_t)WG * U) {
+    uint32_t t[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint64_t q = p + (uint64_t)u * WG;
+      t[u] = q < p1 ? bkt[q] - slice0 : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t l = t[u];
+      if (l == 0xFFFFFFFFu) continue;
+      const uint32_t bit = 1u << (l & 31);
+      if (!(s_vis[l >> 5] & bit)) atomicOr(&s_found[l >> 5], bit);
+    }
   }
   __syncthreads();
-  // thread t owns words [t * RUN, (t + 1) * RUN) of the slice: queue order
-  constexpr uint32_t RUN = WPS / WG;
+  // Output.  Wave w owns words [w * WPW, (w + 1) * WPW) of the slice.
+  // Pass A: visited / next-frontier words and the wave's count; one atomic
+  // per group reserves the queue slots.  Pass B: 64 consecutive vertices
+  // per round, one per lane (coalesced col_ptr reads), slots by ballot and
+  // mbcnt, so the queue is in vertex order.
+  constexpr uint32_t NW = WG / 64;
+  constexpr uint32_t WPW = WPS / NW;  // words per wave
+  __shared__ uint32_t s_wcnt[NW + 1];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t cnt = 0;
-#pragma unroll
-  for (uint32_t r = 0; r < RUN; r++) {
-    const uint32_t i = threadIdx.x * RUN + r;
-    cnt += __popc(s_found[i] & ~s_vis[i]);
-  }
-  uint32_t total;
-  const uint32_t ex = block_excl_scan<WG>(cnt, total, s_wave);
-  if (threadIdx.x == 0) s_qbase = total ? atomicAdd(q_count, total) : 0u;
-  __syncthreads();
-  uint32_t pos = s_qbase + ex;
-  unsigned long long dsum = 0;
-#pragma unroll
-  for (uint32_t r = 0; r < RUN; r++) {
-    const uint32_t i = threadIdx.x * RUN + r;
+  for (uint32_t i = wv * WPW + lane; i < (wv + 1) * WPW; i += 64) {
     const uint32_t w = w0 + i;
     if (w >= n_words) continue;
     const uint32_t nw = s_found[i] & ~s_vis[i];
+    s_found[i] = nw;
     if (nw) vis[w] = s_vis[i] | nw;
     if (fbm_next) fbm_next[w] = nw;
-    for (uint32_t b = nw; b; b &= b - 1) {
-      const uint32_t v = w * 32 + (__ffs(b) - 1);
+    cnt += __popc(nw);
+  }
+  cnt = wave_sum(cnt);
+  if (lane == 0) s_wcnt[wv] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (uint32_t w = 0; w < NW; w++) {
+      const uint32_t c = s_wcnt[w];
+      s_wcnt[w] = run;
+      run += c;
+    }
+    s_qbase = run ? atomicAdd(q_count, run) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos = s_qbase + s_wcnt[wv];
+  unsigned long long dsum = 0;
+  for (uint32_t i = wv * WPW; i < (wv + 1) * WPW; i += 2) {  // two words = 64 vertices
+    if (w0 + i >= n_words) break;
+    const uint32_t word = s_found[i + (lane >> 5)];
+    const bool isnew = (word >> (lane & 31)) & 1u;
+    const unsigned long long m = __ballot(isnew);
+    if (!m) continue;
+    if (isnew) {
+      const uint32_t v = (w0 + i) * 32 + lane;
       levels[v] = next_level;
-      q_out[pos++] = v;
+      q_out[pos + lanes_below(m)] = v;
       if (q_edges) dsum += out_degree(col_ptr, v, n_rows);
     }
+    pos += (uint32_t)__popcll(m);
   }
   if (q_edges) block_atomic_add<WG>(dsum, q_edges);
 }
@@ -1347,13 +1391,13 @@ static void glb_bucketed(bfsx_ctx *ctx, const StepArgs &s, const KArgs &a, const
   uint32_t *bkt = ctx->bkt.as<uint32_t>();
   const uint32_t *clo = ctx->glb_chunk_lo.as<uint32_t>();
   KernelTimer kt(s, st);
-  hipLaunchKernelGGL((k_glb_bucket<PtrT, 256, 1>), dim3(grid), dim3(256), 0, st, a, g_scan,
+  hipLaunchKernelGGL((k_glb_bucket<PtrT, BKT_WG, 1>), dim3(grid), dim3(BKT_WG), 0, st, a, g_scan,
                      g_start, clo, s.ctl, chunk, n_slices, cnt, (const uint64_t *)nullptr,
                      (uint32_t *)nullptr, (uint64_t)0);
   hipLaunchKernelGGL((k_bucket_colscan<256>), dim3(n_slices), dim3(256), 0, st, cnt,
                      (uint32_t)grid, n_slices, tot);
   hipLaunchKernelGGL(k_bucket_base, dim3(1), dim3(1024), 0, st, tot, n_slices, base);
-  hipLaunchKernelGGL((k_glb_bucket<PtrT, 256, 2>), dim3(grid), dim3(256), 0, st, a, g_scan,
+  hipLaunchKernelGGL((k_glb_bucket<PtrT, BKT_WG, 2>), dim3(grid), dim3(BKT_WG), 0, st, a, g_scan,
                      g_start, clo, s.ctl, chunk, n_slices, cnt, base, bkt, cap);
   hipLaunchKernelGGL((k_bucket_apply<PtrT, 512>), dim3(n_slices), dim3(512), 0, st,
                      static_cast<const PtrT *>(s.col_ptr), s.n_rows, s.n_cols, bkt, base, cap, s.vis,
@@ -1384,7 +1428,9 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   const unsigned grid = (unsigned)ctx->cu_count * ctx->params.glb_blocks_per_cu;
   const uint32_t n_slices = (uint32_t)(((uint64_t)ctx->n_cols + (1u << BKT_SHIFT) - 1) >> BKT_SHIFT);
   if (s.mark && s.bucket_ok && !s.parents && n_slices <= BKT_MAX) {
-    glb_bucketed<PtrT>(ctx, s, a, g_scan, g_start, chunk, grid, n_slices);
+    // fewer, larger groups than the GLB expand grid: longer runs per
+    // (group, slice) in the segments that pass 2 writes
+    glb_bucketed<PtrT>(ctx, s, a, g_scan, g_start, chunk, (unsigned)ctx->cu_count * 2, n_slices);
     return true;
   }
   KernelTimer kt(s, st);
