@@ -27,7 +27,7 @@ WORKLOADS = {
 # 7200+k); none of them is an evaluation workload above.
 TRAIN_WORKLOADS = {}
 for _k, (_scale, _ef) in enumerate([(16, 16), (18, 16), (20, 8), (20, 16), (20, 32), (22, 16),
-                                     (23, 16), (24, 16)]):
+                                     (23, 16), (24, 16), (25, 16)]):
     TRAIN_WORKLOADS[f"train_rmat{_scale}_ef{_ef}"] = dict(
         kind="rmat", scale=_scale, edge_factor=_ef, seed_graph=7000 + _k, seed_perm=7100 + _k,
         seed_roots=7200 + _k, n_roots=16, ptr_bytes=4)
