@@ -580,7 +580,49 @@ __global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
       __syncthreads();
       const PtrT lo = e0 > s_win_lo ? e0 : s_win_lo;
       const PtrT hi = e1 < s_win_hi ? e1 : s_win_hi;
-      if (PASS == 1) {
+      if (lo < hi && s_scan[1] >= hi) {
+        // The range lies in one sorted list (a hub's chunk).  Instruction k
+        // of a wave takes 64 consecutive edges, one per lane, so equal
+        // slices form runs of adjacent lanes: the first lane of each run
+        // updates the LDS counter once for the run (no same-address
+        // conflicts), and in pass 2 the run's lanes store to consecutive
+        // slots.
+        const PtrT off = s_start[0] - s_scan[0];
+        const uint32_t lane = t & 63;
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        for (PtrT b0 = lo + (PtrT)(t & ~63u) * EPT; b0 < hi; b0 += (PtrT)WG * EPT) {
+          uint32_t n[EPT];
+#pragma unroll
+          for (int k = 0; k < EPT; k++) {
+            const PtrT e = b0 + (PtrT)k * 64 + lane;
+            n[k] = e < hi ? a.row_idx[off + e] : 0u;
+          }
+#pragma unroll
+          for (int k = 0; k < EPT; k++) {
+            const bool v = b0 + (PtrT)k * 64 + lane < hi;
+            const uint32_t sl = v ? n[k] >> BKT_SHIFT : 0xFFFFFFFFu;
+            const uint32_t prev = __shfl_up(sl, 1, 64);
+            const bool head = v && (lane == 0 || prev != sl);
+            const unsigned long long hm = __ballot(head);
+            if (!hm) continue;
+            const unsigned long long vm = __ballot(v);
+            uint32_t slot = 0;
+            if (head) {
+              const unsigned long long later = hm & ~upto;
+              const uint32_t end = later ? (uint32_t)(__ffsll(later) - 1) : (uint32_t)__popcll(vm);
+              slot = atomicAdd(&s_cnt[sl], end - lane);
+            }
+            if (PASS == 2) {
+              const uint32_t h = 63u - (uint32_t)__clzll(hm & upto);
+              const uint32_t hs = __shfl(slot, (int)h, 64);
+              if (v) {
+                const uint64_t p = s_base[sl] + hs + (lane - h);
+                if (p < cap) bkt[p] = n[k];
+              }
+            }
+          }
+        }
+      } else if (PASS == 1) {
         window_edges<PtrT, WG>(a, s_scan, s_start, lo, hi,
                                [&](uint32_t n) { atomicAdd(&s_cnt[n >> BKT_SHIFT], 1u); });
       } else {
@@ -715,19 +757,31 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
   __syncthreads();
   uint32_t pos = s_qbase + s_wcnt[wv];
   unsigned long long dsum = 0;
-  for (uint32_t i = wv * WPW; i < (wv + 1) * WPW; i += 2) {  // two words = 64 vertices
-    if (w0 + i >= n_words) break;
-    const uint32_t word = s_found[i + (lane >> 5)];
-    const bool isnew = (word >> (lane & 31)) & 1u;
-    const unsigned long long m = __ballot(isnew);
-    if (!m) continue;
-    if (isnew) {
-      const uint32_t v = (w0 + i) * 32 + lane;
-      levels[v] = next_level;
-      q_out[pos + lanes_below(m)] = v;
-      if (q_edges) dsum += out_degree(col_ptr, v, n_rows);
+  // eight rounds at a time, so their col_ptr loads are in flight together
+  constexpr uint32_t R = 8;
+  for (uint32_t i0 = wv * WPW; i0 < (wv + 1) * WPW; i0 += 2 * R) {  // two words = 64 vertices
+    PtrT lo_[R], hi_[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++) {
+      const uint32_t i = i0 + 2 * r;
+      lo_[r] = hi_[r] = 0;
+      if (w0 + i >= n_words) continue;
+      const uint32_t word = s_found[i + (lane >> 5)];
+      const bool isnew = (word >> (lane & 31)) & 1u;
+      const unsigned long long m = __ballot(isnew);
+      if (isnew) {
+        const uint32_t v = (w0 + i) * 32 + lane;
+        levels[v] = next_level;
+        q_out[pos + lanes_below(m)] = v;
+        if (q_edges && v < n_rows) {
+          lo_[r] = col_ptr[v];
+          hi_[r] = col_ptr[v + 1];
+        }
+      }
+      pos += (uint32_t)__popcll(m);
     }
-    pos += (uint32_t)__popcll(m);
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++) dsum += (unsigned long long)(hi_[r] - lo_[r]);
   }
   if (q_edges) block_atomic_add<WG>(dsum, q_edges);
 }
