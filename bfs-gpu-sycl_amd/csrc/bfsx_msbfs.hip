@@ -171,24 +171,40 @@ __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_tot
     const bool valid = u < a.n_cols;
     const size_t o = (size_t)u * W + j;
     const uint32_t s = valid ? a.seen[o] : ~0u;
-    // a group whose vertex every search has reached (or that cannot expand) skips
-    bool todo = valid && u < a.n_rows && s != ~0u;
+    // a group whose vertex every search has reached (or that cannot expand)
+    // skips; vertices with more than MS_HUB neighbours are k_ms_pull_hubs'
+    const bool row = valid && u < a.n_rows;
+    const PtrT b = row ? col_ptr[u] : PtrT(0), e = row ? col_ptr[u + 1] : PtrT(0);
+    const bool hub = (uint64_t)(e - b) > MS_HUB;
+    bool todo = row && !hub && s != ~0u;
     const bool group_todo = (__ballot(todo) & gmask) != 0;
     uint32_t acc = 0;
     if (group_todo) {
-      const PtrT b = col_ptr[u], e = col_ptr[u + 1];
-      for (PtrT k = b; k < e; k++) {
-        const uint32_t w = a.row_idx[k];
-        acc |= a.cur[(size_t)w * W + j];
-        if (((k - b) & 3) == 3) {
-          // stop once every unseen bit of the group's vertex has arrived
-          const bool full = (acc | s) == ~0u;
-          if ((__ballot(!full) & gmask) == 0) break;
+      // eight neighbours per round: the group's lanes load their ids in one
+      // coalesced read, then every lane loads its word of all eight frontier
+      // bitsets (eight independent loads in flight)
+      constexpr int PER = 8 / W;
+      for (PtrT k = b; k < e; k += 8) {
+        uint32_t ids[PER];
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+          const PtrT kk = k + (PtrT)(t * W + j);
+          ids[t] = kk < e ? a.row_idx[kk] : 0xFFFFFFFFu;
         }
+        uint32_t part = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          const uint32_t w = __shfl(ids[i / W], (int)(grp * W + (i % W)), 64);
+          if (w != 0xFFFFFFFFu) part |= a.cur[(size_t)w * W + j];
+        }
+        acc |= part;
+        // stop once every unseen bit of the group's vertex has arrived
+        const bool full = (acc | s) == ~0u;
+        if ((__ballot(!full) & gmask) == 0) break;
       }
     }
     const uint32_t nb = acc & ~s;
-    if (valid) {
+    if (valid && !hub) {
       if (nb) a.seen[o] = s | nb;
       a.nxt[o] = nb;
     }
@@ -201,6 +217,78 @@ __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_tot
     ms_append(app, u, a.q_out, a.q_count_out);
     ms_track<PtrT>(col_ptr, a.n_rows, app, u, a.q_edges_out);
   }
+}
+
+// Bottom-up for the vertices with more than MS_HUB neighbours: a work-group
+// per hub, its W-lane groups striding over the list (eight neighbours per
+// round as in k_ms_pull), partial words ORed in LDS.
+template <typename PtrT, int W>
+__global__ __launch_bounds__(256) void k_ms_pull_hubs(MsArgs a, const uint32_t *hubs) {
+  const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
+  constexpr uint32_t NG = 256 / W;  // groups per work-group
+  constexpr int PER = 8 / W;
+  __shared__ uint32_t s_acc[W];
+  __shared__ uint32_t s_seen[W];
+  const uint32_t t = threadIdx.x, j = t % W, grp = t / W, lane = t & 63;
+  const uint32_t u = hubs[blockIdx.x];
+  if (t < W) {
+    s_acc[t] = 0;
+    s_seen[t] = a.seen[(size_t)u * W + t];
+  }
+  __syncthreads();
+  const uint32_t s = s_seen[j];
+  bool all = true;
+#pragma unroll
+  for (int w = 0; w < W; w++) all = all && s_seen[w] == ~0u;
+  if (!all) {  // uniform over the work-group
+    const unsigned long long gmask = ((W == 64) ? ~0ull : ((1ull << W) - 1)) << ((lane / W) * W);
+    const PtrT b = col_ptr[u], e = col_ptr[u + 1];
+    uint32_t acc = 0;
+    for (PtrT k = b + (PtrT)grp * 8; k < e; k += (PtrT)NG * 8) {
+      uint32_t ids[PER];
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const PtrT kk = k + (PtrT)(q * W + j);
+        ids[q] = kk < e ? a.row_idx[kk] : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const uint32_t w = __shfl(ids[i / W], (int)((lane / W) * W + (i % W)), 64);
+        if (w != 0xFFFFFFFFu) acc |= a.cur[(size_t)w * W + j];
+      }
+      const bool full = (acc | s) == ~0u;
+      if ((__ballot(!full) & gmask) == 0) break;
+    }
+    if (acc & ~s) atomicOr(&s_acc[j], acc & ~s);
+  }
+  __syncthreads();
+  if (t < W) {
+    const uint32_t nb = s_acc[t];
+    const size_t o = (size_t)u * W + t;
+    if (nb) {
+      a.seen[o] = s_seen[t] | nb;
+      if (a.level > 254) atomicOr(a.overflow, 1u);
+      else ms_write_levels(a.lv, 32 * W, u, t, nb, (uint8_t)a.level);
+    }
+    a.nxt[o] = nb;
+  }
+  if (t == 0) {
+    bool any = false;
+    for (int w = 0; w < W; w++) any = any || s_acc[w] != 0;
+    if (any) {
+      a.q_out[atomicAdd(a.q_count_out, 1u)] = u;
+      atomicAdd(a.q_edges_out, (unsigned long long)(col_ptr[u + 1] - col_ptr[u]));
+    }
+  }
+}
+
+// vertices with more than MS_HUB neighbours (once per batch, outside the timer)
+template <typename PtrT>
+__global__ void k_ms_hub_list(const PtrT *col_ptr, uint32_t n_rows, uint32_t *hubs,
+                              uint32_t *n_hubs) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_rows;
+       u += gridDim.x * blockDim.x)
+    if ((uint64_t)(col_ptr[u + 1] - col_ptr[u]) > MS_HUB) hubs[atomicAdd(n_hubs, 1u)] = u;
 }
 
 // ---- set-up ------------------------------------------------------------------
@@ -315,6 +403,18 @@ static MsResult ms_run_t(bfsx_ctx *ctx, const uint32_t *d_src, const uint16_t *d
   BFSX_HIP(hipMemcpyAsync(ctx->queue[0].p, distinct_src.data(),
                           sizeof(uint32_t) * distinct_src.size(), hipMemcpyHostToDevice, s));
   BFSX_HIP(hipMemsetAsync(ctl, 0, sizeof(LevelCtl), s));
+  // vertices that bottom-up levels hand to k_ms_pull_hubs
+  uint32_t n_phubs = 0;
+  if (symmetric) {
+    ensure(ctx->ms_phubs, sizeof(uint32_t) * ((uint64_t)ctx->n_rows + 1), s);
+    uint32_t *d_np = ctx->scratch.as<uint32_t>() + 12;
+    BFSX_HIP(hipMemsetAsync(d_np, 0, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_ms_hub_list<PtrT>, dim3(ms_grid(ctx->n_rows, 256, cap)), dim3(256), 0, s,
+                       static_cast<const PtrT *>(ctx->col_ptr.p), ctx->n_rows,
+                       ctx->ms_phubs.as<uint32_t>(), d_np);
+    BFSX_HIP(hipGetLastError());
+    BFSX_HIP(hipMemcpyAsync(&n_phubs, d_np, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  }
   BFSX_HIP(hipStreamSynchronize(s));
 
   MsResult r;
@@ -357,6 +457,9 @@ static MsResult ms_run_t(bfsx_ctx *ctx, const uint32_t *d_src, const uint16_t *d
     } else {
       hipLaunchKernelGGL((k_ms_pull<PtrT, W>), dim3(ms_grid((uint64_t)n_groups * 64, 256, cap)),
                          dim3(256), 0, s, a, n_groups);
+      if (n_phubs)
+        hipLaunchKernelGGL((k_ms_pull_hubs<PtrT, W>), dim3(n_phubs), dim3(256), 0, s, a,
+                           (const uint32_t *)ctx->ms_phubs.as<uint32_t>());
       r.pull_levels++;
     }
     BFSX_HIP(hipGetLastError());
