@@ -76,6 +76,10 @@ __device__ __forceinline__ void ms_track(const PtrT *col_ptr, uint32_t n_rows, b
 // Active vertices with more than MS_HUB edges go to a list that every wave
 // then splits (k_ms_push_hubs); the rest are expanded by one wave each.
 constexpr uint32_t MS_HUB = 2048;
+// Bottom-up levels hand vertices with more neighbours than this to
+// k_ms_pull_hubs (a work-group each): a W-lane group scanning a long list
+// alone would hold its wave for hundreds of rounds.
+constexpr uint32_t MS_PULL_HUB = 256;
 
 template <typename PtrT, int W>
 __device__ __forceinline__ void ms_push_edges(const MsArgs &a, const PtrT *col_ptr, uint32_t v,
@@ -172,10 +176,10 @@ __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_tot
     const size_t o = (size_t)u * W + j;
     const uint32_t s = valid ? a.seen[o] : ~0u;
     // a group whose vertex every search has reached (or that cannot expand)
-    // skips; vertices with more than MS_HUB neighbours are k_ms_pull_hubs'
+    // skips; vertices with more than MS_PULL_HUB neighbours are k_ms_pull_hubs'
     const bool row = valid && u < a.n_rows;
     const PtrT b = row ? col_ptr[u] : PtrT(0), e = row ? col_ptr[u + 1] : PtrT(0);
-    const bool hub = (uint64_t)(e - b) > MS_HUB;
+    const bool hub = (uint64_t)(e - b) > MS_PULL_HUB;
     bool todo = row && !hub && s != ~0u;
     const bool group_todo = (__ballot(todo) & gmask) != 0;
     uint32_t acc = 0;
@@ -219,7 +223,7 @@ __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_tot
   }
 }
 
-// Bottom-up for the vertices with more than MS_HUB neighbours: a work-group
+// Bottom-up for the vertices with more than MS_PULL_HUB neighbours: a work-group
 // per hub, its W-lane groups striding over the list (eight neighbours per
 // round as in k_ms_pull), partial words ORed in LDS.
 template <typename PtrT, int W>
@@ -282,13 +286,13 @@ __global__ __launch_bounds__(256) void k_ms_pull_hubs(MsArgs a, const uint32_t *
   }
 }
 
-// vertices with more than MS_HUB neighbours (once per batch, outside the timer)
+// vertices with more than MS_PULL_HUB neighbours (once per batch, outside the timer)
 template <typename PtrT>
 __global__ void k_ms_hub_list(const PtrT *col_ptr, uint32_t n_rows, uint32_t *hubs,
                               uint32_t *n_hubs) {
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_rows;
        u += gridDim.x * blockDim.x)
-    if ((uint64_t)(col_ptr[u + 1] - col_ptr[u]) > MS_HUB) hubs[atomicAdd(n_hubs, 1u)] = u;
+    if ((uint64_t)(col_ptr[u + 1] - col_ptr[u]) > MS_PULL_HUB) hubs[atomicAdd(n_hubs, 1u)] = u;
 }
 
 // ---- set-up ------------------------------------------------------------------
