@@ -73,13 +73,20 @@ KERNEL_NAMES = {"llb": "k_llb", "slb": "k_llb (strided)", "glb": "k_glb_expand +
 def roofline_leg(eng, roots, strategy, info):
     """Per-level kernel times (HIP events around each step's main kernel on
     the engine stream) and level histograms -> algorithmic bytes / time of
-    the dominant step kind's kernel."""
+    the dominant step kind's kernel.
+
+    `achieved` charges every launch the top-down bytes of its level
+    (algorithmic_bytes), whatever the step kind: the fixed denominator that
+    makes GTEPS comparable across kinds.  `achieved_own_model` charges a
+    bottom-up launch what it must read itself (pull_bytes, with the
+    adjacency entries it examined)."""
     import bfsx
     w_ptr = info["col_ptr_bytes"]
     n = float(info["n_cols"])
     live = n - float(info["n_isolated"])  # vertices a bottom-up step may test
     pull = bfsx.STEP_NAMES.index("pull")
-    per_kind_bytes = np.zeros(4)
+    per_kind_bytes = np.zeros(4)   # top-down bytes of the levels
+    per_kind_own = np.zeros(4)     # the kind's own model
     per_kind_us = np.zeros(4)
     per_kind_step_us = np.zeros(4)
     per_kind_launches = np.zeros(4)
@@ -94,16 +101,19 @@ def roofline_leg(eng, roots, strategy, info):
             E = float(edges[L])
             D = float(counts[L + 1]) if L + 1 < len(counts) else 0.0
             k = rec["choice"]
+            td = algorithmic_bytes(F, E, D, w_ptr)
+            per_kind_bytes[k] += td
             if k == pull:
-                per_kind_bytes[k] += pull_bytes(n, live - reached[L], float(rec["pull_scanned"]),
-                                                D, w_ptr)
+                per_kind_own[k] += pull_bytes(n, live - reached[L], float(rec["pull_scanned"]),
+                                              D, w_ptr)
             else:
-                per_kind_bytes[k] += algorithmic_bytes(F, E, D, w_ptr)
+                per_kind_own[k] += td
             per_kind_us[k] += rec["kernel_us"]
             per_kind_step_us[k] += rec["us"]
             per_kind_launches[k] += 1
     k = int(np.argmax(per_kind_us))
     achieved = per_kind_bytes[k] / (per_kind_us[k] * 1e-6) / 1e9
+    own = per_kind_own[k] / (per_kind_us[k] * 1e-6) / 1e9
     name = bfsx.STEP_NAMES[k]
     return {
         "kernel": KERNEL_NAMES[name],
@@ -118,6 +128,8 @@ def roofline_leg(eng, roots, strategy, info):
         "avg_launch_us": round(per_kind_us[k] / max(per_kind_launches[k], 1), 2),
         "avg_step_us": round(per_kind_step_us[k] / max(per_kind_launches[k], 1), 2),
         "alg_bytes_per_launch": round(per_kind_bytes[k] / max(per_kind_launches[k], 1), 1),
+        "achieved_own_model": round(own, 1),
+        "own_bytes_per_launch": round(per_kind_own[k] / max(per_kind_launches[k], 1), 1),
     }
 
 

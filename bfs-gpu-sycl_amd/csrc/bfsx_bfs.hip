@@ -1481,7 +1481,7 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   const KArgs a = kargs(s, 0);
   const unsigned grid = (unsigned)ctx->cu_count * ctx->params.glb_blocks_per_cu;
   const uint32_t n_slices = (uint32_t)(((uint64_t)ctx->n_cols + (1u << BKT_SHIFT) - 1) >> BKT_SHIFT);
-  if (s.mark && s.bucket_ok && !s.parents && n_slices <= BKT_MAX) {
+  if (s.bucket_ok && !s.parents && n_slices <= BKT_MAX) {
     // fewer, larger groups than the GLB expand grid: longer runs per
     // (group, slice) in the segments that pass 2 writes
     glb_bucketed<PtrT>(ctx, s, a, g_scan, g_start, chunk, (unsigned)ctx->cu_count * 2, n_slices);

@@ -178,8 +178,12 @@ struct Walker {
     a.EF = EF;
     const uint32_t br = ctx->params.bucket_ratio_q10;
     const unsigned long long UE = ctx->nnz > VE ? ctx->nnz - VE : 0ull;
-    a.bucket_ok = br != 0xFFFFFFFFu && (double)UE * 1024.0 >= (double)ctx->nnz * br;
     const bool use_mark = choice != BFSX_PULL && EF >= mark_min_edges(ctx);
+    // bucketed GLB from n_cols/256 frontier edges on (below that its fixed
+    // cost, about a pass over the visited bitmap, exceeds exact-once's
+    // atomics) or wherever a marking step would run
+    a.bucket_ok = br != 0xFFFFFFFFu && (double)UE * 1024.0 >= (double)ctx->nnz * br &&
+                  (use_mark || EF * 256 >= ctx->n_cols) && mark_min_edges(ctx) != ~0ull;
     a.mark = use_mark ? ctx->mark.as<uint8_t>() : nullptr;
     // next frontier bitmap of a top-down step (kept for bottom-up steps)
     uint32_t *td_fbm_next = pull_possible ? ctx->fbm[fbm_cur ^ 1].as<uint32_t>() : nullptr;

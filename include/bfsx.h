@@ -123,13 +123,15 @@ typedef struct bfsx_params {
                                afterwards; smaller ones discover exact-once
                                with atomics (0 = always mark,
                                0xFFFFFFFF = never) */
-  uint32_t bucket_ratio_q10; /* a GLB step that would mark uses bucketed
-                               discovery instead (sequential writes of
-                               every target, no random stores) when the
-                               unvisited vertices' edges are at least
-                               nnz * bucket_ratio_q10 / 1024, i.e. when
+  uint32_t bucket_ratio_q10; /* a GLB step with at least n_cols/256
+                               frontier edges (or that would mark) uses
+                               bucketed discovery (sequential writes of
+                               every target, no random stores or atomics)
+                               when the unvisited vertices' edges are at
+                               least nnz * bucket_ratio_q10 / 1024, i.e. when
                                most frontier edges lead to new vertices
-                               (0 = always, 0xFFFFFFFF = never; no parents,
+                               (0 = always, 0xFFFFFFFF = never; not with
+                               mark_ratio_q10 = 0xFFFFFFFF or parents;
                                graphs of up to 2^27 vertices) */
 } bfsx_params;
 
