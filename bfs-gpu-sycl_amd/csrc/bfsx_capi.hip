@@ -52,7 +52,8 @@ static void free_search_state(bfsx_ctx *ctx) {
                     &ctx->queue[0], &ctx->queue[1], &ctx->glb_scan, &ctx->glb_start,
                     &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark, &ctx->fsum, &ctx->bkt,
                     &ctx->bkt_cnt, &ctx->bkt_tot, &ctx->bkt_base, &ctx->ms_seen,
-                    &ctx->ms_cur, &ctx->ms_nxt, &ctx->ms_lv, &ctx->ms_hubs, &ctx->ms_phubs})
+                    &ctx->ms_cur, &ctx->ms_nxt, &ctx->ms_lv, &ctx->ms_hubs, &ctx->ms_phubs,
+                    &ctx->ms_act})
     release(*b);
 }
 

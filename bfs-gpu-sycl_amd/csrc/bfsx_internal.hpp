@@ -93,7 +93,7 @@ struct bfsx_ctx {
   hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr;  // main kernel of a timed step
 
   // batched multi-source search state
-  bfsx::DevBuf ms_seen, ms_cur, ms_nxt, ms_lv, ms_hubs, ms_phubs;
+  bfsx::DevBuf ms_seen, ms_cur, ms_nxt, ms_lv, ms_hubs, ms_phubs, ms_act;
 
   // captured level loop (BFSX_RUN_GRAPH_LOOP), rebuilt when its key changes
   hipGraph_t loop_graph = nullptr;

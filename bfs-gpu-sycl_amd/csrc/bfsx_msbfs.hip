@@ -61,7 +61,35 @@ struct MsArgs {
   uint32_t level;  // level being written (the new bits' level)
   uint32_t *overflow;
   unsigned long long *q_edges_out;  // out-degree sum of the next active queue
+  // one bit per vertex: in some search's frontier (cur != 0); act_cur is
+  // null on the first level.  Bottom-up levels probe it (a few MiB, L2
+  // resident) before reading a neighbour's bitset, whose 32 bytes cost a
+  // whole line.
+  const uint32_t *act_cur;
+  uint32_t *act_next;
 };
+
+// Sets the act_next bits of the G = 64 / W consecutive vertices of a wave
+// (first one u0) whose group found new bits (m: ballot of nb != 0).
+template <int W>
+__device__ __forceinline__ void ms_mark_active(uint32_t *act_next, uint32_t u0,
+                                               unsigned long long m) {
+  constexpr uint32_t G = 64 / W;
+  if ((threadIdx.x & 63) != 0 || !m) return;
+  unsigned long long bits = 0;
+#pragma unroll
+  for (uint32_t g = 0; g < G; g++)
+    if ((m >> (g * W)) & ((W == 64) ? ~0ull : ((1ull << W) - 1))) bits |= 1ull << g;
+  // u0 is a multiple of G, so the G bits stay inside one or two words
+  const uint32_t w = u0 >> 5, sh = u0 & 31;
+  const unsigned long long x = bits << sh;
+  if ((uint32_t)x) atomicOr(&act_next[w], (uint32_t)x);
+  if ((uint32_t)(x >> 32)) atomicOr(&act_next[w + 1], (uint32_t)(x >> 32));
+}
+
+__device__ __forceinline__ bool ms_active(const uint32_t *act, uint32_t w) {
+  return !act || ((act[w >> 5] >> (w & 31)) & 1u);
+}
 
 // Adds deg(u) of each group leader that appended u, one atomic per wave.
 template <typename PtrT>
@@ -158,6 +186,7 @@ __global__ __launch_bounds__(256) void k_ms_finalize(MsArgs a, uint32_t n_groups
     const bool app = group_new && j == 0 && u < a.n_cols;
     ms_append(app, u, a.q_out, a.q_count_out);
     ms_track<PtrT>(col_ptr, a.n_rows, app, u, a.q_edges_out);
+    ms_mark_active<W>(a.act_next, wv * G, m);
   }
 }
 
@@ -199,7 +228,7 @@ __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_tot
 #pragma unroll
         for (int i = 0; i < 8; i++) {
           const uint32_t w = __shfl(ids[i / W], (int)(grp * W + (i % W)), 64);
-          if (w != 0xFFFFFFFFu) part |= a.cur[(size_t)w * W + j];
+          if (w != 0xFFFFFFFFu && ms_active(a.act_cur, w)) part |= a.cur[(size_t)w * W + j];
         }
         acc |= part;
         // stop once every unseen bit of the group's vertex has arrived
@@ -220,6 +249,7 @@ __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_tot
     const bool app = (m & gmask) != 0 && j == 0 && valid;
     ms_append(app, u, a.q_out, a.q_count_out);
     ms_track<PtrT>(col_ptr, a.n_rows, app, u, a.q_edges_out);
+    ms_mark_active<W>(a.act_next, wv * G, m);
   }
 }
 
@@ -258,7 +288,7 @@ __global__ __launch_bounds__(256) void k_ms_pull_hubs(MsArgs a, const uint32_t *
 #pragma unroll
       for (int i = 0; i < 8; i++) {
         const uint32_t w = __shfl(ids[i / W], (int)((lane / W) * W + (i % W)), 64);
-        if (w != 0xFFFFFFFFu) acc |= a.cur[(size_t)w * W + j];
+        if (w != 0xFFFFFFFFu && ms_active(a.act_cur, w)) acc |= a.cur[(size_t)w * W + j];
       }
       const bool full = (acc | s) == ~0u;
       if ((__ballot(!full) & gmask) == 0) break;
@@ -282,6 +312,7 @@ __global__ __launch_bounds__(256) void k_ms_pull_hubs(MsArgs a, const uint32_t *
     if (any) {
       a.q_out[atomicAdd(a.q_count_out, 1u)] = u;
       atomicAdd(a.q_edges_out, (unsigned long long)(col_ptr[u + 1] - col_ptr[u]));
+      atomicOr(&a.act_next[u >> 5], 1u << (u & 31));
     }
   }
 }
@@ -390,6 +421,7 @@ static MsResult ms_run_t(bfsx_ctx *ctx, const uint32_t *d_src, const uint16_t *d
   ensure(ctx->ms_nxt, sizeof(uint32_t) * n * W, s);
   ensure(ctx->ms_lv, (size_t)n * NB, s);
   ensure(ctx->ms_hubs, sizeof(uint32_t) * (n + 1), s);
+  ensure(ctx->ms_act, sizeof(uint32_t) * 2 * (n / 32 + 2), s);
   ensure(ctx->scratch, 64, s);
   const bool symmetric = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols;
   uint32_t *seen = ctx->ms_seen.as<uint32_t>();
@@ -429,8 +461,15 @@ static MsResult ms_run_t(bfsx_ctx *ctx, const uint32_t *d_src, const uint16_t *d
   uint32_t *hubs = ctx->ms_hubs.as<uint32_t>();
   uint32_t *n_hubs = ctx->scratch.as<uint32_t>() + 8;
   const uint32_t n_groups = (uint32_t)((n + (64 / W) - 1) / (64 / W));
+  const size_t act_words = n / 32 + 2;
+  uint32_t *act[2] = {ctx->ms_act.as<uint32_t>(), ctx->ms_act.as<uint32_t>() + act_words};
+  int ai = 0;  // act[ai] receives the next level's active vertices
   while (F > 0) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(ms_grid(act_words, 256, cap)), dim3(256), 0, s, act[ai],
+                       act_words, 0u);
     MsArgs a{};
+    a.act_cur = level == 0 ? nullptr : act[ai ^ 1];
+    a.act_next = act[ai];
     a.col_ptr = ctx->col_ptr.p;
     a.row_idx = ctx->row_idx.as<uint32_t>();
     a.n_rows = ctx->n_rows;
@@ -478,6 +517,7 @@ static MsResult ms_run_t(bfsx_ctx *ctx, const uint32_t *d_src, const uint16_t *d
     f_edges = (double)ctx->h_ctl->q_edges[qi ^ 1];
     std::swap(cur, nxt);
     qi ^= 1;
+    ai ^= 1;
     level++;
     r.n_levels++;
   }
