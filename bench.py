@@ -237,7 +237,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the search shards by roots or by destination slices, see DESIGN.md)",
+              "(the search would shard by roots or by destination slices)",
               file=sys.stderr)
         sys.exit(2)
 

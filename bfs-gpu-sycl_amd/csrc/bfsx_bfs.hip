@@ -1205,20 +1205,6 @@ __global__ void k_queue_bits(const uint32_t *q, uint32_t F, uint32_t *fbm) {
   }
 }
 
-// frontier bitmap of one level from the levels array (a wave owns 64 vertices)
-__global__ void k_levels_to_fbm(const int32_t *levels, uint32_t n_cols, int32_t level,
-                                uint32_t *fbm, uint32_t n_groups) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
-  for (uint32_t g = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); g < n_groups;
-       g += nwaves) {
-    const uint32_t u = g * 64 + lane;
-    const bool f = u < n_cols && levels[u] == level;
-    const unsigned long long m = __ballot(f);
-    if (lane == 0) fbm[2 * g] = (uint32_t)m;
-    if (lane == 32) fbm[2 * g + 1] = (uint32_t)(m >> 32);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // search set-up and per-level bookkeeping
@@ -1640,14 +1626,6 @@ void queue_to_fbm(bfsx_ctx *ctx, const uint32_t *q, uint32_t F, uint32_t *fbm) {
   BFSX_HIP(hipGetLastError());
 }
 
-void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level, uint32_t *fbm) {
-  const uint32_t n_groups = (ctx->n_cols + 63) / 64;
-  const unsigned grid = cap_grid(((uint64_t)n_groups * 64 + 255) / 256,
-                                 (unsigned)ctx->cu_count * 32);
-  hipLaunchKernelGGL(k_levels_to_fbm, dim3(grid), dim3(256), 0, ctx->stream, levels,
-                     ctx->n_cols, level, fbm, n_groups);
-  BFSX_HIP(hipGetLastError());
-}
 
 void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *par) {
   const uint32_t n_words = (uint32_t)bitmap_words(ctx->n_cols);

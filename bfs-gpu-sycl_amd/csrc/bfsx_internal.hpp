@@ -167,8 +167,6 @@ bool step_glb(bfsx_ctx *ctx, const StepArgs &a);
 void step_pull(bfsx_ctx *ctx, const StepArgs &a);
 void bitmap_to_queue(bfsx_ctx *ctx, const uint32_t *bm, uint32_t *q);
 void queue_to_fbm(bfsx_ctx *ctx, const uint32_t *q, uint32_t F, uint32_t *fbm);
-void levels_to_fbm(bfsx_ctx *ctx, const int32_t *levels, int32_t level,
-                   uint32_t *fbm);
 void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next);
 void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
                bool stamp, int kind);

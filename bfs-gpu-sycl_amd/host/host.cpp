@@ -158,7 +158,7 @@ int main(int argc, char *argv[]) {
     parser.addArgument("dataset", "default");
     parser.addArgument("root", "1");
     parser.addArgument("output", "data.json");
-    parser.addArgument("strategy", strategy_key(bfsx::gpurun_config().strategy));
+    parser.addArgument("strategy", strategy_key(bfsx::run_config().strategy));
     parser.addArgument("partitions", std::to_string(NUM_GPU));
     parser.addArgument("generate", "");
     parser.addArgument("write_dataset", "");
@@ -172,7 +172,7 @@ int main(int argc, char *argv[]) {
     const std::string output_json_name = parser.getArgument("output");
     const int strategy = strategy_from(parser.getArgument("strategy"));
     const uint32_t P = (uint32_t)std::stoul(parser.getArgument("partitions"));
-    bfsx::GPURunConfig &cfg = bfsx::gpurun_config();
+    bfsx::GPURunConfig &cfg = bfsx::run_config();
     cfg.strategy = strategy;
     cfg.device = std::stoi(parser.getArgument("device"));
     parser.printArguments();

@@ -23,7 +23,7 @@
 //     copied back), the reference's boundary (:679, :728).
 //   * The load-balancing strategy is the reference's compile-time choice
 //     (USE_GLOBAL_LOAD_BALANCE / USE_STRIDED_LOCAL_LOAD_BALANCE) unless set
-//     at run time with bfsx::gpurun_config().
+//     at run time with bfsx::run_config().
 //   * Errors: the reference prints and calls std::terminate on a SYCL
 //     exception (:794-806); this shim throws std::runtime_error with the
 //     engine's message instead (the signature is noexcept(false)).
@@ -60,7 +60,7 @@ struct GPURunConfig {
   uint64_t last_edges_count = 0;    // edgesCount of the last run (device-side)
 };
 
-inline GPURunConfig &gpurun_config() {
+inline GPURunConfig &run_config() {
   static GPURunConfig c;
   return c;
 }
@@ -84,7 +84,7 @@ void GPURun(uint32_t vertexCount, vectorT &IndexHost, vectorT &OffsetHost,
   (void)h_visit_mask;
   (void)h_visit;
   (void)h_visit_offsets;
-  GPURunConfig &cfg = gpurun_config();
+  GPURunConfig &cfg = run_config();
   if (num_runs < 1 || (int)h_distance.size() < num_runs)
     throw std::runtime_error("GPURun: h_distance must hold num_runs rows");
   const uint32_t n_cols = (uint32_t)h_distance.back().size();
