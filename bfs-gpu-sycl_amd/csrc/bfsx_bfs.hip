@@ -25,6 +25,7 @@
 //         frontier bitmap and stops at the first hit; a wave owns 64
 //         consecutive vertices, so bitmap words are written without atomics.
 #include "bfsx_internal.hpp"
+#include "bfsx_mlp.hpp"
 
 namespace bfsx {
 
@@ -1247,27 +1248,36 @@ __global__ void k_level_begin(LevelCtl *ctl) {
 // kind >= 0: the step kind that ran; kind < 0: the one in ctl->choice.  A
 // level with an empty frontier (after the end, in a captured loop) changes
 // nothing but the done flag.
-__global__ void k_level_end(LevelCtl *ctl, uint32_t consumed, uint32_t produced, int stamp,
-                            int kind) {
+__global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consumed,
+                                                  uint32_t produced, int stamp, int kind,
+                                                  const float *Wg, uint64_t n_cols, uint64_t nnz,
+                                                  uint64_t n_iso, uint32_t allow) {
+  __shared__ int s_live;
   if (threadIdx.x == 0) {
+    s_live = 0;
     if (ctl->q_count[consumed] == 0) {
       ctl->done = 1;
-      return;
+    } else {
+      if (stamp) ctl->t_end = (unsigned long long)wall_clock64();
+      const uint32_t k = kind >= 0 ? (uint32_t)kind : ctl->choice;
+      if (k < 4) ctl->kind_count[k]++;
+      const uint32_t n = ctl->q_count[produced];
+      ctl->discovered = n;
+      ctl->visited_vertices += n;
+      ctl->visited_edges += ctl->q_edges[produced];
+      ctl->q_count[consumed] = 0;
+      ctl->q_edges[consumed] = 0;
+      ctl->glb_cursor = 0;
+      ctl->conv = 0;
+      ctl->level += 1;
+      ctl->done = n == 0;
+      s_live = n != 0;
     }
-    if (stamp) ctl->t_end = (unsigned long long)wall_clock64();
-    const uint32_t k = kind >= 0 ? (uint32_t)kind : ctl->choice;
-    if (k < 4) ctl->kind_count[k]++;
-    const uint32_t n = ctl->q_count[produced];
-    ctl->discovered = n;
-    ctl->visited_vertices += n;
-    ctl->visited_edges += ctl->q_edges[produced];
-    ctl->q_count[consumed] = 0;
-    ctl->q_edges[consumed] = 0;
-    ctl->glb_cursor = 0;
-    ctl->conv = 0;
-    ctl->level += 1;
-    ctl->done = n == 0;
   }
+  if (!Wg) return;  // no scheduler: uniform over the work-group
+  __syncthreads();
+  // the learned scheduler picks the next level's step in the same launch
+  if (s_live) decide_wave(ctl, Wg, produced, n_cols, nnz, n_iso, allow, 0);
 }
 
 // sum of out-degree over reached vertices < n_rows, reached count, max level
@@ -1650,9 +1660,15 @@ void level_begin(bfsx_ctx *ctx) {
   BFSX_HIP(hipGetLastError());
 }
 
-void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, int kind) {
+void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, int kind,
+               uint32_t decide_allow) {
+  const uint64_t n_iso = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols
+                             ? ctx->n_isolated
+                             : 0;
   hipLaunchKernelGGL(k_level_end, dim3(1), dim3(64), 0, ctx->stream, ctx->ctl.as<LevelCtl>(),
-                     consumed, produced, stamp ? 1 : 0, kind);
+                     consumed, produced, stamp ? 1 : 0, kind,
+                     decide_allow ? ctx->sched_w.as<float>() : (const float *)nullptr,
+                     (uint64_t)ctx->n_cols, (uint64_t)ctx->nnz, n_iso, decide_allow);
   BFSX_HIP(hipGetLastError());
 }
 

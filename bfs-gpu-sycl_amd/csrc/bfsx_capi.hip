@@ -152,6 +152,7 @@ struct Walker {
   bool count = false;      // count adjacency entries read by bottom-up steps
   uint32_t *par = nullptr;
   int last = BFSX_LLB;     // step kind of the last launch
+  uint32_t decide_allow = 0;  // learned: kinds the next decision may pick
 
   // Launches the kernels of one level with step kind `choice`, then the
   // level bookkeeping kernel.  The caller reads the control block back and
@@ -219,7 +220,7 @@ struct Walker {
       compact(ctx, a, a.mark, td_fbm_next);
       new_fbm = td_fbm_next != nullptr;
     }
-    level_end(ctx, cur, cur ^ 1, stamp, choice);
+    level_end(ctx, cur, cur ^ 1, stamp, choice, decide_allow);
     last = choice;
     return new_fbm;
   }
@@ -294,8 +295,8 @@ static hipGraphExec_t loop_graph(bfsx_ctx *ctx, int fixed, bool learned) {
       const uint32_t cur = k & 1u;
       if (learned) level_begin(ctx);
       dev_level(ctx, cur, fixed, true, mme);
-      level_end(ctx, cur, cur ^ 1, learned, fixed);
-      if (learned) decide(ctx, cur ^ 1, false, (1u << BFSX_LLB) | (1u << BFSX_SLB));
+      level_end(ctx, cur, cur ^ 1, learned, fixed,
+                learned ? (1u << BFSX_LLB) | (1u << BFSX_SLB) : 0u);
     }
   } catch (...) {
     hipGraph_t g = nullptr;
@@ -367,6 +368,7 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   BFSX_HIP(hipEventRecord(ctx->ev0, s));
   uint64_t f_edges = 0, visited_v = 1, visited_e = 0, examined = 0;
   if (learned) decide(ctx, 0, true, allow_mask(w));
+  w.decide_allow = learned ? allow_mask(w) : 0u;  // later levels decide in level_end
   if (trace || learned) {
     read_ctl(ctx);
     f_edges = ctx->h_ctl->q_edges[0];
@@ -380,7 +382,6 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
     const bool new_fbm = w.launch(choice);
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
-    if (learned) decide(ctx, cur ^ 1, false, allow_mask(w));
     read_ctl(ctx);
     w.advance(new_fbm);
     if (timed) {
@@ -468,6 +469,7 @@ static uint32_t record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, u
     Walker w = start_search(ctx, roots[ri], symmetric, true, nullptr);
     const uint32_t allow = allow_mask(w);
     decide(ctx, 0, true, allow);
+    w.decide_allow = allow;
     read_ctl(ctx);
     while (w.F > 0 && rows < max_rows) {
       if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
@@ -502,14 +504,12 @@ static uint32_t record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, u
           if ((allow >> kind) & 1u) opts[n++] = kind;
         take = opts[std::min(n - 1, (int)(next_u01() * n))];
       }
-      // run the chosen kind for real; its device time feeds the next inputs
+      // run the chosen kind for real; the next row's inputs come from its state
       snapshot(ctx, false);
       w = saved;
-      const uint32_t cur = w.cur;
       BFSX_HIP(hipMemcpyAsync(&ctx->ctl.as<LevelCtl>()->choice, &take, sizeof(uint32_t),
                               hipMemcpyHostToDevice, ctx->stream));
-      const bool new_fbm = w.launch(take);
-      decide(ctx, cur ^ 1, false, allow);
+      const bool new_fbm = w.launch(take);  // its level end decides the next level
       read_ctl(ctx);
       w.advance(new_fbm);
       rows++;

@@ -168,8 +168,10 @@ void step_pull(bfsx_ctx *ctx, const StepArgs &a);
 void bitmap_to_queue(bfsx_ctx *ctx, const uint32_t *bm, uint32_t *q);
 void queue_to_fbm(bfsx_ctx *ctx, const uint32_t *q, uint32_t F, uint32_t *fbm);
 void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next);
+// Level bookkeeping; with decide_allow != 0 the learned scheduler also picks
+// the next level's step kind (among the allowed ones) in the same launch.
 void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
-               bool stamp, int kind);
+               bool stamp, int kind, uint32_t decide_allow = 0);
 // mark_min_E: top-down levels whose frontier has at least this many edges
 // mark and compact (needs track: the edge counts are kept in the control block)
 void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,

@@ -1,0 +1,94 @@
+// Device side of the learned per-level scheduler: the network's layout and
+// its inference, shared by the stand-alone decision kernel and the
+// level-end kernel (which decides the next level in the same launch).
+#pragma once
+#include "bfsx_internal.hpp"
+
+namespace bfsx {
+
+constexpr int IN = 8, H1 = 32, H2 = 32, OUT = 4;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + H1 * IN, OFF_W2 = OFF_B1 + H1,
+              OFF_B2 = OFF_W2 + H2 * H1, OFF_W3 = OFF_B2 + H2, OFF_B3 = OFF_W3 + OUT * H2,
+              NPARAM = OFF_B3 + OUT;
+static_assert(NPARAM == BFSX_SCHED_PARAMS, "parameter count");
+static_assert(NPARAM % 4 == 0, "weights load as float4");
+
+__device__ __forceinline__ float flog2p1(double v) { return (float)log2(1.0 + (v > 0 ? v : 0.0)); }
+// log2((1 + a) / (1 + b)) for a, b >= 0
+__device__ __forceinline__ float flog2ratio(double a, double b) {
+  return (float)(log2(1.0 + (a > 0 ? a : 0.0)) - log2(1.0 + (b > 0 ? b : 0.0)));
+}
+
+// Inference by one work-group of 64 threads (every thread calls it): the
+// inputs of frontier buffer `idx` from the control block, the network from
+// LDS-staged weights (float4 loads), ctl->choice = argmin over the allowed
+// kinds (fixed LLB when an output is NaN), inputs and outputs recorded.
+//   x0 = log2((1+F)/(1+n))/32          x1 = log2((1+EF)/(1+nnz))/32
+//   x2 = log2((1+UV)/(1+n))/32         x3 = log2((1+UE)/(1+nnz))/32
+//   x4 = log2(1+EF/F)/16               x5 = log2(1+nnz/n)/16
+//   x6 = log2(1+n)/32                  x7 = 1 if the last step was bottom-up
+// F/EF: frontier vertices/edges, UV/UE: unvisited vertices/edges (isolated
+// vertices count as visited).
+__device__ __forceinline__ void decide_wave(LevelCtl *ctl, const float *Wg, uint32_t idx,
+                                            uint64_t n_cols, uint64_t nnz, uint64_t n_iso,
+                                            uint32_t allow_mask, int first) {
+  __shared__ float4 W4[NPARAM / 4];
+  __shared__ float x[IN], h1[H1], h2[H2], y[OUT];
+  const float *W = reinterpret_cast<const float *>(W4);
+  const int l = threadIdx.x;
+  const float4 *G4 = reinterpret_cast<const float4 *>(Wg);
+  for (int i = l; i < NPARAM / 4; i += 64) W4[i] = G4[i];
+  if (l == 0) {
+    const double F = ctl->q_count[idx];
+    const double EF = (double)ctl->q_edges[idx];
+    const double vv = (double)ctl->visited_vertices + (double)n_iso;
+    const double UV = vv < (double)n_cols ? (double)n_cols - vv : 0.0;
+    const double ve = (double)ctl->visited_edges;
+    const double UE = ve < (double)nnz ? (double)nnz - ve : 0.0;
+    const double n = (double)n_cols, m = (double)nnz;
+    x[0] = flog2ratio(F, n) / 32.f;
+    x[1] = flog2ratio(EF, m) / 32.f;
+    x[2] = flog2ratio(UV, n) / 32.f;
+    x[3] = flog2ratio(UE, m) / 32.f;
+    x[4] = flog2p1(F > 0 ? EF / F : 0.0) / 16.f;
+    x[5] = flog2p1(n > 0 ? m / n : 0.0) / 16.f;
+    x[6] = flog2p1(n) / 32.f;
+    x[7] = (!first && ctl->choice == BFSX_PULL) ? 1.f : 0.f;
+  }
+  __syncthreads();
+  if (l < H1) {
+    float s = W[OFF_B1 + l];
+    for (int k = 0; k < IN; k++) s = fmaf(W[OFF_W1 + l * IN + k], x[k], s);
+    h1[l] = s > 0.f ? s : 0.f;
+  }
+  __syncthreads();
+  if (l < H2) {
+    float s = W[OFF_B2 + l];
+    for (int k = 0; k < H1; k++) s = fmaf(W[OFF_W2 + l * H1 + k], h1[k], s);
+    h2[l] = s > 0.f ? s : 0.f;
+  }
+  __syncthreads();
+  if (l < OUT) {
+    float s = W[OFF_B3 + l];
+    for (int k = 0; k < H2; k++) s = fmaf(W[OFF_W3 + l * H2 + k], h2[k], s);
+    y[l] = s;
+  }
+  __syncthreads();
+  if (l == 0) {
+    uint32_t best = 0;
+    float bv = INFINITY;
+    bool ok = true;
+    for (int j = 0; j < OUT; j++) {
+      ctl->sched_out[j] = y[j];
+      if (!(y[j] == y[j])) ok = false;  // NaN
+      if (((allow_mask >> j) & 1u) && y[j] < bv) {
+        bv = y[j];
+        best = (uint32_t)j;
+      }
+    }
+    for (int k = 0; k < IN; k++) ctl->sched_in[k] = x[k];
+    ctl->choice = ok && bv < INFINITY ? best : (uint32_t)BFSX_LLB;
+  }
+}
+
+}  // namespace bfsx

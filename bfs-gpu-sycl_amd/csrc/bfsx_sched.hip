@@ -1,17 +1,9 @@
 // Learned per-level scheduler: an FP32 MLP 8 -> 32 -> 32 -> 4 (ReLU) that,
 // before each level, predicts log2 of the device time of each step kind
-// (LLB, GLB, SLB, PULL) from frontier statistics; the level runs the step
-// with the smallest prediction.  It only chooses the schedule: every step
-// kind computes the same exact levels.
-//
-// Inputs (all computed on the device from the level control block):
-//   x0 = log2(1 + frontier vertices) / 32
-//   x1 = log2(1 + frontier edges) / 32
-//   x2 = log2(1 + unvisited vertices) / 32
-//   x3 = log2(1 + unvisited edges) / 32
-//   x4 = level index / 64
-//   x5, x6 = previous step kind as two bits (kind & 1, kind >> 1)
-//   x7 = log2(1 + previous step time in microseconds) / 16
+// (LLB, GLB, SLB, PULL) relative to LLB from frontier statistics; the level
+// runs the step with the smallest prediction.  It only chooses the
+// schedule: every step kind computes the same exact levels.  The inputs and
+// the inference are in bfsx_mlp.hpp.
 // Parameter layout (1476 floats): W1[32][8], b1[32], W2[32][32], b2[32],
 // W3[4][32], b3[4], row-major.
 //
@@ -19,88 +11,17 @@
 // kernel stores activations and deltas column-major, a gradient kernel
 // reduces each parameter row over all samples in a fixed order (one block
 // per row, deterministic), and an Adam kernel updates the parameters.
-#include "bfsx_internal.hpp"
+#include "bfsx_mlp.hpp"
 
 namespace bfsx {
 
-constexpr int IN = 8, H1 = 32, H2 = 32, OUT = 4;
-constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + H1 * IN, OFF_W2 = OFF_B1 + H1,
-              OFF_B2 = OFF_W2 + H2 * H1, OFF_W3 = OFF_B2 + H2, OFF_B3 = OFF_W3 + OUT * H2,
-              NPARAM = OFF_B3 + OUT;
-static_assert(NPARAM == BFSX_SCHED_PARAMS, "parameter count");
-
-__device__ __forceinline__ float flog2p1(double v) { return (float)log2(1.0 + (v > 0 ? v : 0.0)); }
-// log2((1 + a) / (1 + b)) for a, b >= 0
-__device__ __forceinline__ float flog2ratio(double a, double b) {
-  return (float)(log2(1.0 + (a > 0 ? a : 0.0)) - log2(1.0 + (b > 0 ? b : 0.0)));
-}
-
 // ---------------------------------------------------------------------------
-// inference inside the level loop (one wave)
+// inference before the first level (later levels decide in k_level_end)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_decide(LevelCtl *ctl, const float *Wg, uint32_t idx,
                                                uint64_t n_cols, uint64_t nnz, uint64_t n_iso,
                                                uint32_t allow_mask, int first) {
-  __shared__ float W[NPARAM];
-  __shared__ float x[IN], h1[H1], h2[H2], y[OUT];
-  const int l = threadIdx.x;
-  // all weight loads in flight at once, then the layers read LDS
-  for (int i = l; i < NPARAM; i += 64) W[i] = Wg[i];
-  if (l == 0) {
-    const double F = ctl->q_count[idx];
-    const double EF = (double)ctl->q_edges[idx];
-    const double vv = (double)ctl->visited_vertices + (double)n_iso;
-    const double UV = vv < (double)n_cols ? (double)n_cols - vv : 0.0;
-    const double ve = (double)ctl->visited_edges;
-    const double UE = ve < (double)nnz ? (double)nnz - ve : 0.0;
-    const double n = (double)n_cols, m = (double)nnz;
-    // size-free inputs: fractions of the graph on a log scale, mean degrees,
-    // plus the graph size for fixed per-launch costs
-    x[0] = flog2ratio(F, n) / 32.f;
-    x[1] = flog2ratio(EF, m) / 32.f;
-    x[2] = flog2ratio(UV, n) / 32.f;
-    x[3] = flog2ratio(UE, m) / 32.f;
-    x[4] = flog2p1(F > 0 ? EF / F : 0.0) / 16.f;
-    x[5] = flog2p1(n > 0 ? m / n : 0.0) / 16.f;
-    x[6] = flog2p1(n) / 32.f;
-    // the frontier exists only as a bitmap after a bottom-up step
-    x[7] = (!first && ctl->choice == BFSX_PULL) ? 1.f : 0.f;
-  }
-  __syncthreads();
-  if (l < H1) {
-    float s = W[OFF_B1 + l];
-    for (int k = 0; k < IN; k++) s = fmaf(W[OFF_W1 + l * IN + k], x[k], s);
-    h1[l] = s > 0.f ? s : 0.f;
-  }
-  __syncthreads();
-  if (l < H2) {
-    float s = W[OFF_B2 + l];
-    for (int k = 0; k < H1; k++) s = fmaf(W[OFF_W2 + l * H1 + k], h1[k], s);
-    h2[l] = s > 0.f ? s : 0.f;
-  }
-  __syncthreads();
-  if (l < OUT) {
-    float s = W[OFF_B3 + l];
-    for (int k = 0; k < H2; k++) s = fmaf(W[OFF_W3 + l * H2 + k], h2[k], s);
-    y[l] = s;
-  }
-  __syncthreads();
-  if (l == 0) {
-    uint32_t best = 0;
-    float bv = INFINITY;
-    bool ok = true;
-    for (int j = 0; j < OUT; j++) {
-      ctl->sched_out[j] = y[j];
-      if (!(y[j] == y[j])) ok = false;  // NaN
-      if (((allow_mask >> j) & 1u) && y[j] < bv) {
-        bv = y[j];
-        best = (uint32_t)j;
-      }
-    }
-    for (int k = 0; k < IN; k++) ctl->sched_in[k] = x[k];
-    // fixed LLB is the fallback when the network gives no usable answer
-    ctl->choice = ok && bv < INFINITY ? best : (uint32_t)BFSX_LLB;
-  }
+  decide_wave(ctl, Wg, idx, n_cols, nnz, n_iso, allow_mask, first);
 }
 
 void decide(bfsx_ctx *ctx, uint32_t idx, bool first, uint32_t allow_mask) {
