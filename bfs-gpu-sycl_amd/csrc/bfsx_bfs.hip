@@ -1251,7 +1251,7 @@ __global__ void k_level_begin(LevelCtl *ctl) {
 __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consumed,
                                                   uint32_t produced, int stamp, int kind,
                                                   const float *Wg, uint64_t n_cols, uint64_t nnz,
-                                                  uint64_t n_iso, uint32_t allow) {
+                                                  uint64_t n_iso, uint32_t allow, int reuse) {
   __shared__ int s_live;
   if (threadIdx.x == 0) {
     s_live = 0;
@@ -1272,11 +1272,31 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
       ctl->level += 1;
       ctl->done = n == 0;
       s_live = n != 0;
+      if (s_live && Wg) {
+        // keep the last decision while the frontier stays within 25 % of
+        // the one it was computed for, after the same kind of step
+        const unsigned long long ef = ctl->q_edges[produced];
+        const uint32_t pull = k == BFSX_PULL;
+        const uint32_t f0 = ctl->dec_F;
+        const unsigned long long e0 = ctl->dec_EF;
+        const bool close = (uint64_t)n * 5 >= (uint64_t)f0 * 4 && (uint64_t)n * 4 <= (uint64_t)f0 * 5 &&
+                           ef * 5 >= e0 * 4 && ef * 4 <= e0 * 5;
+        if (reuse && ctl->dec_valid && ctl->dec_pull == pull && close) {
+          s_live = 0;
+        } else {
+          ctl->dec_F = n;
+          ctl->dec_EF = ef;
+          ctl->dec_pull = pull;
+          ctl->dec_valid = 1;
+        }
+      }
     }
   }
   if (!Wg) return;  // no scheduler: uniform over the work-group
   __syncthreads();
   // the learned scheduler picks the next level's step in the same launch
+  // (the weights are staged only when the network runs: a level that keeps
+  // the last decision costs what a level without the scheduler does)
   if (s_live) decide_wave(ctl, Wg, produced, n_cols, nnz, n_iso, allow, 0);
 }
 
@@ -1661,14 +1681,15 @@ void level_begin(bfsx_ctx *ctx) {
 }
 
 void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, int kind,
-               uint32_t decide_allow) {
+               uint32_t decide_allow, bool reuse) {
   const uint64_t n_iso = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols
                              ? ctx->n_isolated
                              : 0;
   hipLaunchKernelGGL(k_level_end, dim3(1), dim3(64), 0, ctx->stream, ctx->ctl.as<LevelCtl>(),
                      consumed, produced, stamp ? 1 : 0, kind,
                      decide_allow ? ctx->sched_w.as<float>() : (const float *)nullptr,
-                     (uint64_t)ctx->n_cols, (uint64_t)ctx->nnz, n_iso, decide_allow);
+                     (uint64_t)ctx->n_cols, (uint64_t)ctx->nnz, n_iso, decide_allow,
+                     reuse ? 1 : 0);
   BFSX_HIP(hipGetLastError());
 }
 

@@ -91,7 +91,7 @@ static bfsx_params default_params() {
   p.hub_degree = 0;
   p.pull_wg = 512;
   p.graph_levels = 8;
-  p.mark_ratio_q10 = 64;
+  p.mark_ratio_q10 = 16;
   p.bucket_ratio_q10 = 512;
   return p;
 }
@@ -153,6 +153,7 @@ struct Walker {
   uint32_t *par = nullptr;
   int last = BFSX_LLB;     // step kind of the last launch
   uint32_t decide_allow = 0;  // learned: kinds the next decision may pick
+  bool decide_reuse = false;  // keep the last decision on a similar frontier
 
   // Launches the kernels of one level with step kind `choice`, then the
   // level bookkeeping kernel.  The caller reads the control block back and
@@ -220,7 +221,7 @@ struct Walker {
       compact(ctx, a, a.mark, td_fbm_next);
       new_fbm = td_fbm_next != nullptr;
     }
-    level_end(ctx, cur, cur ^ 1, stamp, choice, decide_allow);
+    level_end(ctx, cur, cur ^ 1, stamp, choice, decide_allow, decide_reuse);
     last = choice;
     return new_fbm;
   }
@@ -296,7 +297,7 @@ static hipGraphExec_t loop_graph(bfsx_ctx *ctx, int fixed, bool learned) {
       if (learned) level_begin(ctx);
       dev_level(ctx, cur, fixed, true, mme);
       level_end(ctx, cur, cur ^ 1, learned, fixed,
-                learned ? (1u << BFSX_LLB) | (1u << BFSX_SLB) : 0u);
+                learned ? (1u << BFSX_LLB) | (1u << BFSX_SLB) : 0u, true);
     }
   } catch (...) {
     hipGraph_t g = nullptr;
@@ -369,6 +370,7 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   uint64_t f_edges = 0, visited_v = 1, visited_e = 0, examined = 0;
   if (learned) decide(ctx, 0, true, allow_mask(w));
   w.decide_allow = learned ? allow_mask(w) : 0u;  // later levels decide in level_end
+  w.decide_reuse = true;
   if (trace || learned) {
     read_ctl(ctx);
     f_edges = ctx->h_ctl->q_edges[0];
@@ -487,6 +489,9 @@ static uint32_t record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, u
         for (uint32_t rep = 0; rep < repeats; rep++) {
           snapshot(ctx, false);
           w = saved;
+          // from an idle stream, as in a search, so that the time includes
+          // the gaps of issuing the step's launches one after the other
+          BFSX_HIP(hipStreamSynchronize(ctx->stream));
           w.launch(kind);
           read_ctl(ctx);
           t = std::min(t, step_us(ctx));

@@ -34,6 +34,10 @@ struct LevelCtl {
   uint32_t kind_count[4];          // levels expanded by each step kind
   uint32_t conv;                   // slot cursor of a bitmap-to-queue conversion
   unsigned long long examined;     // adjacency entries read by bottom-up steps (running)
+  // frontier of the last network evaluation (a level whose frontier is
+  // within 25 % of it, after the same kind of step, keeps that decision)
+  unsigned long long dec_EF;
+  uint32_t dec_F, dec_pull, dec_valid;
 };
 
 // Bitmaps hold whole 1024-vertex chunks (32 words) so that a wave can own a
@@ -169,9 +173,11 @@ void bitmap_to_queue(bfsx_ctx *ctx, const uint32_t *bm, uint32_t *q);
 void queue_to_fbm(bfsx_ctx *ctx, const uint32_t *q, uint32_t F, uint32_t *fbm);
 void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next);
 // Level bookkeeping; with decide_allow != 0 the learned scheduler also picks
-// the next level's step kind (among the allowed ones) in the same launch.
+// the next level's step kind (among the allowed ones) in the same launch,
+// and with reuse it keeps the last decision while the frontier stays close
+// to the one that decision was computed for.
 void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
-               bool stamp, int kind, uint32_t decide_allow = 0);
+               bool stamp, int kind, uint32_t decide_allow = 0, bool reuse = false);
 // mark_min_E: top-down levels whose frontier has at least this many edges
 // mark and compact (needs track: the edge counts are kept in the control block)
 void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,

@@ -174,6 +174,19 @@ def test_record_rows(engine):
     want = mlp_ref.features(1, deg[r0], info["n_cols"] - info["n_isolated"] - 1,
                             info["nnz"] - deg[r0], info["n_cols"], info["nnz"], False)
     np.testing.assert_allclose(X[0], want, rtol=1e-6, atol=1e-6)
+    # every row of the first root: frontier and unvisited totals from the
+    # oracle's levels (the last input, the kind of the previous step, follows
+    # the recorded path and is only checked to be 0 or 1)
+    lev = cpu.bfs_queue(cp, ri, r0)[0]
+    n, nnz, n_iso = info["n_cols"], info["nnz"], info["n_isolated"]
+    for L in range(int(lev.max()) + 1):
+        at = lev == L
+        seen = (lev >= 0) & (lev <= L)
+        x7 = float(X[L, 7])
+        assert x7 in (0.0, 1.0) and (L > 0 or x7 == 0.0)
+        want = mlp_ref.features(int(at.sum()), int(deg[at].sum()), n - n_iso - int(seen.sum()),
+                                nnz - int(deg[seen].sum()), n, nnz, x7 == 1.0)
+        np.testing.assert_allclose(X[L], want, rtol=1e-6, atol=1e-6, err_msg=f"level {L}")
 
 
 @pytest.mark.gpu
