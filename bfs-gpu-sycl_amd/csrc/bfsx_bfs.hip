@@ -18,12 +18,15 @@
 //   SLB   the same kernel with the strided entry mapping of
 //         run_bfs_gpu.hpp:416-417 (group + lane * n_groups).
 //   GLB   device-wide scan of frontier degrees, then a persistent grid pulls
-//         fixed-size edge chunks from an atomic work queue; each chunk finds
+//         edge chunks (smaller ones for a frontier with few edges) from an
+//         atomic work queue; each chunk finds
 //         its frontier entries from a precomputed chunk->entry table and
 //         stages them in LDS (no global binary search per edge).
 //   PULL  bottom-up: every unvisited vertex looks for a neighbour in the
 //         frontier bitmap and stops at the first hit; a wave owns 64
 //         consecutive vertices, so bitmap words are written without atomics.
+#include <algorithm>
+
 #include "bfsx_internal.hpp"
 #include "bfsx_mlp.hpp"
 
@@ -329,6 +332,7 @@ __global__ __launch_bounds__(WG) void k_td_dev(KArgs a, LevelCtl *ctl, DevStep d
 // GLB: tile scan -> tile-offset scan -> global offsets + chunk table -> expand
 // ---------------------------------------------------------------------------
 constexpr int GLB_TILE = 2048;  // frontier entries per scan tile (256 x 8)
+constexpr uint32_t GLB_MIN_CHUNK = 256;  // smallest work-queue item (edges)
 
 template <typename PtrT>
 __global__ __launch_bounds__(256) void k_glb_tiles(const PtrT *col_ptr, uint32_t n_rows,
@@ -1480,7 +1484,18 @@ template <typename PtrT>
 static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   hipStream_t st = ctx->stream;
   const uint32_t F = s.F;
-  const uint32_t chunk = ctx->params.glb_chunk;
+  const unsigned grid = (unsigned)ctx->cu_count * ctx->params.glb_blocks_per_cu;
+  const uint32_t n_slices = (uint32_t)(((uint64_t)ctx->n_cols + (1u << BKT_SHIFT) - 1) >> BKT_SHIFT);
+  const bool bucketed = s.bucket_ok && !s.parents && n_slices <= BKT_MAX;
+  // Edges per work-queue item: glb_chunk, or, when the frontier's edges
+  // (exact, kept on the device) would not give every expand work-group an
+  // item, as few as GLB_MIN_CHUNK so that they spread over the grid (a
+  // frontier of two hubs otherwise runs on a handful of work-groups).
+  uint32_t chunk = ctx->params.glb_chunk;
+  if (!bucketed && s.EF > 0 && s.EF < (unsigned long long)chunk * grid) {
+    const unsigned long long want = (s.EF + grid - 1) / grid;
+    chunk = (uint32_t)std::max<unsigned long long>(GLB_MIN_CHUNK, (want + 255) & ~255ull);
+  }
   const uint32_t n_tiles = (F + GLB_TILE - 1) / GLB_TILE;
   PtrT *g_loc = ctx->glb_scan.as<PtrT>();             // F entries, then F+1 global
   PtrT *g_scan = g_loc + ((uint64_t)ctx->n_cols + 1);
@@ -1495,9 +1510,7 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   hipLaunchKernelGGL(k_glb_offsets<PtrT>, dim3(cap_grid((F + 255) / 256, 65535)), dim3(256), 0,
                      st, g_loc, tile, g_end, F, g_scan, chunk, ctx->glb_chunk_lo.as<uint32_t>());
   const KArgs a = kargs(s, 0);
-  const unsigned grid = (unsigned)ctx->cu_count * ctx->params.glb_blocks_per_cu;
-  const uint32_t n_slices = (uint32_t)(((uint64_t)ctx->n_cols + (1u << BKT_SHIFT) - 1) >> BKT_SHIFT);
-  if (s.bucket_ok && !s.parents && n_slices <= BKT_MAX) {
+  if (bucketed) {
     // fewer, larger groups than the GLB expand grid: longer runs per
     // (group, slice) in the segments that pass 2 writes
     glb_bucketed<PtrT>(ctx, s, a, g_scan, g_start, chunk, (unsigned)ctx->cu_count * 2, n_slices);
@@ -1533,8 +1546,7 @@ bool step_glb(bfsx_ctx *ctx, const StepArgs &s) {
   ensure(ctx->glb_scan, pw * 2 * ((uint64_t)ctx->n_cols + 1), ctx->stream);
   ensure(ctx->glb_start, pw * ((uint64_t)ctx->n_cols + 1), ctx->stream);
   ensure(ctx->glb_tile, pw * ((uint64_t)ctx->n_cols / GLB_TILE + 4), ctx->stream);
-  ensure(ctx->glb_chunk_lo, sizeof(uint32_t) * (ctx->nnz / ctx->params.glb_chunk + 2),
-         ctx->stream);
+  ensure(ctx->glb_chunk_lo, sizeof(uint32_t) * (ctx->nnz / GLB_MIN_CHUNK + 2), ctx->stream);
   const bool bucketed =
       ctx->ptr_bytes == 8 ? step_glb_t<uint64_t>(ctx, s) : step_glb_t<uint32_t>(ctx, s);
   BFSX_HIP(hipGetLastError());
