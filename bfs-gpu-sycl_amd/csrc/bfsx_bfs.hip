@@ -490,7 +490,8 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
 //           each slice's segment, and a scan of the slice totals the
 //           segments' bases;
 //   pass 2  the groups walk the same chunks again (static assignment) and
-//           append each target to its slice's segment (LDS cursors);
+//           append each target to its slice's segment (LDS cursors), as
+//           its 16-bit offset inside the slice;
 //   apply   a work-group per slice holds the slice's visited words in LDS,
 //           ORs the targets into an LDS found-bitmap, and emits the new
 //           vertices: levels, visited and next-frontier words, the queue in
@@ -541,7 +542,8 @@ __device__ __forceinline__ void window_edges(const KArgs &a, const PtrT *s_scan,
 }
 
 // PASS 1: per-slice counts of this group's edges -> cnt[group][slice].
-// PASS 2: slots from pos[group][slice] + base[slice]; targets -> bkt.
+// PASS 2: slots from pos[group][slice] + base[slice]; targets -> bkt (the
+// low 16 bits: the slice is implied by the segment).
 // Chunks are assigned statically (group g takes c = g, g + grid, ...), so
 // both passes see the same edges in every group.
 template <typename PtrT, int WG, int PASS>
@@ -550,7 +552,7 @@ __global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
                                                    const uint32_t *chunk_lo, LevelCtl *ctl,
                                                    uint32_t chunk, uint32_t n_slices,
                                                    uint32_t *cnt, const uint64_t *base,
-                                                   uint32_t *bkt, uint64_t cap) {
+                                                   uint16_t *bkt, uint64_t cap) {
   __shared__ PtrT s_scan[WG];
   __shared__ PtrT s_start[WG];
   __shared__ PtrT s_win_lo, s_win_hi;
@@ -622,7 +624,7 @@ __global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
               const uint32_t hs = __shfl(slot, (int)h, 64);
               if (v) {
                 const uint64_t p = s_base[sl] + hs + (lane - h);
-                if (p < cap) bkt[p] = n[k];
+                if (p < cap) bkt[p] = (uint16_t)n[k];
               }
             }
           }
@@ -634,7 +636,7 @@ __global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
         window_edges<PtrT, WG>(a, s_scan, s_start, lo, hi, [&](uint32_t n) {
           const uint32_t k = n >> BKT_SHIFT;
           const uint64_t p = s_base[k] + atomicAdd(&s_cnt[k], 1u);
-          if (p < cap) bkt[p] = n;
+          if (p < cap) bkt[p] = (uint16_t)n;
         });
       }
       __syncthreads();
@@ -688,7 +690,7 @@ __global__ __launch_bounds__(1024) void k_bucket_base(const uint64_t *tot, uint3
 // A work-group per slice: new vertices of the slice from its targets.
 template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32_t n_rows,
-                                                     uint32_t n_cols, const uint32_t *bkt,
+                                                     uint32_t n_cols, const uint16_t *bkt,
                                                      const uint64_t *base, uint64_t cap,
                                                      uint32_t *vis,
                                                      int32_t *levels, int32_t next_level,
@@ -709,7 +711,6 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
   }
   __syncthreads();
   const uint64_t p0 = base[k], p1 = base[k + 1] < cap ? base[k + 1] : cap;
-  const uint32_t slice0 = k << BKT_SHIFT;
   // eight loads in flight per thread, then the LDS updates
   constexpr uint32_t U = 8;
   for (uint64_t p = p0 + threadIdx.x; p < p1; p += (uint64_t)WG * U) {
@@ -717,7 +718,7 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
 #pragma unroll
     for (uint32_t u = 0; u < U; u++) {
       const uint64_t q = p + (uint64_t)u * WG;
-      t[u] = q < p1 ? bkt[q] - slice0 : 0xFFFFFFFFu;
+      t[u] = q < p1 ? (uint32_t)bkt[q] : 0xFFFFFFFFu;
     }
 #pragma unroll
     for (uint32_t u = 0; u < U; u++) {
@@ -1455,19 +1456,19 @@ static void glb_bucketed(bfsx_ctx *ctx, const StepArgs &s, const KArgs &a, const
   hipStream_t st = ctx->stream;
   // the frontier's edge count (kept exactly by every step) bounds the targets
   const uint64_t cap = s.EF ? s.EF : 1;
-  ensure(ctx->bkt, sizeof(uint32_t) * cap, st);
+  ensure(ctx->bkt, sizeof(uint16_t) * cap, st);
   ensure(ctx->bkt_cnt, sizeof(uint32_t) * (size_t)grid * n_slices, st);
   ensure(ctx->bkt_tot, sizeof(uint64_t) * n_slices, st);
   ensure(ctx->bkt_base, sizeof(uint64_t) * (n_slices + 1), st);
   uint32_t *cnt = ctx->bkt_cnt.as<uint32_t>();
   uint64_t *tot = ctx->bkt_tot.as<uint64_t>();
   uint64_t *base = ctx->bkt_base.as<uint64_t>();
-  uint32_t *bkt = ctx->bkt.as<uint32_t>();
+  uint16_t *bkt = ctx->bkt.as<uint16_t>();
   const uint32_t *clo = ctx->glb_chunk_lo.as<uint32_t>();
   KernelTimer kt(s, st);
   hipLaunchKernelGGL((k_glb_bucket<PtrT, BKT_WG, 1>), dim3(grid), dim3(BKT_WG), 0, st, a, g_scan,
                      g_start, clo, s.ctl, chunk, n_slices, cnt, (const uint64_t *)nullptr,
-                     (uint32_t *)nullptr, (uint64_t)0);
+                     (uint16_t *)nullptr, (uint64_t)0);
   hipLaunchKernelGGL((k_bucket_colscan<256>), dim3(n_slices), dim3(256), 0, st, cnt,
                      (uint32_t)grid, n_slices, tot);
   hipLaunchKernelGGL(k_bucket_base, dim3(1), dim3(1024), 0, st, tot, n_slices, base);
