@@ -500,46 +500,8 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
 constexpr uint32_t BKT_SHIFT = 16;
 constexpr uint32_t BKT_MAX = 2048;  // slices: graphs of up to 2^27 vertices
 constexpr int BKT_WG = 1024;        // work-group size of the two passes
-
-// Calls act(target) for every edge of [lo, hi) of the staged window (as
-// expand_edges: four consecutive edges per lane, all loads first); a range
-// inside one list (a hub's chunk) needs no owner search.  In pass 2 the
-// lanes that hit the same slice in one instruction get consecutive slots
-// from the LDS cursor, so the stores of a hub's sorted list coalesce.
-template <typename PtrT, int WG, class Act>
-__device__ __forceinline__ void window_edges(const KArgs &a, const PtrT *s_scan,
-                                             const PtrT *s_start, PtrT lo, PtrT hi, Act act) {
-  if (lo < hi && s_scan[1] >= hi) {
-    const PtrT off = s_start[0] - s_scan[0];
-    for (PtrT base = lo + (PtrT)threadIdx.x * EPT; base < hi; base += (PtrT)WG * EPT) {
-      uint32_t n[EPT];
-#pragma unroll
-      for (int k = 0; k < EPT; k++) n[k] = base + k < hi ? a.row_idx[off + base + k] : 0u;
-#pragma unroll
-      for (int k = 0; k < EPT; k++)
-        if (base + k < hi) act(n[k]);
-    }
-    return;
-  }
-  for (PtrT base = lo + (PtrT)threadIdx.x * EPT; base < hi; base += (PtrT)WG * EPT) {
-    uint32_t j = 0;
-#pragma unroll
-    for (uint32_t step = WG / 2; step > 0; step >>= 1)
-      if (s_scan[j + step] <= base) j += step;
-    uint32_t n[EPT];
-    bool valid[EPT];
-#pragma unroll
-    for (int k = 0; k < EPT; k++) {
-      const PtrT e = base + k;
-      valid[k] = e < hi;
-      while (valid[k] && j + 1 < WG && s_scan[j + 1] <= e) j++;
-      n[k] = valid[k] ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < EPT; k++)
-      if (valid[k]) act(n[k]);
-  }
-}
+constexpr int BKT_EPT = 4;          // edges per lane per round
+constexpr uint32_t BKT_CHUNK = BKT_WG * BKT_EPT;  // edges per chunk: one round
 
 // PASS 1: per-slice counts of this group's edges -> cnt[group][slice].
 // PASS 2: slots from pos[group][slice] + base[slice]; targets -> bkt (the
@@ -587,57 +549,81 @@ __global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
       __syncthreads();
       const PtrT lo = e0 > s_win_lo ? e0 : s_win_lo;
       const PtrT hi = e1 < s_win_hi ? e1 : s_win_hi;
-      if (lo < hi && s_scan[1] >= hi) {
-        // The range lies in one sorted list (a hub's chunk).  Instruction k
-        // of a wave takes 64 consecutive edges, one per lane, so equal
-        // slices form runs of adjacent lanes: the first lane of each run
-        // updates the LDS counter once for the run (no same-address
-        // conflicts), and in pass 2 the run's lanes store to consecutive
-        // slots.
-        const PtrT off = s_start[0] - s_scan[0];
-        const uint32_t lane = t & 63;
-        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-        for (PtrT b0 = lo + (PtrT)(t & ~63u) * EPT; b0 < hi; b0 += (PtrT)WG * EPT) {
-          uint32_t n[EPT];
+      // Instruction k of a wave takes 64 consecutive edges of the window,
+      // one per lane.  Lists are sorted, so inside a list equal slices form
+      // runs of adjacent lanes: the first lane of each run updates the LDS
+      // counter once for the run (no same-address conflicts), and in pass 2
+      // the run's lanes store to consecutive slots.  A run may continue
+      // into the next list; its slots are still distinct.
+      const bool single = s_scan[1] >= hi;  // the range lies in one list
+      if (PASS == 1 && !single) {
+        // counting only: EPT consecutive edges per lane and one LDS atomic
+        // per edge (measured faster than the runs when lists are short)
+        for (PtrT base = lo + (PtrT)t * BKT_EPT; base < hi; base += (PtrT)WG * BKT_EPT) {
+          uint32_t j = 0;
 #pragma unroll
-          for (int k = 0; k < EPT; k++) {
-            const PtrT e = b0 + (PtrT)k * 64 + lane;
-            n[k] = e < hi ? a.row_idx[off + e] : 0u;
+          for (uint32_t step = WG / 2; step > 0; step >>= 1)
+            if (s_scan[j + step] <= base) j += step;
+          uint32_t n[BKT_EPT];
+#pragma unroll
+          for (int k = 0; k < BKT_EPT; k++) {
+            const PtrT e = base + k;
+            const bool v = e < hi;
+            while (v && j + 1 < WG && s_scan[j + 1] <= e) j++;
+            n[k] = v ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0xFFFFFFFFu;
           }
 #pragma unroll
-          for (int k = 0; k < EPT; k++) {
-            const bool v = b0 + (PtrT)k * 64 + lane < hi;
-            const uint32_t sl = v ? n[k] >> BKT_SHIFT : 0xFFFFFFFFu;
-            const uint32_t prev = __shfl_up(sl, 1, 64);
-            const bool head = v && (lane == 0 || prev != sl);
-            const unsigned long long hm = __ballot(head);
-            if (!hm) continue;
-            const unsigned long long vm = __ballot(v);
-            uint32_t slot = 0;
-            if (head) {
-              const unsigned long long later = hm & ~upto;
-              const uint32_t end = later ? (uint32_t)(__ffsll(later) - 1) : (uint32_t)__popcll(vm);
-              slot = atomicAdd(&s_cnt[sl], end - lane);
-            }
-            if (PASS == 2) {
-              const uint32_t h = 63u - (uint32_t)__clzll(hm & upto);
-              const uint32_t hs = __shfl(slot, (int)h, 64);
-              if (v) {
-                const uint64_t p = s_base[sl] + hs + (lane - h);
-                if (p < cap) bkt[p] = (uint16_t)n[k];
-              }
+          for (int k = 0; k < BKT_EPT; k++)
+            if (n[k] != 0xFFFFFFFFu) atomicAdd(&s_cnt[n[k] >> BKT_SHIFT], 1u);
+        }
+        __syncthreads();
+        continue;
+      }
+      const uint32_t lane = t & 63;
+      const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+      for (PtrT b0 = lo + (PtrT)(t & ~63u) * BKT_EPT; b0 < hi; b0 += (PtrT)WG * BKT_EPT) {
+        // owner entry of this lane's first edge (binary search in LDS), then
+        // a forward walk for its later edges, 64 apart
+        uint32_t j = 0;
+        if (!single) {
+          const PtrT e = b0 + lane;
+#pragma unroll
+          for (uint32_t step = WG / 2; step > 0; step >>= 1)
+            if (s_scan[j + step] <= e) j += step;
+        }
+        uint32_t n[BKT_EPT];
+#pragma unroll
+        for (int k = 0; k < BKT_EPT; k++) {
+          const PtrT e = b0 + (PtrT)k * 64 + lane;
+          const bool v = e < hi;
+          if (!single)
+            while (v && j + 1 < WG && s_scan[j + 1] <= e) j++;
+          n[k] = v ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < BKT_EPT; k++) {
+          const bool v = b0 + (PtrT)k * 64 + lane < hi;
+          const uint32_t sl = v ? n[k] >> BKT_SHIFT : 0xFFFFFFFFu;
+          const uint32_t prev = __shfl_up(sl, 1, 64);
+          const bool head = v && (lane == 0 || prev != sl);
+          const unsigned long long hm = __ballot(head);
+          if (!hm) continue;
+          const unsigned long long vm = __ballot(v);
+          uint32_t slot = 0;
+          if (head) {
+            const unsigned long long later = hm & ~upto;
+            const uint32_t end = later ? (uint32_t)(__ffsll(later) - 1) : (uint32_t)__popcll(vm);
+            slot = atomicAdd(&s_cnt[sl], end - lane);
+          }
+          if (PASS == 2) {
+            const uint32_t h = 63u - (uint32_t)__clzll(hm & upto);
+            const uint32_t hs = __shfl(slot, (int)h, 64);
+            if (v) {
+              const uint64_t p = s_base[sl] + hs + (lane - h);
+              if (p < cap) bkt[p] = (uint16_t)n[k];
             }
           }
         }
-      } else if (PASS == 1) {
-        window_edges<PtrT, WG>(a, s_scan, s_start, lo, hi,
-                               [&](uint32_t n) { atomicAdd(&s_cnt[n >> BKT_SHIFT], 1u); });
-      } else {
-        window_edges<PtrT, WG>(a, s_scan, s_start, lo, hi, [&](uint32_t n) {
-          const uint32_t k = n >> BKT_SHIFT;
-          const uint64_t p = s_base[k] + atomicAdd(&s_cnt[k], 1u);
-          if (p < cap) bkt[p] = (uint16_t)n;
-        });
       }
       __syncthreads();
     }
@@ -1492,7 +1478,7 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   // (exact, kept on the device) would not give every expand work-group an
   // item, as few as GLB_MIN_CHUNK so that they spread over the grid (a
   // frontier of two hubs otherwise runs on a handful of work-groups).
-  uint32_t chunk = ctx->params.glb_chunk;
+  uint32_t chunk = bucketed ? BKT_CHUNK : ctx->params.glb_chunk;
   if (!bucketed && s.EF > 0 && s.EF < (unsigned long long)chunk * grid) {
     const unsigned long long want = (s.EF + grid - 1) / grid;
     chunk = (uint32_t)std::max<unsigned long long>(GLB_MIN_CHUNK, (want + 255) & ~255ull);
