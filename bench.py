@@ -213,6 +213,7 @@ def bench_batch(eng, args):
         "data": "synthetic (graphs generated on the GPU from fixed seeds)",
         "config": {"workload": "batch22", "vertices": info["n_cols"], "nnz": info["nnz"],
                    "searches": int(len(sources)), "bits": n_bits, "graphs": int(len(off) - 1),
+                   "adjacency_order": workloads.ORDER,
                    "levels": st["n_levels"], "push_levels": st["push_levels"],
                    "pull_levels": st["pull_levels"], "batch_ms": round(t, 3),
                    "edges_all_searches": st["edges_count"]},

@@ -24,6 +24,7 @@ STRATEGIES = {"llb": LLB, "local": LLB, "glb": GLB, "global": GLB, "slb": SLB,
 STEP_NAMES = ["llb", "glb", "slb", "pull"]
 GRAPH_SYMMETRIC = 1
 GRAPH_HUB_FIRST = 2
+GRAPH_PULL_COPY = 4
 RUN_GRAPH_LOOP = 1
 RUN_TRACE = 2
 RUN_STEP_TIMES = 4
@@ -228,8 +229,10 @@ class Engine:
 
     def order_adjacency(self, order="hub_first"):
         """Re-sorts the resident adjacency lists: "id" (ascending neighbour
-        ids) or "hub_first" (neighbour degree descending, ties by id)."""
-        code = {"id": 0, "hub_first": 1}[order]
+        ids) or "hub_first" (neighbour degree descending, ties by id); or,
+        with "pull_hub_first", keeps them and builds a hub-first copy that
+        only bottom-up steps read."""
+        code = {"id": 0, "hub_first": 1, "pull_hub_first": 2}[order]
         self._check(lib().bfsx_order_adjacency(self._h, code))
 
     def pick_roots(self, n, seed, mode=0):

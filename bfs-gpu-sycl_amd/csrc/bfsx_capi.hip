@@ -77,6 +77,8 @@ static void ensure_pull_state(bfsx_ctx *ctx) {
 }
 
 static void after_graph_change(bfsx_ctx *ctx) {
+  release(ctx->row_idx_pull);  // a copy of the old lists
+  ctx->gflags &= ~BFSX_GRAPH_PULL_COPY;
   free_search_state(ctx);
   graph_summary(ctx);
   alloc_search_state(ctx);
@@ -204,6 +206,7 @@ struct Walker {
         break;
       case BFSX_PULL:
         if (!pull_possible) fail(BFSX_ERR_UNSUPPORTED, "bottom-up step on a non-symmetric graph");
+        if (ctx->row_idx_pull.p) a.row_idx = ctx->row_idx_pull.as<uint32_t>();
         // (the queue is valid whenever the bitmap is not: only a bottom-up
         // step leaves the queue stale, and it writes the bitmap)
         if (!fbm_valid) queue_to_fbm(ctx, a.q_in, F, ctx->fbm[fbm_cur].as<uint32_t>());
@@ -647,6 +650,7 @@ void bfsx_destroy(bfsx_ctx *ctx) {
   free_search_state(ctx);
   release(ctx->col_ptr);
   release(ctx->row_idx);
+  release(ctx->row_idx_pull);
   release(ctx->iso_bm);
   release(ctx->ctl);
   release(ctx->scratch);
@@ -794,9 +798,21 @@ int bfsx_order_adjacency(bfsx_ctx *ctx, int order) {
   return guarded(ctx, [&] {
     set_device(ctx);
     if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
-    if (order != BFSX_ORDER_ID && order != BFSX_ORDER_HUB_FIRST)
-      fail(BFSX_ERR_ARG, "order must be BFSX_ORDER_ID or BFSX_ORDER_HUB_FIRST");
-    order_adjacency(ctx, order == BFSX_ORDER_HUB_FIRST);
+    if (order != BFSX_ORDER_ID && order != BFSX_ORDER_HUB_FIRST &&
+        order != BFSX_ORDER_PULL_HUB_FIRST)
+      fail(BFSX_ERR_ARG,
+           "order must be BFSX_ORDER_ID, BFSX_ORDER_HUB_FIRST or BFSX_ORDER_PULL_HUB_FIRST");
+    if (order == BFSX_ORDER_PULL_HUB_FIRST) {
+      ensure(ctx->row_idx_pull, sizeof(uint32_t) * std::max<uint64_t>(ctx->nnz, 1), ctx->stream);
+      order_adjacency(ctx, true, ctx->row_idx_pull.as<uint32_t>());
+      ctx->gflags |= BFSX_GRAPH_PULL_COPY;
+      return;
+    }
+    release(ctx->row_idx_pull);
+    ctx->gflags &= ~BFSX_GRAPH_PULL_COPY;
+    order_adjacency(ctx, order == BFSX_ORDER_HUB_FIRST, ctx->row_idx.as<uint32_t>());
+    if (order == BFSX_ORDER_HUB_FIRST) ctx->gflags |= BFSX_GRAPH_HUB_FIRST;
+    else ctx->gflags &= ~BFSX_GRAPH_HUB_FIRST;
   });
 }
 

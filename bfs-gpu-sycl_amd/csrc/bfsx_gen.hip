@@ -413,7 +413,7 @@ __global__ void k_rows_from_keys(const uint64_t *sorted, uint64_t nnz, uint64_t 
 }
 
 template <typename PtrT>
-static void order_adjacency_t(bfsx_ctx *ctx, bool hub_first) {
+static void order_adjacency_t(bfsx_ctx *ctx, bool hub_first, uint32_t *out) {
   hipStream_t s = ctx->stream;
   const uint32_t n = ctx->n_cols;
   const uint64_t nnz = ctx->nnz;
@@ -467,18 +467,15 @@ static void order_adjacency_t(bfsx_ctx *ctx, bool hub_first) {
                                       sorted.as<uint64_t>(), (size_t)nnz, 0, end_bit, s));
     hipLaunchKernelGGL(k_rows_from_keys, dim3(grid_for(nnz)), dim3(256), 0, s,
                        sorted.as<uint64_t>(), nnz, mask,
-                       hub_first ? order.as<uint32_t>() : (const uint32_t *)nullptr,
-                       ctx->row_idx.as<uint32_t>());
+                       hub_first ? order.as<uint32_t>() : (const uint32_t *)nullptr, out);
     BFSX_HIP(hipGetLastError());
   }
   BFSX_HIP(hipStreamSynchronize(s));
 }
 
-void order_adjacency(bfsx_ctx *ctx, bool hub_first) {
-  if (ctx->ptr_bytes == 8) order_adjacency_t<uint64_t>(ctx, hub_first);
-  else order_adjacency_t<uint32_t>(ctx, hub_first);
-  if (hub_first) ctx->gflags |= BFSX_GRAPH_HUB_FIRST;
-  else ctx->gflags &= ~BFSX_GRAPH_HUB_FIRST;
+void order_adjacency(bfsx_ctx *ctx, bool hub_first, uint32_t *out) {
+  if (ctx->ptr_bytes == 8) order_adjacency_t<uint64_t>(ctx, hub_first, out);
+  else order_adjacency_t<uint32_t>(ctx, hub_first, out);
 }
 
 // ---- seeded root candidates ------------------------------------------------

@@ -76,6 +76,7 @@ struct bfsx_ctx {
   uint32_t gflags = 0;
   uint32_t max_degree = 0, n_isolated = 0;
   bfsx::DevBuf col_ptr, row_idx, iso_bm;
+  bfsx::DevBuf row_idx_pull;  // hub-first copy read by bottom-up steps (optional)
 
   // per-search state, sized for the resident graph
   bfsx::DevBuf levels, parents, vis, fbm[2], queue[2], ctl, mark;
@@ -132,7 +133,9 @@ uint64_t gen_grid_pairs(bfsx_ctx *ctx, uint32_t side, uint32_t t_drop,
                         uint32_t *d_v);
 void graph_summary(bfsx_ctx *ctx);  // max degree, isolated bitmap/count
 // re-sorts every adjacency list by neighbour id or hub first (needs graph_summary)
-void order_adjacency(bfsx_ctx *ctx, bool hub_first);
+// Sorts every adjacency list (id order, or hub first) into `out`, which is
+// ctx->row_idx (in place) or a copy of the same size.
+void order_adjacency(bfsx_ctx *ctx, bool hub_first, uint32_t *out);
 void root_candidates(bfsx_ctx *ctx, uint64_t seed, uint64_t first,
                      uint32_t count, const int32_t *d_component_levels,
                      uint32_t *d_out_vertex, uint8_t *d_out_ok);

@@ -486,6 +486,7 @@ static MsResult ms_run_t(bfsx_ctx *ctx, const uint32_t *d_src, const uint16_t *d
     a.overflow = &ctl->overflow;
     a.q_edges_out = &ctl->q_edges[qi ^ 1];
     const bool push = !symmetric || f_edges * push_divisor < (double)ctx->nnz;
+    if (!push && ctx->row_idx_pull.p) a.row_idx = ctx->row_idx_pull.as<uint32_t>();
     if (push) {
       hipLaunchKernelGGL(k_fill_u32, dim3(ms_grid(n * W, 256, cap)), dim3(256), 0, s, nxt,
                          (size_t)(n * W), 0u);

@@ -37,13 +37,15 @@ for _k, _side in enumerate([1024, 2048, 3072, 4096]):
         seed_graph=7050 + _k, seed_roots=7250 + _k, n_roots=4, ptr_bytes=4)
 
 
-# Adjacency order the workloads are searched in (bfsx_order_adjacency).
-ORDER = "id"
+# Adjacency order the workloads are searched in (bfsx_order_adjacency):
+# lists in id order for top-down steps, a hub-first copy for bottom-up ones.
+ORDER = "pull_hub_first"
 
 
 def build(engine, name, order=ORDER, **override):
     """Generates workload `name` in HBM and orders its adjacency lists
-    (`order` = "hub_first", "id" or None to keep the builder's id order);
+    (`order` = "pull_hub_first", "hub_first", "id" or None to keep the
+    builder's id order);
     returns (config dict, nnz)."""
     w = dict(WORKLOADS[name] if name in WORKLOADS else TRAIN_WORKLOADS[name], **override)
     if w["kind"] == "rmat":
@@ -74,10 +76,10 @@ def small_graph_shapes(n_graphs=64):
     return out
 
 
-def build_batch(engine, big="rmat22", n_small=64, n_big_roots=256):
+def build_batch(engine, big="rmat22", n_small=64, n_big_roots=256, order=ORDER):
     """The batched workload: R-MAT scale 22 (256 roots, bits 0..255) packed in
-    one CSC with 64 small R-MAT graphs, graph i searched from one root in bit i.
-    Returns (sources, bits, offsets, n_bits)."""
+    one CSC with 64 small R-MAT graphs, graph i searched from one root in bit i,
+    adjacency ordered as in build().  Returns (sources, bits, offsets, n_bits)."""
     import numpy as np
     w = WORKLOADS[big]
     a, b, c = RMAT_ABC
@@ -99,6 +101,8 @@ def build_batch(engine, big="rmat22", n_small=64, n_big_roots=256):
         rng = np.random.default_rng(6000 + i)
         cand = lo + rng.permutation(hi - lo)
         small_roots.append(int(cand[np.argmax(deg[cand] > 0)]))
+    if order and order != "id":
+        engine.order_adjacency(order)
     sources = np.concatenate([big_roots, small_roots]).astype(np.uint32)
     bits = np.concatenate([np.arange(len(big_roots)), np.arange(n_small)]).astype(np.uint16)
     return sources, bits, off, max(len(big_roots), n_small)

@@ -58,6 +58,7 @@ extern "C" {
 /* graph flags */
 #define BFSX_GRAPH_SYMMETRIC 1u /* every stored edge u->v has v->u */
 #define BFSX_GRAPH_HUB_FIRST 2u /* adjacency lists ordered hub first (below) */
+#define BFSX_GRAPH_PULL_COPY 4u /* bottom-up steps read a hub-first copy (below) */
 
 /* run flags */
 #define BFSX_RUN_GRAPH_LOOP 1u  /* capture the level loop in a hipGraph with a
@@ -165,12 +166,18 @@ int bfsx_graph_info_get(bfsx_ctx *ctx, bfsx_graph_info *out);
 /* Re-sorts every adjacency list of the resident graph in place (col_ptr is
  * unchanged): BFSX_ORDER_ID = ascending neighbour ids (the builder's and
  * scipy's CSC order), BFSX_ORDER_HUB_FIRST = neighbours by degree
- * descending, ties by id (sets BFSX_GRAPH_HUB_FIRST).  Levels do not change;
- * bottom-up steps end their scans earlier on power-law graphs.  A layout
- * step of this engine: the reference keeps lists in file order
- * (src/graph_load.hpp:23-131). */
+ * descending, ties by id (sets BFSX_GRAPH_HUB_FIRST).  Either drops a
+ * bottom-up copy.  BFSX_ORDER_PULL_HUB_FIRST keeps the lists as they are for
+ * top-down steps and builds a second, hub-first copy of row_idx (4 bytes per
+ * stored edge of HBM) that only bottom-up steps read (sets
+ * BFSX_GRAPH_PULL_COPY; a new graph drops it).  Levels do not change;
+ * bottom-up steps end their scans earlier on power-law graphs, while
+ * top-down steps keep the id order's sequential sweep of the visited
+ * bitmap.  A layout step of this engine: the reference keeps lists in file
+ * order (src/graph_load.hpp:23-131). */
 #define BFSX_ORDER_ID 0
 #define BFSX_ORDER_HUB_FIRST 1
+#define BFSX_ORDER_PULL_HUB_FIRST 2
 int bfsx_order_adjacency(bfsx_ctx *ctx, int order);
 /* Copies the resident graph to host buffers of n_cols+1 / nnz entries. */
 int bfsx_download_csc(bfsx_ctx *ctx, void *col_ptr, uint32_t *row_idx);

@@ -215,6 +215,39 @@ def test_hub_first_adjacency_order(engine, ptr):
     assert (cp3 == cp).all() and (ri3 == ri).all()
 
 
+@pytest.mark.parametrize("ptr", [4, 8])
+def test_pull_hub_first_copy(engine, ptr):
+    """A hub-first copy read only by bottom-up steps: the resident lists stay
+    in id order, every step kind (bottom-up ones through the copy) gives the
+    oracle's levels and valid parents, the multi-source search too; a new
+    graph, or re-sorting the lists, drops the copy."""
+    engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 85, 86, ptr)
+    cp, ri = engine.download_csc()
+    engine.order_adjacency("pull_hub_first")
+    flags = engine.info()["flags"]
+    assert flags & bfsx.GRAPH_PULL_COPY and not flags & bfsx.GRAPH_HUB_FIRST
+    cp2, ri2 = engine.download_csc()
+    assert (cp2 == cp).all() and (ri2 == ri).all()
+    roots = engine.pick_roots(6, 87)
+    assert_levels(engine, cp, ri, roots, STEPS, parents=True)
+    engine.load_scheduler()
+    try:
+        assert_levels(engine, cp, ri, roots, ["learned"])
+    finally:
+        engine.set_scheduler_weights(None)
+    many = engine.pick_roots(128, 88)
+    lev, st = engine.run_multi(many, np.arange(128), 128)
+    assert st["pull_levels"] > 0
+    assert (lev == cpu.msbfs_u8(cp, ri, many, np.arange(128), 128)).all()
+    engine.order_adjacency("id")
+    assert not engine.info()["flags"] & bfsx.GRAPH_PULL_COPY
+    engine.order_adjacency("pull_hub_first")
+    engine.generate_rmat(12, 8, 0.57, 0.19, 0.19, 85, 86, ptr)
+    assert not engine.info()["flags"] & bfsx.GRAPH_PULL_COPY
+    cp, ri = engine.download_csc()
+    assert_levels(engine, cp, ri, engine.pick_roots(3, 89), ["pull"])
+
+
 def test_directed_graph_with_fewer_rows(engine):
     """Directed input, duplicates, self loops, n_rows < n_cols (reference quirk)."""
     rng = np.random.default_rng(9)
