@@ -927,9 +927,25 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         }
         if (a.examined) scanned += (unsigned long long)((k + 4 < e ? k + 4 : e) - b);
       }
-      // long lists: four at a time, each by a quarter wave (16 lanes x 4
-      // loads = 64 consecutive neighbours per round, all loads in flight)
-      unsigned long long big = __ballot(cand && deg > PULL_WAVE_DEG);
+      // long lists: first four probes by the candidate's own lane (with
+      // hub-first lists most candidates stop here), then the rest four
+      // lists at a time, each by a quarter wave (16 lanes x 4 loads = 64
+      // consecutive neighbours per round, all loads in flight)
+      if (cand && deg > PULL_WAVE_DEG) {
+        const uint32_t w0 = a.row_idx[b], w1 = a.row_idx[b + 1];
+        const uint32_t w2 = a.row_idx[b + 2], w3 = a.row_idx[b + 3];
+        const bool h0 = in_frontier(fbm_cur, s_sum, shift, w0);
+        const bool h1 = in_frontier(fbm_cur, s_sum, shift, w1);
+        const bool h2 = in_frontier(fbm_cur, s_sum, shift, w2);
+        const bool h3 = in_frontier(fbm_cur, s_sum, shift, w3);
+        if (h0 | h1 | h2 | h3) {
+          found = true;
+          par = h0 ? w0 : h1 ? w1 : h2 ? w2 : w3;
+        }
+        b += 4;
+        if (a.examined) scanned += 4;
+      }
+      unsigned long long big = __ballot(cand && deg > PULL_WAVE_DEG && !found);
       const uint32_t q = lane >> 4, ql = lane & 15;
       const unsigned long long qmask = 0xFFFFull << (16 * q);
       while (big) {

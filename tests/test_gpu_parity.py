@@ -248,6 +248,19 @@ def test_pull_hub_first_copy(engine, ptr):
     assert_levels(engine, cp, ri, engine.pick_roots(3, 89), ["pull"])
 
 
+def test_pull_scale24_hub_first_copy(engine):
+    """R-MAT scale 24 (16.8 M vertices: the LDS frontier summary is one bit
+    per 64 vertices, coarser than on the smaller graphs) through bottom-up
+    steps only, reading the hub-first copy: levels bit-exact."""
+    engine.generate_rmat(24, 16, 0.57, 0.19, 0.19, 91, 92, 8)
+    engine.order_adjacency("pull_hub_first")
+    cp, ri = engine.download_csc()
+    for r in engine.pick_roots(2, 93):
+        lev, st = engine.run(int(r), "pull")
+        want, ec, _ = cpu.bfs_queue(cp, ri, int(r))
+        assert np.array_equal(lev, want) and st["edges_count"] == ec, int(r)
+
+
 def test_directed_graph_with_fewer_rows(engine):
     """Directed input, duplicates, self loops, n_rows < n_cols (reference quirk)."""
     rng = np.random.default_rng(9)
