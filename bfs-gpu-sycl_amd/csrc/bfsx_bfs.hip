@@ -781,8 +781,9 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
 // ---------------------------------------------------------------------------
 // PULL (bottom-up)
 // ---------------------------------------------------------------------------
-// Candidates with more neighbours than this are scanned by the whole wave.
-constexpr uint32_t PULL_WAVE_DEG = 64;
+// Neighbours a candidate scans in its own lane before the rest of its list
+// goes to a quarter wave (a multiple of 4).
+constexpr uint32_t PULL_LANE_PREFIX = 256;
 // A wave takes 2048 vertices (64 visited-bitmap words) at a time.
 constexpr uint32_t PULL_CHUNK = 2048;
 // Frontier summary: one bit per 2^shift vertices, at most
@@ -842,8 +843,9 @@ __device__ __forceinline__ uint32_t kth_set_bit(uint32_t m, uint32_t k) {
 // unvisited bits.  The unvisited vertices are then checked 64 at a time,
 // one per lane: a wave prefix sum of the per-word counts and a shuffle
 // binary search give each lane its word, a popcount search its bit (no
-// LDS list).  Lists longer than PULL_WAVE_DEG are scanned by the whole
-// wave.  Frontier probes go through the LDS summary when the frontier is
+// LDS list).  A candidate scans the first PULL_LANE_PREFIX neighbours in
+// its lane, the rest of a longer list is scanned by a quarter wave.
+// Frontier probes go through the LDS summary when the frontier is
 // sparse.  Found vertices set bits in 64 LDS words that the wave then writes
 // as the next frontier bitmap and ORs into the visited bitmap (the wave owns
 // both word ranges: no global atomics).  The next frontier's size and degree
@@ -905,16 +907,22 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         e = col_ptr[u + 1];
       }
       const PtrT deg = e - b;
-      // short lists: four independent probes per round (indices past the
-      // end repeat the last neighbour, which is harmless)
-      if (cand && deg <= PULL_WAVE_DEG) {
+      // Every candidate first scans up to PULL_LANE_PREFIX neighbours in its
+      // own lane, four independent probes per round (indices past the end of
+      // the prefix repeat its last neighbour, which is harmless): all 64
+      // candidates of the wave progress together, and with hub-first lists
+      // most stop in the first round.  Longer lists not decided by their
+      // prefix continue four at a time, each by a quarter wave (16 lanes x 4
+      // loads = 64 consecutive neighbours per round, all loads in flight).
+      if (cand) {
+        const PtrT lim = deg <= PULL_LANE_PREFIX ? e : b + PULL_LANE_PREFIX;
         PtrT k = b;
-        for (; k < e; k += 4) {
-          const PtrT last = e - 1;
+        for (; k < lim; k += 4) {
+          const PtrT last = lim - 1;
           const uint32_t w0 = a.row_idx[k];
-          const uint32_t w1 = a.row_idx[k + 1 < e ? k + 1 : last];
-          const uint32_t w2 = a.row_idx[k + 2 < e ? k + 2 : last];
-          const uint32_t w3 = a.row_idx[k + 3 < e ? k + 3 : last];
+          const uint32_t w1 = a.row_idx[k + 1 < lim ? k + 1 : last];
+          const uint32_t w2 = a.row_idx[k + 2 < lim ? k + 2 : last];
+          const uint32_t w3 = a.row_idx[k + 3 < lim ? k + 3 : last];
           const bool h0 = in_frontier(fbm_cur, s_sum, shift, w0);
           const bool h1 = in_frontier(fbm_cur, s_sum, shift, w1);
           const bool h2 = in_frontier(fbm_cur, s_sum, shift, w2);
@@ -925,27 +933,10 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
             break;
           }
         }
-        if (a.examined) scanned += (unsigned long long)((k + 4 < e ? k + 4 : e) - b);
+        if (a.examined) scanned += (unsigned long long)((k + 4 < lim ? k + 4 : lim) - b);
+        b = lim;  // where a long list's quarter-wave scan resumes
       }
-      // long lists: first four probes by the candidate's own lane (with
-      // hub-first lists most candidates stop here), then the rest four
-      // lists at a time, each by a quarter wave (16 lanes x 4 loads = 64
-      // consecutive neighbours per round, all loads in flight)
-      if (cand && deg > PULL_WAVE_DEG) {
-        const uint32_t w0 = a.row_idx[b], w1 = a.row_idx[b + 1];
-        const uint32_t w2 = a.row_idx[b + 2], w3 = a.row_idx[b + 3];
-        const bool h0 = in_frontier(fbm_cur, s_sum, shift, w0);
-        const bool h1 = in_frontier(fbm_cur, s_sum, shift, w1);
-        const bool h2 = in_frontier(fbm_cur, s_sum, shift, w2);
-        const bool h3 = in_frontier(fbm_cur, s_sum, shift, w3);
-        if (h0 | h1 | h2 | h3) {
-          found = true;
-          par = h0 ? w0 : h1 ? w1 : h2 ? w2 : w3;
-        }
-        b += 4;
-        if (a.examined) scanned += 4;
-      }
-      unsigned long long big = __ballot(cand && deg > PULL_WAVE_DEG && !found);
+      unsigned long long big = __ballot(cand && deg > PULL_LANE_PREFIX && !found);
       const uint32_t q = lane >> 4, ql = lane & 15;
       const unsigned long long qmask = 0xFFFFull << (16 * q);
       while (big) {
