@@ -853,7 +853,8 @@ __device__ __forceinline__ uint32_t kth_set_bit(uint32_t m, uint32_t k) {
 template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
                                              uint32_t *fbm_next, const uint32_t *g_sum,
-                                             uint32_t sum_words, uint32_t shift) {
+                                             uint32_t sum_words, uint32_t shift,
+                                             const uint32_t *first) {
   constexpr int NW = WG / 64;
   extern __shared__ uint32_t s_sum_mem[];  // sum_words entries when g_sum is set
   __shared__ uint32_t s_found[NW][64];
@@ -902,11 +903,25 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
       bool found = false;
       uint32_t par = 0;
       PtrT b = 0, e = 0;
+      uint32_t f = ~0u;
       if (cand) {
         b = col_ptr[u];
         e = col_ptr[u + 1];
+        if (first) f = first[u];  // issued with the col_ptr loads
       }
       const PtrT deg = e - b;
+      const PtrT lim = deg <= PULL_LANE_PREFIX ? e : b + PULL_LANE_PREFIX;
+      // With the hub-first copy, first[u] (a dense array read in vertex
+      // order) holds u's highest-degree neighbour: the probe that decides
+      // most candidates of dense frontiers costs no row_idx line.
+      if (first && cand && deg > 0) {
+        if (in_frontier(fbm_cur, s_sum, shift, f)) {
+          found = true;
+          par = f;
+        }
+        b += 1;
+        if (a.examined) scanned += 1;
+      }
       // Every candidate first scans up to PULL_LANE_PREFIX neighbours in its
       // own lane, four independent probes per round (indices past the end of
       // the prefix repeat its last neighbour, which is harmless): all 64
@@ -914,8 +929,7 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
       // most stop in the first round.  Longer lists not decided by their
       // prefix continue four at a time, each by a quarter wave (16 lanes x 4
       // loads = 64 consecutive neighbours per round, all loads in flight).
-      if (cand) {
-        const PtrT lim = deg <= PULL_LANE_PREFIX ? e : b + PULL_LANE_PREFIX;
+      if (cand && !found) {
         PtrT k = b;
         for (; k < lim; k += 4) {
           const PtrT last = lim - 1;
@@ -1588,12 +1602,15 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
   per_cu = by_lds < per_cu ? (by_lds ? by_lds : 1) : per_cu;
   const unsigned grid = cap_grid(((uint64_t)n_chunks + nw - 1) / nw,
                                  (unsigned)ctx->cu_count * per_cu);
+  // the first-neighbour array belongs to the hub-first copy
+  const uint32_t *first =
+      ctx->first_nbr.p && s.row_idx == ctx->row_idx_pull.as<uint32_t>() ? ctx->first_nbr.as<uint32_t>() : nullptr;
   KernelTimer kt(s, st);
   switch (wg) {
-    case 64: hipLaunchKernelGGL((k_pull<PtrT, 64>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
-    case 128: hipLaunchKernelGGL((k_pull<PtrT, 128>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
-    case 256: hipLaunchKernelGGL((k_pull<PtrT, 256>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
-    case 512: hipLaunchKernelGGL((k_pull<PtrT, 512>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift); break;
+    case 64: hipLaunchKernelGGL((k_pull<PtrT, 64>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, first); break;
+    case 128: hipLaunchKernelGGL((k_pull<PtrT, 128>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, first); break;
+    case 256: hipLaunchKernelGGL((k_pull<PtrT, 256>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, first); break;
+    case 512: hipLaunchKernelGGL((k_pull<PtrT, 512>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, first); break;
     default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
   }
 }
