@@ -854,7 +854,7 @@ template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
                                              uint32_t *fbm_next, const uint32_t *g_sum,
                                              uint32_t sum_words, uint32_t shift,
-                                             const uint32_t *first) {
+                                             const uint2 *fd) {
   constexpr int NW = WG / 64;
   extern __shared__ uint32_t s_sum_mem[];  // sum_words entries when g_sum is set
   __shared__ uint32_t s_found[NW][64];
@@ -902,26 +902,32 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
       const uint32_t u = c * PULL_CHUNK + off;
       bool found = false;
       uint32_t par = 0;
-      PtrT b = 0, e = 0;
-      uint32_t f = ~0u;
-      if (cand) {
+      PtrT b = 0, e = 0, deg = 0;
+      if (fd) {
+        // With the hub-first copy, fd[u] = {first neighbour, degree} (a
+        // dense array read in vertex order) holds u's highest-degree
+        // neighbour: on dense frontiers that probe decides most candidates,
+        // which then cost neither a col_ptr nor a row_idx line.
+        const uint2 x = cand ? fd[u] : make_uint2(~0u, 0u);
+        deg = (PtrT)x.y;
+        if (cand && deg > 0) {
+          if (in_frontier(fbm_cur, s_sum, shift, x.x)) {
+            found = true;
+            par = x.x;
+          }
+          if (a.examined) scanned += 1;
+        }
+        if (cand && !found && deg > 1) {
+          b = col_ptr[u] + 1;  // the first neighbour is done
+          e = b + (deg - 1);
+        }
+      } else if (cand) {
         b = col_ptr[u];
         e = col_ptr[u + 1];
-        if (first) f = first[u];  // issued with the col_ptr loads
+        deg = e - b;
       }
-      const PtrT deg = e - b;
-      const PtrT lim = deg <= PULL_LANE_PREFIX ? e : b + PULL_LANE_PREFIX;
-      // With the hub-first copy, first[u] (a dense array read in vertex
-      // order) holds u's highest-degree neighbour: the probe that decides
-      // most candidates of dense frontiers costs no row_idx line.
-      if (first && cand && deg > 0) {
-        if (in_frontier(fbm_cur, s_sum, shift, f)) {
-          found = true;
-          par = f;
-        }
-        b += 1;
-        if (a.examined) scanned += 1;
-      }
+      const bool longer = e - b > PULL_LANE_PREFIX;  // the rest of the list
+      const PtrT lim = longer ? b + PULL_LANE_PREFIX : e;
       // Every candidate first scans up to PULL_LANE_PREFIX neighbours in its
       // own lane, four independent probes per round (indices past the end of
       // the prefix repeat its last neighbour, which is harmless): all 64
@@ -950,7 +956,7 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         if (a.examined) scanned += (unsigned long long)((k + 4 < lim ? k + 4 : lim) - b);
         b = lim;  // where a long list's quarter-wave scan resumes
       }
-      unsigned long long big = __ballot(cand && deg > PULL_LANE_PREFIX && !found);
+      unsigned long long big = __ballot(cand && longer && !found);
       const uint32_t q = lane >> 4, ql = lane & 15;
       const unsigned long long qmask = 0xFFFFull << (16 * q);
       while (big) {
@@ -1036,11 +1042,11 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
   __shared__ uint32_t s_n[NW];
   __shared__ unsigned long long s_d[NW], s_x[NW];
   const uint32_t nf = wave_sum(n_found);
-  const unsigned long long fd = wave_sum(found_deg);
+  const unsigned long long dg = wave_sum(found_deg);
   const unsigned long long sx = a.examined ? wave_sum(scanned) : 0ull;
   if (lane == 0) {
     s_n[wv] = nf;
-    s_d[wv] = fd;
+    s_d[wv] = dg;
     s_x[wv] = sx;
   }
   __syncthreads();
@@ -1602,15 +1608,16 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
   per_cu = by_lds < per_cu ? (by_lds ? by_lds : 1) : per_cu;
   const unsigned grid = cap_grid(((uint64_t)n_chunks + nw - 1) / nw,
                                  (unsigned)ctx->cu_count * per_cu);
-  // the first-neighbour array belongs to the hub-first copy
-  const uint32_t *first =
-      ctx->first_nbr.p && s.row_idx == ctx->row_idx_pull.as<uint32_t>() ? ctx->first_nbr.as<uint32_t>() : nullptr;
+  // the {first neighbour, degree} array belongs to the hub-first copy
+  const uint2 *fd = ctx->pull_fd.p && s.row_idx == ctx->row_idx_pull.as<uint32_t>()
+                        ? ctx->pull_fd.as<uint2>()
+                        : nullptr;
   KernelTimer kt(s, st);
   switch (wg) {
-    case 64: hipLaunchKernelGGL((k_pull<PtrT, 64>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, first); break;
-    case 128: hipLaunchKernelGGL((k_pull<PtrT, 128>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, first); break;
-    case 256: hipLaunchKernelGGL((k_pull<PtrT, 256>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, first); break;
-    case 512: hipLaunchKernelGGL((k_pull<PtrT, 512>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, first); break;
+    case 64: hipLaunchKernelGGL((k_pull<PtrT, 64>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd); break;
+    case 128: hipLaunchKernelGGL((k_pull<PtrT, 128>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd); break;
+    case 256: hipLaunchKernelGGL((k_pull<PtrT, 256>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd); break;
+    case 512: hipLaunchKernelGGL((k_pull<PtrT, 512>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd); break;
     default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
   }
 }

@@ -78,7 +78,7 @@ static void ensure_pull_state(bfsx_ctx *ctx) {
 
 static void after_graph_change(bfsx_ctx *ctx) {
   release(ctx->row_idx_pull);  // a copy of the old lists
-  release(ctx->first_nbr);
+  release(ctx->pull_fd);
   ctx->gflags &= ~BFSX_GRAPH_PULL_COPY;
   free_search_state(ctx);
   graph_summary(ctx);
@@ -652,7 +652,7 @@ void bfsx_destroy(bfsx_ctx *ctx) {
   release(ctx->col_ptr);
   release(ctx->row_idx);
   release(ctx->row_idx_pull);
-  release(ctx->first_nbr);
+  release(ctx->pull_fd);
   release(ctx->iso_bm);
   release(ctx->ctl);
   release(ctx->scratch);
@@ -807,13 +807,15 @@ int bfsx_order_adjacency(bfsx_ctx *ctx, int order) {
     if (order == BFSX_ORDER_PULL_HUB_FIRST) {
       ensure(ctx->row_idx_pull, sizeof(uint32_t) * std::max<uint64_t>(ctx->nnz, 1), ctx->stream);
       order_adjacency(ctx, true, ctx->row_idx_pull.as<uint32_t>());
-      ensure(ctx->first_nbr, sizeof(uint32_t) * std::max<uint64_t>(ctx->n_rows, 1), ctx->stream);
-      first_neighbours(ctx, ctx->row_idx_pull.as<uint32_t>(), ctx->first_nbr.as<uint32_t>());
+      if (ctx->max_degree < 0xFFFFFFFFull) {  // degrees fit the array's 32 bits
+        ensure(ctx->pull_fd, sizeof(uint2) * std::max<uint64_t>(ctx->n_rows, 1), ctx->stream);
+        first_neighbours(ctx, ctx->row_idx_pull.as<uint32_t>(), ctx->pull_fd.as<uint2>());
+      }
       ctx->gflags |= BFSX_GRAPH_PULL_COPY;
       return;
     }
     release(ctx->row_idx_pull);
-    release(ctx->first_nbr);
+    release(ctx->pull_fd);
     ctx->gflags &= ~BFSX_GRAPH_PULL_COPY;
     order_adjacency(ctx, order == BFSX_ORDER_HUB_FIRST, ctx->row_idx.as<uint32_t>());
     if (order == BFSX_ORDER_HUB_FIRST) ctx->gflags |= BFSX_GRAPH_HUB_FIRST;

@@ -474,23 +474,23 @@ static void order_adjacency_t(bfsx_ctx *ctx, bool hub_first, uint32_t *out) {
 }
 
 template <typename PtrT>
-__global__ void k_first_nbr(const PtrT *col_ptr, const uint32_t *row_idx, uint32_t n_rows,
-                            uint32_t *first) {
+__global__ void k_first_deg(const PtrT *col_ptr, const uint32_t *row_idx, uint32_t n_rows,
+                            uint2 *fd) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n_rows;
        v += gridDim.x * blockDim.x) {
-    const PtrT b = col_ptr[v];
-    first[v] = col_ptr[v + 1] > b ? row_idx[b] : ~0u;
+    const PtrT b = col_ptr[v], e = col_ptr[v + 1];
+    fd[v] = make_uint2(e > b ? row_idx[b] : ~0u, (uint32_t)(e - b));
   }
 }
 
-void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint32_t *first) {
+void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint2 *fd) {
   const uint32_t n = ctx->n_rows;
   if (ctx->ptr_bytes == 8)
-    hipLaunchKernelGGL(k_first_nbr<uint64_t>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
-                       ctx->col_ptr.as<uint64_t>(), row_idx, n, first);
+    hipLaunchKernelGGL(k_first_deg<uint64_t>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                       ctx->col_ptr.as<uint64_t>(), row_idx, n, fd);
   else
-    hipLaunchKernelGGL(k_first_nbr<uint32_t>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
-                       ctx->col_ptr.as<uint32_t>(), row_idx, n, first);
+    hipLaunchKernelGGL(k_first_deg<uint32_t>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                       ctx->col_ptr.as<uint32_t>(), row_idx, n, fd);
   BFSX_HIP(hipGetLastError());
   BFSX_HIP(hipStreamSynchronize(ctx->stream));
 }

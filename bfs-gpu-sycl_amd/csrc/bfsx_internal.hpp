@@ -77,7 +77,7 @@ struct bfsx_ctx {
   uint32_t max_degree = 0, n_isolated = 0;
   bfsx::DevBuf col_ptr, row_idx, iso_bm;
   bfsx::DevBuf row_idx_pull;  // hub-first copy read by bottom-up steps (optional)
-  bfsx::DevBuf first_nbr;     // with it: each vertex's first (highest-degree) neighbour
+  bfsx::DevBuf pull_fd;       // with it: {first (highest-degree) neighbour, degree} per vertex
 
   // per-search state, sized for the resident graph
   bfsx::DevBuf levels, parents, vis, fbm[2], queue[2], ctl, mark;
@@ -137,8 +137,9 @@ void graph_summary(bfsx_ctx *ctx);  // max degree, isolated bitmap/count
 // Sorts every adjacency list (id order, or hub first) into `out`, which is
 // ctx->row_idx (in place) or a copy of the same size.
 void order_adjacency(bfsx_ctx *ctx, bool hub_first, uint32_t *out);
-// first[v] = row_idx[col_ptr[v]] for v < n_rows with neighbours, else ~0u.
-void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint32_t *first);
+// fd[v] = {row_idx[col_ptr[v]] (~0u without neighbours), degree} for v < n_rows
+// (degrees below 2^32).
+void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint2 *fd);
 void root_candidates(bfsx_ctx *ctx, uint64_t seed, uint64_t first,
                      uint32_t count, const int32_t *d_component_levels,
                      uint32_t *d_out_vertex, uint8_t *d_out_ok);
