@@ -191,21 +191,26 @@ __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr
       if (s_scan[j + step] <= base) j += step;
     uint32_t n[EPT], from[EPT];
     bool valid[EPT];
+    PtrT idx[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; k++) {
       const PtrT e = base + k;
       valid[k] = e < hi;
       while (valid[k] && j + 1 < WG && s_scan[j + 1] <= e) j++;
-      n[k] = valid[k] ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0u;
+      // past hi: repeat the last valid entry (edge base is always valid), so
+      // that the loads below need no branch and are in flight together
+      idx[k] = valid[k] ? s_start[j] + (e - s_scan[j]) : idx[k > 0 ? k - 1 : 0];
       from[k] = s_v[j];
     }
+#pragma unroll
+    for (int k = 0; k < EPT; k++) n[k] = a.row_idx[idx[k]];
     if (a.mark) {
       uint32_t w[EPT];
 #pragma unroll
-      for (int k = 0; k < EPT; k++) w[k] = valid[k] ? a.vis[n[k] >> 5] : ~0u;
+      for (int k = 0; k < EPT; k++) w[k] = a.vis[n[k] >> 5];
 #pragma unroll
       for (int k = 0; k < EPT; k++)
-        if (!((w[k] >> (n[k] & 31)) & 1u)) {
+        if (valid[k] && !((w[k] >> (n[k] & 31)) & 1u)) {
           a.mark[n[k]] = 1;
           if (a.parents) a.parents[n[k]] = from[k];
         }
@@ -564,17 +569,22 @@ __global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
 #pragma unroll
           for (uint32_t step = WG / 2; step > 0; step >>= 1)
             if (s_scan[j + step] <= base) j += step;
-          uint32_t n[BKT_EPT];
+          PtrT idx[BKT_EPT];
 #pragma unroll
           for (int k = 0; k < BKT_EPT; k++) {
             const PtrT e = base + k;
             const bool v = e < hi;
             while (v && j + 1 < WG && s_scan[j + 1] <= e) j++;
-            n[k] = v ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0xFFFFFFFFu;
+            // past hi: repeat the previous entry (edge base is valid), so
+            // the loads need no branch and are in flight together
+            idx[k] = v ? s_start[j] + (e - s_scan[j]) : idx[k > 0 ? k - 1 : 0];
           }
+          uint32_t n[BKT_EPT];
+#pragma unroll
+          for (int k = 0; k < BKT_EPT; k++) n[k] = a.row_idx[idx[k]];
 #pragma unroll
           for (int k = 0; k < BKT_EPT; k++)
-            if (n[k] != 0xFFFFFFFFu) atomicAdd(&s_cnt[n[k] >> BKT_SHIFT], 1u);
+            if (base + k < hi) atomicAdd(&s_cnt[n[k] >> BKT_SHIFT], 1u);
         }
         __syncthreads();
         continue;
@@ -591,15 +601,21 @@ __global__ __launch_bounds__(WG) void k_glb_bucket(KArgs a, const PtrT *g_scan,
           for (uint32_t step = WG / 2; step > 0; step >>= 1)
             if (s_scan[j + step] <= e) j += step;
         }
-        uint32_t n[BKT_EPT];
+        PtrT idx[BKT_EPT];
 #pragma unroll
         for (int k = 0; k < BKT_EPT; k++) {
           const PtrT e = b0 + (PtrT)k * 64 + lane;
           const bool v = e < hi;
           if (!single)
             while (v && j + 1 < WG && s_scan[j + 1] <= e) j++;
-          n[k] = v ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0u;
+          idx[k] = v ? s_start[j] + (e - s_scan[j]) : ~PtrT(0);
         }
+        // lanes past hi read lane 0's first entry (always valid: b0 < hi), so
+        // the loads need no branch and are in flight together
+        const PtrT safe = __shfl(idx[0], 0, 64);
+        uint32_t n[BKT_EPT];
+#pragma unroll
+        for (int k = 0; k < BKT_EPT; k++) n[k] = a.row_idx[idx[k] == ~PtrT(0) ? safe : idx[k]];
 #pragma unroll
         for (int k = 0; k < BKT_EPT; k++) {
           const bool v = b0 + (PtrT)k * 64 + lane < hi;
@@ -704,7 +720,8 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
 #pragma unroll
     for (uint32_t u = 0; u < U; u++) {
       const uint64_t q = p + (uint64_t)u * WG;
-      t[u] = q < p1 ? (uint32_t)bkt[q] : 0xFFFFFFFFu;
+      const uint32_t x = bkt[q < p1 ? q : p];  // p < p1: no branch around the load
+      t[u] = q < p1 ? x : 0xFFFFFFFFu;
     }
 #pragma unroll
     for (uint32_t u = 0; u < U; u++) {
@@ -753,27 +770,34 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
   constexpr uint32_t R = 8;
   for (uint32_t i0 = wv * WPW; i0 < (wv + 1) * WPW; i0 += 2 * R) {  // two words = 64 vertices
     PtrT lo_[R], hi_[R];
+    bool take[R];
 #pragma unroll
     for (uint32_t r = 0; r < R; r++) {
       const uint32_t i = i0 + 2 * r;
       lo_[r] = hi_[r] = 0;
+      take[r] = false;
       if (w0 + i >= n_words) continue;
       const uint32_t word = s_found[i + (lane >> 5)];
       const bool isnew = (word >> (lane & 31)) & 1u;
       const unsigned long long m = __ballot(isnew);
+      const uint32_t v = (w0 + i) * 32 + lane;
       if (isnew) {
-        const uint32_t v = (w0 + i) * 32 + lane;
         levels[v] = next_level;
         q_out[pos + lanes_below(m)] = v;
-        if (q_edges && v < n_rows) {
-          lo_[r] = col_ptr[v];
-          hi_[r] = col_ptr[v + 1];
-        }
+      }
+      if (q_edges) {
+        // every lane loads (a clamped vertex): no branch around the loads,
+        // so the rounds' loads are in flight together
+        const uint32_t vv = v < n_rows ? v : (n_rows ? n_rows - 1 : 0u);
+        lo_[r] = col_ptr[vv];
+        hi_[r] = col_ptr[vv + 1];
+        take[r] = isnew && v < n_rows;
       }
       pos += (uint32_t)__popcll(m);
     }
 #pragma unroll
-    for (uint32_t r = 0; r < R; r++) dsum += (unsigned long long)(hi_[r] - lo_[r]);
+    for (uint32_t r = 0; r < R; r++)
+      if (take[r]) dsum += (unsigned long long)(hi_[r] - lo_[r]);
   }
   if (q_edges) block_atomic_add<WG>(dsum, q_edges);
 }
@@ -797,14 +821,43 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 // Whether w is in the current frontier; a clear summary bit answers without
-// touching the (L2-missing) frontier bitmap.
+// touching the (L2-missing) frontier bitmap: such a probe reads word 0
+// instead, a line every wave keeps hot.  The load is unconditional, so
+// consecutive probes have their loads in flight together (a load inside a
+// branch is waited for before the branch joins).
+// SUM says whether the LDS summary is in use (a template argument, so no
+// branch separates the loads of consecutive probes).
+template <bool SUM>
 __device__ __forceinline__ bool in_frontier(const uint32_t *fbm, const uint32_t *s_sum,
                                             uint32_t shift, uint32_t w) {
-  if (s_sum) {
+  bool may = true;
+  if (SUM) {
     const uint32_t b = w >> shift;
-    if (!((s_sum[b >> 5] >> (b & 31)) & 1u)) return false;
+    may = (s_sum[b >> 5] >> (b & 31)) & 1u;
   }
-  return (fbm[w >> 5] >> (w & 31)) & 1u;
+  const uint32_t word = fbm[may ? (w >> 5) : 0u];
+  return may && ((word >> (w & 31)) & 1u);
+}
+
+// Four probes, all four loads issued before any result is used.
+template <bool SUM>
+__device__ __forceinline__ void in_frontier4(const uint32_t *fbm, const uint32_t *s_sum,
+                                             uint32_t shift, const uint32_t (&w)[4],
+                                             bool (&h)[4]) {
+  bool may[4];
+  uint32_t word[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    may[i] = true;
+    if (SUM) {
+      const uint32_t b = w[i] >> shift;
+      may[i] = (s_sum[b >> 5] >> (b & 31)) & 1u;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) word[i] = fbm[may[i] ? (w[i] >> 5) : 0u];
+#pragma unroll
+  for (int i = 0; i < 4; i++) h[i] = may[i] && ((word[i] >> (w[i] & 31)) & 1u);
 }
 
 // Summary of a frontier bitmap: bit b = any frontier vertex in
@@ -850,7 +903,7 @@ __device__ __forceinline__ uint32_t kth_set_bit(uint32_t m, uint32_t k) {
 // as the next frontier bitmap and ORs into the visited bitmap (the wave owns
 // both word ranges: no global atomics).  The next frontier's size and degree
 // sum leave with one atomic per work-group; no queue is written.
-template <typename PtrT, int WG>
+template <typename PtrT, int WG, bool SUM>
 __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
                                              uint32_t *fbm_next, const uint32_t *g_sum,
                                              uint32_t sum_words, uint32_t shift,
@@ -908,12 +961,16 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         // dense array read in vertex order) holds u's highest-degree
         // neighbour: on dense frontiers that probe decides most candidates,
         // which then cost neither a col_ptr nor a row_idx line.
-        const uint2 x = cand ? fd[u] : make_uint2(~0u, 0u);
-        deg = (PtrT)x.y;
-        if (cand && deg > 0) {
-          if (in_frontier(fbm_cur, s_sum, shift, x.x)) {
+        // (u is a vertex of this chunk below n_rows for every lane, so the
+        // loads need no branch)
+        const uint2 x = fd[u];
+        deg = cand ? (PtrT)x.y : PtrT(0);
+        const uint32_t f = deg > 0 ? x.x : 0u;
+        const bool hf = in_frontier<SUM>(fbm_cur, s_sum, shift, f);
+        if (deg > 0) {
+          if (hf) {
             found = true;
-            par = x.x;
+            par = f;
           }
           if (a.examined) scanned += 1;
         }
@@ -921,10 +978,13 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
           b = col_ptr[u] + 1;  // the first neighbour is done
           e = b + (deg - 1);
         }
-      } else if (cand) {
-        b = col_ptr[u];
-        e = col_ptr[u + 1];
-        deg = e - b;
+      } else {
+        const PtrT cb = col_ptr[u], ce = col_ptr[u + 1];
+        if (cand) {
+          b = cb;
+          e = ce;
+          deg = e - b;
+        }
       }
       const bool longer = e - b > PULL_LANE_PREFIX;  // the rest of the list
       const PtrT lim = longer ? b + PULL_LANE_PREFIX : e;
@@ -939,17 +999,14 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         PtrT k = b;
         for (; k < lim; k += 4) {
           const PtrT last = lim - 1;
-          const uint32_t w0 = a.row_idx[k];
-          const uint32_t w1 = a.row_idx[k + 1 < lim ? k + 1 : last];
-          const uint32_t w2 = a.row_idx[k + 2 < lim ? k + 2 : last];
-          const uint32_t w3 = a.row_idx[k + 3 < lim ? k + 3 : last];
-          const bool h0 = in_frontier(fbm_cur, s_sum, shift, w0);
-          const bool h1 = in_frontier(fbm_cur, s_sum, shift, w1);
-          const bool h2 = in_frontier(fbm_cur, s_sum, shift, w2);
-          const bool h3 = in_frontier(fbm_cur, s_sum, shift, w3);
-          if (h0 | h1 | h2 | h3) {
+          const uint32_t w[4] = {a.row_idx[k], a.row_idx[k + 1 < lim ? k + 1 : last],
+                                 a.row_idx[k + 2 < lim ? k + 2 : last],
+                                 a.row_idx[k + 3 < lim ? k + 3 : last]};
+          bool h[4];
+          in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
+          if (h[0] | h[1] | h[2] | h[3]) {
             found = true;
-            par = h0 ? w0 : h1 ? w1 : h2 ? w2 : w3;
+            par = h[0] ? w[0] : h[1] ? w[1] : h[2] ? w[2] : w[3];
             break;
           }
         }
@@ -979,14 +1036,14 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
 #pragma unroll
           for (uint32_t j = 0; j < 4; j++) {
             const PtrT k = base + j * 16 + ql;
-            h[j] = false;
-            w[j] = 0;
-            if (!done && k < se) w[j] = a.row_idx[k];
+            // (entry 0 exists: some candidate has a long list)
+            w[j] = a.row_idx[(!done && k < se) ? k : PtrT(0)];
           }
+          in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
 #pragma unroll
           for (uint32_t j = 0; j < 4; j++) {
             const PtrT k = base + j * 16 + ql;
-            if (!done && k < se) h[j] = in_frontier(fbm_cur, s_sum, shift, w[j]);
+            h[j] = h[j] && !done && k < se;
           }
           bool got = false;
           uint32_t gw = 0;
@@ -1614,10 +1671,22 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
                         : nullptr;
   KernelTimer kt(s, st);
   switch (wg) {
-    case 64: hipLaunchKernelGGL((k_pull<PtrT, 64>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd); break;
-    case 128: hipLaunchKernelGGL((k_pull<PtrT, 128>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd); break;
-    case 256: hipLaunchKernelGGL((k_pull<PtrT, 256>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd); break;
-    case 512: hipLaunchKernelGGL((k_pull<PtrT, 512>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd); break;
+    case 64:
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 64, true>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      else hipLaunchKernelGGL((k_pull<PtrT, 64, false>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      break;
+    case 128:
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 128, true>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      else hipLaunchKernelGGL((k_pull<PtrT, 128, false>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      break;
+    case 256:
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 256, true>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      else hipLaunchKernelGGL((k_pull<PtrT, 256, false>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      break;
+    case 512:
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 512, true>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      else hipLaunchKernelGGL((k_pull<PtrT, 512, false>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      break;
     default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
   }
 }
