@@ -87,8 +87,37 @@ __device__ __forceinline__ void ms_mark_active(uint32_t *act_next, uint32_t u0,
   if ((uint32_t)(x >> 32)) atomicOr(&act_next[w + 1], (uint32_t)(x >> 32));
 }
 
-__device__ __forceinline__ bool ms_active(const uint32_t *act, uint32_t w) {
-  return !act || ((act[w >> 5] >> (w & 31)) & 1u);
+
+// Word j of the frontier bitsets of eight neighbours, ORed.  The W-lane group
+// starting at lane g0 holds the ids, PER = 8 / W per lane (0xFFFFFFFF past
+// the list's end).  Every load is unconditional (a skipped neighbour reads
+// vertex 0's hot line instead), so each phase has its eight loads in flight
+// together: the active-map words, then the bitset words.
+template <int W>
+__device__ __forceinline__ uint32_t ms_gather8(const MsArgs &a, const uint32_t (&ids)[8 / W],
+                                               uint32_t g0, uint32_t j) {
+  uint32_t w[8];
+  bool ok[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    w[i] = __shfl(ids[i / W], (int)(g0 + (i % W)), 64);
+    ok[i] = w[i] != 0xFFFFFFFFu;
+    w[i] = ok[i] ? w[i] : 0u;
+  }
+  if (a.act_cur) {
+    uint32_t aw[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) aw[i] = a.act_cur[w[i] >> 5];
+#pragma unroll
+    for (int i = 0; i < 8; i++) ok[i] = ok[i] && ((aw[i] >> (w[i] & 31)) & 1u);
+  }
+  uint32_t x[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) x[i] = a.cur[(size_t)(ok[i] ? w[i] : 0u) * W + j];
+  uint32_t part = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) part |= ok[i] ? x[i] : 0u;
+  return part;
 }
 
 // Adds deg(u) of each group leader that appended u, one atomic per wave.
@@ -222,15 +251,10 @@ __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_tot
 #pragma unroll
         for (int t = 0; t < PER; t++) {
           const PtrT kk = k + (PtrT)(t * W + j);
-          ids[t] = kk < e ? a.row_idx[kk] : 0xFFFFFFFFu;
+          const uint32_t r = a.row_idx[kk < e ? kk : b];  // b < e: no branch around the load
+          ids[t] = kk < e ? r : 0xFFFFFFFFu;
         }
-        uint32_t part = 0;
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-          const uint32_t w = __shfl(ids[i / W], (int)(grp * W + (i % W)), 64);
-          if (w != 0xFFFFFFFFu && ms_active(a.act_cur, w)) part |= a.cur[(size_t)w * W + j];
-        }
-        acc |= part;
+        acc |= ms_gather8<W>(a, ids, grp * W, j);
         // stop once every unseen bit of the group's vertex has arrived
         const bool full = (acc | s) == ~0u;
         if ((__ballot(!full) & gmask) == 0) break;
@@ -283,13 +307,10 @@ __global__ __launch_bounds__(256) void k_ms_pull_hubs(MsArgs a, const uint32_t *
 #pragma unroll
       for (int q = 0; q < PER; q++) {
         const PtrT kk = k + (PtrT)(q * W + j);
-        ids[q] = kk < e ? a.row_idx[kk] : 0xFFFFFFFFu;
+        const uint32_t r = a.row_idx[kk < e ? kk : b];  // b < e: no branch around the load
+        ids[q] = kk < e ? r : 0xFFFFFFFFu;
       }
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        const uint32_t w = __shfl(ids[i / W], (int)((lane / W) * W + (i % W)), 64);
-        if (w != 0xFFFFFFFFu && ms_active(a.act_cur, w)) acc |= a.cur[(size_t)w * W + j];
-      }
+      acc |= ms_gather8<W>(a, ids, (lane / W) * W, j);
       const bool full = (acc | s) == ~0u;
       if ((__ballot(!full) & gmask) == 0) break;
     }
