@@ -189,25 +189,23 @@ __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr
 #pragma unroll
     for (uint32_t step = WG / 2; step > 0; step >>= 1)
       if (s_scan[j + step] <= base) j += step;
+    // (the loads stay under their branches: measured faster for LLB than
+    // clamped unconditional loads, whose extra reads and later issue cost
+    // more than the overlap gained)
     uint32_t n[EPT], from[EPT];
     bool valid[EPT];
-    PtrT idx[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; k++) {
       const PtrT e = base + k;
       valid[k] = e < hi;
       while (valid[k] && j + 1 < WG && s_scan[j + 1] <= e) j++;
-      // past hi: repeat the last valid entry (edge base is always valid), so
-      // that the loads below need no branch and are in flight together
-      idx[k] = valid[k] ? s_start[j] + (e - s_scan[j]) : idx[k > 0 ? k - 1 : 0];
+      n[k] = valid[k] ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0u;
       from[k] = s_v[j];
     }
-#pragma unroll
-    for (int k = 0; k < EPT; k++) n[k] = a.row_idx[idx[k]];
     if (a.mark) {
       uint32_t w[EPT];
 #pragma unroll
-      for (int k = 0; k < EPT; k++) w[k] = a.vis[n[k] >> 5];
+      for (int k = 0; k < EPT; k++) w[k] = valid[k] ? a.vis[n[k] >> 5] : ~0u;
 #pragma unroll
       for (int k = 0; k < EPT; k++)
         if (valid[k] && !((w[k] >> (n[k] & 31)) & 1u)) {
