@@ -806,6 +806,8 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
 // Neighbours a candidate scans in its own lane before the rest of its list
 // goes to a quarter wave (a multiple of 4).
 constexpr uint32_t PULL_LANE_PREFIX = 256;
+// Candidates per lane per round of the bottom-up step.
+constexpr int PULL_SLOTS = 2;
 // A wave takes 2048 vertices (64 visited-bitmap words) at a time.
 constexpr uint32_t PULL_CHUNK = 2048;
 // Frontier summary: one bit per 2^shift vertices, at most
@@ -936,153 +938,185 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
     const uint32_t excl = incl - cnt;
     fw[lane] = 0;
     wave_sync_lds();
-    for (uint32_t r = 0; r < total; r += 64) {
-      const uint32_t idx = r + lane;
-      const bool cand = idx < total;
-      // owner word: the last lane whose exclusive count is <= idx (it has
-      // a non-zero count, since a zero-count lane ties with the next one)
-      uint32_t o = 0;
+    for (uint32_t r = 0; r < total; r += 64 * PULL_SLOTS) {
+      // PULL_SLOTS candidates per lane per round: their first loads and
+      // probes (phase 1) are in flight together; the rest of each one's scan
+      // (phase 2) follows slot by slot, uniformly over the wave.
+      bool cand_[PULL_SLOTS], found_[PULL_SLOTS];
+      uint32_t off_[PULL_SLOTS], u_[PULL_SLOTS], par_[PULL_SLOTS];
+      PtrT b_[PULL_SLOTS], e_[PULL_SLOTS], deg_[PULL_SLOTS];
 #pragma unroll
-      for (uint32_t st = 32; st > 0; st >>= 1) {
-        const uint32_t e = __shfl(excl, (int)(o + st), 64);
-        if (e <= idx) o += st;
+      for (int sl = 0; sl < PULL_SLOTS; sl++) {
+        const uint32_t idx = r + 64 * sl + lane;
+        cand_[sl] = idx < total;
+        // owner word: the last lane whose exclusive count is <= idx (it has
+        // a non-zero count, since a zero-count lane ties with the next one)
+        uint32_t o = 0;
+#pragma unroll
+        for (uint32_t st = 32; st > 0; st >>= 1) {
+          const uint32_t ex = __shfl(excl, (int)(o + st), 64);
+          if (ex <= idx) o += st;
+        }
+        const uint32_t om = __shfl(cm, (int)o, 64);
+        const uint32_t oe = __shfl(excl, (int)o, 64);
+        off_[sl] = cand_[sl] ? o * 32 + kth_set_bit(om, idx - oe) : 0u;
+        u_[sl] = c * PULL_CHUNK + off_[sl];
+        found_[sl] = false;
+        par_[sl] = 0;
+        b_[sl] = e_[sl] = deg_[sl] = 0;
       }
-      const uint32_t om = __shfl(cm, (int)o, 64);
-      const uint32_t oe = __shfl(excl, (int)o, 64);
-      const uint32_t off = cand ? o * 32 + kth_set_bit(om, idx - oe) : 0u;
-      const uint32_t u = c * PULL_CHUNK + off;
-      bool found = false;
-      uint32_t par = 0;
-      PtrT b = 0, e = 0, deg = 0;
       if (fd) {
         // With the hub-first copy, fd[u] = {first neighbour, degree} (a
         // dense array read in vertex order) holds u's highest-degree
         // neighbour: on dense frontiers that probe decides most candidates,
-        // which then cost neither a col_ptr nor a row_idx line.
-        // (u is a vertex of this chunk below n_rows for every lane, so the
-        // loads need no branch)
-        const uint2 x = fd[u];
-        deg = cand ? (PtrT)x.y : PtrT(0);
-        const uint32_t f = deg > 0 ? x.x : 0u;
-        const bool hf = in_frontier<SUM>(fbm_cur, s_sum, shift, f);
-        if (deg > 0) {
-          if (hf) {
-            found = true;
-            par = f;
-          }
-          if (a.examined) scanned += 1;
+        // which then cost neither a col_ptr nor a row_idx line.  (u is a
+        // vertex of this chunk below n_rows for every lane, so the loads
+        // need no branch.)
+        uint2 x[PULL_SLOTS];
+#pragma unroll
+        for (int sl = 0; sl < PULL_SLOTS; sl++) x[sl] = fd[u_[sl]];
+        uint32_t f[PULL_SLOTS];
+        bool hf[PULL_SLOTS];
+#pragma unroll
+        for (int sl = 0; sl < PULL_SLOTS; sl++) {
+          deg_[sl] = cand_[sl] ? (PtrT)x[sl].y : PtrT(0);
+          f[sl] = deg_[sl] > 0 ? x[sl].x : 0u;
         }
-        if (cand && !found && deg > 1) {
-          b = col_ptr[u] + 1;  // the first neighbour is done
-          e = b + (deg - 1);
+#pragma unroll
+        for (int sl = 0; sl < PULL_SLOTS; sl++) hf[sl] = in_frontier<SUM>(fbm_cur, s_sum, shift, f[sl]);
+#pragma unroll
+        for (int sl = 0; sl < PULL_SLOTS; sl++) {
+          if (deg_[sl] > 0) {
+            if (hf[sl]) {
+              found_[sl] = true;
+              par_[sl] = f[sl];
+            }
+            if (a.examined) scanned += 1;
+          }
+          if (cand_[sl] && !found_[sl] && deg_[sl] > 1) {
+            b_[sl] = col_ptr[u_[sl]] + 1;  // the first neighbour is done
+            e_[sl] = b_[sl] + (deg_[sl] - 1);
+          }
         }
       } else {
-        const PtrT cb = col_ptr[u], ce = col_ptr[u + 1];
-        if (cand) {
-          b = cb;
-          e = ce;
-          deg = e - b;
+#pragma unroll
+        for (int sl = 0; sl < PULL_SLOTS; sl++) {
+          const PtrT cb = col_ptr[u_[sl]], ce = col_ptr[u_[sl] + 1];
+          if (cand_[sl]) {
+            b_[sl] = cb;
+            e_[sl] = ce;
+            deg_[sl] = ce - cb;
+          }
         }
       }
-      const bool longer = e - b > PULL_LANE_PREFIX;  // the rest of the list
-      const PtrT lim = longer ? b + PULL_LANE_PREFIX : e;
-      // Every candidate first scans up to PULL_LANE_PREFIX neighbours in its
-      // own lane, four independent probes per round (indices past the end of
-      // the prefix repeat its last neighbour, which is harmless): all 64
-      // candidates of the wave progress together, and with hub-first lists
-      // most stop in the first round.  Longer lists not decided by their
-      // prefix continue four at a time, each by a quarter wave (16 lanes x 4
-      // loads = 64 consecutive neighbours per round, all loads in flight).
-      if (cand && !found) {
-        PtrT k = b;
-        for (; k < lim; k += 4) {
-          const PtrT last = lim - 1;
-          const uint32_t w[4] = {a.row_idx[k], a.row_idx[k + 1 < lim ? k + 1 : last],
-                                 a.row_idx[k + 2 < lim ? k + 2 : last],
-                                 a.row_idx[k + 3 < lim ? k + 3 : last]};
-          bool h[4];
-          in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
-          if (h[0] | h[1] | h[2] | h[3]) {
-            found = true;
-            par = h[0] ? w[0] : h[1] ? w[1] : h[2] ? w[2] : w[3];
-            break;
-          }
-        }
-        if (a.examined) scanned += (unsigned long long)((k + 4 < lim ? k + 4 : lim) - b);
-        b = lim;  // where a long list's quarter-wave scan resumes
-      }
-      unsigned long long big = __ballot(cand && longer && !found);
-      const uint32_t q = lane >> 4, ql = lane & 15;
-      const unsigned long long qmask = 0xFFFFull << (16 * q);
-      while (big) {
-        int my_src = -1;  // the candidate lane my quarter serves
 #pragma unroll
-        for (uint32_t i = 0; i < 4; i++) {
-          if (big) {
-            if (i == q) my_src = __ffsll(big) - 1;
-            big &= big - 1;
-          }
-        }
-        const PtrT sb = __shfl(b, my_src < 0 ? 0 : my_src, 64);
-        const PtrT se = __shfl(e, my_src < 0 ? 0 : my_src, 64);
-        PtrT base = my_src < 0 ? se : sb;
-        bool done = base >= se, hit = false;
-        uint32_t hp = 0;
-        while (__ballot(!done)) {
-          uint32_t w[4];
-          bool h[4];
-#pragma unroll
-          for (uint32_t j = 0; j < 4; j++) {
-            const PtrT k = base + j * 16 + ql;
-            // (entry 0 exists: some candidate has a long list)
-            w[j] = a.row_idx[(!done && k < se) ? k : PtrT(0)];
-          }
-          in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
-#pragma unroll
-          for (uint32_t j = 0; j < 4; j++) {
-            const PtrT k = base + j * 16 + ql;
-            h[j] = h[j] && !done && k < se;
-          }
-          bool got = false;
-          uint32_t gw = 0;
-#pragma unroll
-          for (uint32_t j = 0; j < 4; j++) {
-            const unsigned long long hb = __ballot(h[j]) & qmask;
-            const uint32_t pw = __shfl(w[j], hb ? __ffsll(hb) - 1 : (int)lane, 64);
-            if (!got && hb) {
-              got = true;
-              gw = pw;
+      for (int sl = 0; sl < PULL_SLOTS; sl++) {
+        const bool cand = cand_[sl];
+        const uint32_t off = off_[sl], u = u_[sl];
+        bool found = found_[sl];
+        uint32_t par = par_[sl];
+        PtrT b = b_[sl];
+        const PtrT e = e_[sl], deg = deg_[sl];
+        const bool longer = e - b > PULL_LANE_PREFIX;  // the rest of the list
+        const PtrT lim = longer ? b + PULL_LANE_PREFIX : e;
+        // Every candidate first scans up to PULL_LANE_PREFIX neighbours in its
+        // own lane, four independent probes per round (indices past the end of
+        // the prefix repeat its last neighbour, which is harmless): all 64
+        // candidates of the wave progress together, and with hub-first lists
+        // most stop in the first round.  Longer lists not decided by their
+        // prefix continue four at a time, each by a quarter wave (16 lanes x 4
+        // loads = 64 consecutive neighbours per round, all loads in flight).
+        if (cand && !found) {
+          PtrT k = b;
+          for (; k < lim; k += 4) {
+            const PtrT last = lim - 1;
+            const uint32_t w[4] = {a.row_idx[k], a.row_idx[k + 1 < lim ? k + 1 : last],
+                                   a.row_idx[k + 2 < lim ? k + 2 : last],
+                                   a.row_idx[k + 3 < lim ? k + 3 : last]};
+            bool h[4];
+            in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
+            if (h[0] | h[1] | h[2] | h[3]) {
+              found = true;
+              par = h[0] ? w[0] : h[1] ? w[1] : h[2] ? w[2] : w[3];
+              break;
             }
           }
-          if (!done) {
-            if (a.examined && ql == 0) scanned += (unsigned long long)(se - base < 64 ? se - base : 64);
-            if (got) {
-              hit = true;
-              hp = gw;
-              done = true;
+          if (a.examined) scanned += (unsigned long long)((k + 4 < lim ? k + 4 : lim) - b);
+          b = lim;  // where a long list's quarter-wave scan resumes
+        }
+        unsigned long long big = __ballot(cand && longer && !found);
+        const uint32_t q = lane >> 4, ql = lane & 15;
+        const unsigned long long qmask = 0xFFFFull << (16 * q);
+        while (big) {
+          int my_src = -1;  // the candidate lane my quarter serves
+  #pragma unroll
+          for (uint32_t i = 0; i < 4; i++) {
+            if (big) {
+              if (i == q) my_src = __ffsll(big) - 1;
+              big &= big - 1;
             }
-            base += 64;
-            if (base >= se) done = true;
+          }
+          const PtrT sb = __shfl(b, my_src < 0 ? 0 : my_src, 64);
+          const PtrT se = __shfl(e, my_src < 0 ? 0 : my_src, 64);
+          PtrT base = my_src < 0 ? se : sb;
+          bool done = base >= se, hit = false;
+          uint32_t hp = 0;
+          while (__ballot(!done)) {
+            uint32_t w[4];
+            bool h[4];
+  #pragma unroll
+            for (uint32_t j = 0; j < 4; j++) {
+              const PtrT k = base + j * 16 + ql;
+              // (entry 0 exists: some candidate has a long list)
+              w[j] = a.row_idx[(!done && k < se) ? k : PtrT(0)];
+            }
+            in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
+  #pragma unroll
+            for (uint32_t j = 0; j < 4; j++) {
+              const PtrT k = base + j * 16 + ql;
+              h[j] = h[j] && !done && k < se;
+            }
+            bool got = false;
+            uint32_t gw = 0;
+  #pragma unroll
+            for (uint32_t j = 0; j < 4; j++) {
+              const unsigned long long hb = __ballot(h[j]) & qmask;
+              const uint32_t pw = __shfl(w[j], hb ? __ffsll(hb) - 1 : (int)lane, 64);
+              if (!got && hb) {
+                got = true;
+                gw = pw;
+              }
+            }
+            if (!done) {
+              if (a.examined && ql == 0) scanned += (unsigned long long)(se - base < 64 ? se - base : 64);
+              if (got) {
+                hit = true;
+                hp = gw;
+                done = true;
+              }
+              base += 64;
+              if (base >= se) done = true;
+            }
+          }
+          // hand each result to its candidate lane
+  #pragma unroll
+          for (uint32_t qq = 0; qq < 4; qq++) {
+            const int s_q = __shfl(my_src, (int)(16 * qq), 64);
+            const bool h_q = __shfl((int)hit, (int)(16 * qq), 64) != 0;
+            const uint32_t p_q = __shfl(hp, (int)(16 * qq), 64);
+            if (s_q == (int)lane) {
+              found = h_q;
+              par = p_q;
+            }
           }
         }
-        // hand each result to its candidate lane
-#pragma unroll
-        for (uint32_t qq = 0; qq < 4; qq++) {
-          const int s_q = __shfl(my_src, (int)(16 * qq), 64);
-          const bool h_q = __shfl((int)hit, (int)(16 * qq), 64) != 0;
-          const uint32_t p_q = __shfl(hp, (int)(16 * qq), 64);
-          if (s_q == (int)lane) {
-            found = h_q;
-            par = p_q;
-          }
+        if (found) {
+          a.levels[u] = a.next_level;
+          if (a.parents) a.parents[u] = par;
+          atomicOr(&fw[off >> 5], 1u << (off & 31));
+          found_deg += (unsigned long long)deg;
         }
-      }
-      if (found) {
-        a.levels[u] = a.next_level;
-        if (a.parents) a.parents[u] = par;
-        atomicOr(&fw[off >> 5], 1u << (off & 31));
-        found_deg += (unsigned long long)deg;
-      }
+      }  // slots
     }
     wave_sync_lds();
     const uint32_t f = fw[lane];
