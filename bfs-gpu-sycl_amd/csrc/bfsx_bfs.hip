@@ -738,6 +738,7 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
   constexpr uint32_t NW = WG / 64;
   constexpr uint32_t WPW = WPS / NW;  // words per wave
   __shared__ uint32_t s_wcnt[NW + 1];
+  __shared__ bool s_dense;
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint32_t cnt = 0;
   for (uint32_t i = wv * WPW + lane; i < (wv + 1) * WPW; i += 64) {
@@ -760,8 +761,13 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
       run += c;
     }
     s_qbase = run ? atomicAdd(q_count, run) : 0u;
+    s_dense = run * 8u >= WPS * 32u;
   }
   __syncthreads();
+  // a slice with at least 1/8 of its vertices new loads col_ptr for every
+  // lane (no branch around the loads: the rounds' loads are in flight
+  // together); a sparse one only for its new vertices
+  const bool dense = s_dense;
   uint32_t pos = s_qbase + s_wcnt[wv];
   unsigned long long dsum = 0;
   // eight rounds at a time, so their col_ptr loads are in flight together
@@ -784,12 +790,15 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
         q_out[pos + lanes_below(m)] = v;
       }
       if (q_edges) {
-        // every lane loads (a clamped vertex): no branch around the loads,
-        // so the rounds' loads are in flight together
-        const uint32_t vv = v < n_rows ? v : (n_rows ? n_rows - 1 : 0u);
-        lo_[r] = col_ptr[vv];
-        hi_[r] = col_ptr[vv + 1];
         take[r] = isnew && v < n_rows;
+        if (dense) {
+          const uint32_t vv = v < n_rows ? v : (n_rows ? n_rows - 1 : 0u);
+          lo_[r] = col_ptr[vv];
+          hi_[r] = col_ptr[vv + 1];
+        } else if (take[r]) {
+          lo_[r] = col_ptr[v];
+          hi_[r] = col_ptr[v + 1];
+        }
       }
       pos += (uint32_t)__popcll(m);
     }
