@@ -109,6 +109,11 @@ void GPURun(uint32_t vertexCount, vectorT &IndexHost, vectorT &OffsetHost,
                bfsx_upload_csc(g.ctx, n_cols, vertexCount, cp.data(), 8, ri.data(), ri.size(), flags),
                "bfsx_upload_csc");
   }
+  // bottom-up steps read a hub-first copy of the lists, built with the
+  // upload (outside the timed region, like the upload itself)
+  if (cfg.symmetric && (cfg.strategy == BFSX_PULL || cfg.strategy == BFSX_LEARNED))
+    bfsx_check(g.ctx, bfsx_order_adjacency(g.ctx, BFSX_ORDER_PULL_HUB_FIRST),
+               "bfsx_order_adjacency");
   if (cfg.strategy == BFSX_LEARNED && !cfg.sched_weights.empty())
     bfsx_check(g.ctx,
                bfsx_sched_set_weights(g.ctx, cfg.sched_weights.data(),
