@@ -502,7 +502,7 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
 // Every slice fits in LDS (2^16 vertices: 8 KiB visited + 8 KiB found).
 constexpr uint32_t BKT_SHIFT = 16;
 constexpr uint32_t BKT_MAX = 2048;  // slices: graphs of up to 2^27 vertices
-constexpr int BKT_WG = 1024;        // work-group size of the two passes
+constexpr int BKT_WG = 512;         // work-group size of the two passes
 constexpr int BKT_EPT = 4;          // edges per lane per round
 constexpr uint32_t BKT_CHUNK = BKT_WG * BKT_EPT;  // edges per chunk: one round
 
@@ -1625,7 +1625,7 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   if (bucketed) {
     // fewer, larger groups than the GLB expand grid: longer runs per
     // (group, slice) in the segments that pass 2 writes
-    glb_bucketed<PtrT>(ctx, s, a, g_scan, g_start, chunk, (unsigned)ctx->cu_count * 2, n_slices);
+    glb_bucketed<PtrT>(ctx, s, a, g_scan, g_start, chunk, (unsigned)ctx->cu_count * 4, n_slices);
     return true;
   }
   KernelTimer kt(s, st);
