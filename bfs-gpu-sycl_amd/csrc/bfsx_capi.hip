@@ -323,7 +323,7 @@ static RunOut search_graph_loop(bfsx_ctx *ctx, uint32_t root, int fixed, bool le
   ctx->trace.clear();
   RunOut out;
   BFSX_HIP(hipEventRecord(ctx->ev0, s));
-  if (learned) decide(ctx, 0, true, (1u << BFSX_LLB) | (1u << BFSX_SLB));
+  // (the root level runs LLB, as in the host loop)
   const uint64_t max_launches = (uint64_t)ctx->n_cols / 2 + 2;
   for (uint64_t launches = 0;; launches++) {
     if (launches > max_launches) fail(BFSX_ERR_DEVICE, "captured level loop did not terminate");
@@ -372,10 +372,12 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   RunOut out;
   BFSX_HIP(hipEventRecord(ctx->ev0, s));
   uint64_t f_edges = 0, visited_v = 1, visited_e = 0, examined = 0;
-  if (learned) decide(ctx, 0, true, allow_mask(w));
-  w.decide_allow = learned ? allow_mask(w) : 0u;  // later levels decide in level_end
+  // The root level (one vertex) runs LLB, the choice k_init leaves in the
+  // control block, with no network evaluation or extra round trip; every
+  // later level is picked by the network in the previous level's end.
+  w.decide_allow = learned ? allow_mask(w) : 0u;
   w.decide_reuse = true;
-  if (trace || learned) {
+  if (trace) {
     read_ctl(ctx);
     f_edges = ctx->h_ctl->q_edges[0];
     visited_e = ctx->h_ctl->visited_edges;
