@@ -1367,7 +1367,8 @@ __global__ void k_level_begin(LevelCtl *ctl) {
 __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consumed,
                                                   uint32_t produced, int stamp, int kind,
                                                   const float *Wg, uint64_t n_cols, uint64_t nnz,
-                                                  uint64_t n_iso, uint32_t allow, int reuse) {
+                                                  uint64_t n_iso, uint32_t allow, int reuse,
+                                                  LevelCtl *mirror) {
   __shared__ int s_live;
   if (threadIdx.x == 0) {
     s_live = 0;
@@ -1408,12 +1409,27 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
       }
     }
   }
-  if (!Wg) return;  // no scheduler: uniform over the work-group
-  __syncthreads();
-  // the learned scheduler picks the next level's step in the same launch
-  // (the weights are staged only when the network runs: a level that keeps
-  // the last decision costs what a level without the scheduler does)
-  if (s_live) decide_wave(ctl, Wg, produced, n_cols, nnz, n_iso, allow, 0);
+  if (Wg) {  // uniform over the work-group
+    __syncthreads();
+    // the learned scheduler picks the next level's step in the same launch
+    // (the weights are staged only when the network runs: a level that
+    // keeps the last decision costs what a level without the scheduler does)
+    if (s_live) decide_wave(ctl, Wg, produced, n_cols, nnz, n_iso, allow, 0);
+  }
+  // The finished block into the host-mapped mirror (thread 0 made every
+  // write of this launch), the sequence number last: the host polls it
+  // instead of synchronising the stream.
+  if (mirror && threadIdx.x == 0) {
+    const uint32_t seq = ctl->seq + 1;
+    ctl->seq = seq;
+    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(ctl);
+    unsigned long long *dst = reinterpret_cast<unsigned long long *>(mirror);
+    constexpr uint32_t WORDS = sizeof(LevelCtl) / 8;
+    for (uint32_t i = 0; i + 1 < WORDS; i++) dst[i] = src[i];
+    __threadfence_system();
+    dst[WORDS - 1] = src[WORDS - 1];  // holds seq
+    __threadfence_system();
+  }
 }
 
 // sum of out-degree over reached vertices < n_rows, reached count, max level
@@ -1829,7 +1845,7 @@ void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, 
                      consumed, produced, stamp ? 1 : 0, kind,
                      decide_allow ? ctx->sched_w.as<float>() : (const float *)nullptr,
                      (uint64_t)ctx->n_cols, (uint64_t)ctx->nnz, n_iso, decide_allow,
-                     reuse ? 1 : 0);
+                     reuse ? 1 : 0, ctx->d_mirror);
   BFSX_HIP(hipGetLastError());
 }
 

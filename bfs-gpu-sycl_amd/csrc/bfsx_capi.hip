@@ -6,6 +6,8 @@
 // count, :713-715).  Device memory for the graph and all search state is
 // allocated once per graph and reused across runs.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -124,6 +126,24 @@ static void read_ctl(bfsx_ctx *ctx) {
   BFSX_HIP(hipMemcpyAsync(ctx->h_ctl, ctx->ctl.p, sizeof(LevelCtl), hipMemcpyDeviceToHost,
                           ctx->stream));
   BFSX_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// After a launch that ends with k_level_end: that kernel leaves the control
+// block in the host-mapped mirror, sequence number last, so the host polls
+// the number (no copy, no stream synchronisation).  After a second without
+// it, the stream is synchronised, which reports any device error.
+static void read_ctl_after_level(bfsx_ctx *ctx, uint32_t seq) {
+  const volatile uint32_t *p = &ctx->h_ctl->seq;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t spin = 0; *p != seq; spin++) {
+    if ((spin & 1023u) == 1023u &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
+      BFSX_HIP(hipStreamSynchronize(ctx->stream));
+      if (*p != seq) read_ctl(ctx);
+      break;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 // Top-down levels whose frontier has at least this many edges mark and
@@ -382,6 +402,7 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     f_edges = ctx->h_ctl->q_edges[0];
     visited_e = ctx->h_ctl->visited_edges;
   }
+  uint32_t seq = ctx->h_ctl->seq;  // the mirror holds the block k_init wrote
   while (w.F > 0) {
     // a search has at most n_cols levels; more means a broken invariant
     if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
@@ -390,7 +411,7 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
     const bool new_fbm = w.launch(choice);
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
-    read_ctl(ctx);
+    read_ctl_after_level(ctx, ++seq);
     w.advance(new_fbm);
     if (timed) {
       bfsx_level_record r{};
@@ -409,6 +430,7 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
         visited_e = ctx->h_ctl->visited_edges;
       }
       float us = 0, kus = 0;
+      BFSX_HIP(hipEventSynchronize(ctx->ev_step1));  // the mirror can arrive before it
       BFSX_HIP(hipEventElapsedTime(&us, ctx->ev_step0, ctx->ev_step1));
       BFSX_HIP(hipEventElapsedTime(&kus, ctx->ev_k0, ctx->ev_k1));
       r.us = us * 1000.0;
@@ -635,7 +657,9 @@ int bfsx_create(int device, bfsx_ctx **out) {
     BFSX_HIP(hipEventCreate(&ctx->ev_k0));
     BFSX_HIP(hipEventCreate(&ctx->ev_k1));
     ensure(ctx->ctl, sizeof(LevelCtl), ctx->stream);
-    BFSX_HIP(hipHostMalloc((void **)&ctx->h_ctl, sizeof(LevelCtl), hipHostMallocDefault));
+    BFSX_HIP(hipHostMalloc((void **)&ctx->h_ctl, sizeof(LevelCtl),
+                           hipHostMallocMapped | hipHostMallocCoherent));
+    BFSX_HIP(hipHostGetDevicePointer((void **)&ctx->d_mirror, ctx->h_ctl, 0));
     ctx->params = default_params();
   });
   if (rc != BFSX_OK) {

@@ -38,7 +38,9 @@ struct LevelCtl {
   // within 25 % of it, after the same kind of step, keeps that decision)
   unsigned long long dec_EF;
   uint32_t dec_F, dec_pull, dec_valid;
+  uint32_t seq;  // k_level_end launches so far (written last into the host mirror)
 };
+static_assert(sizeof(LevelCtl) % 8 == 0, "the level-end kernel copies the block as 8-byte words");
 
 // Bitmaps hold whole 1024-vertex chunks (32 words) so that a wave can own a
 // chunk; mark arrays hold one byte per vertex, padded the same way.
@@ -85,7 +87,8 @@ struct bfsx_ctx {
   bfsx::DevBuf fsum;     // bottom-up step: frontier summary bitmap (PULL_SUM_WORDS_MAX)
   bfsx::DevBuf bkt, bkt_cnt, bkt_tot, bkt_base;  // GLB bucketed discovery
   bfsx::DevBuf scratch;  // generic scratch (reductions, root picking)
-  bfsx::LevelCtl *h_ctl = nullptr;  // pinned mirror
+  bfsx::LevelCtl *h_ctl = nullptr;  // pinned, host-mapped mirror
+  bfsx::LevelCtl *d_mirror = nullptr;  // device address of h_ctl, written by k_level_end
   bool mark_dirty = false;          // a mark step did not reach its compaction
 
   // learned scheduler weights and the snapshot used to time alternatives
