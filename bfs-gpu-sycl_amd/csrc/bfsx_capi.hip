@@ -345,10 +345,14 @@ static RunOut search_graph_loop(bfsx_ctx *ctx, uint32_t root, int fixed, bool le
   BFSX_HIP(hipEventRecord(ctx->ev0, s));
   // (the root level runs LLB, as in the host loop)
   const uint64_t max_launches = (uint64_t)ctx->n_cols / 2 + 2;
+  // each launch runs K level ends, each bumping the mirror's sequence number
+  const uint32_t K = (ctx->params.graph_levels + 1) & ~1u;
+  uint32_t seq = ctx->h_ctl->seq;
   for (uint64_t launches = 0;; launches++) {
     if (launches > max_launches) fail(BFSX_ERR_DEVICE, "captured level loop did not terminate");
     BFSX_HIP(hipGraphLaunch(exec, s));
-    read_ctl(ctx);
+    seq += K;
+    read_ctl_after_level(ctx, seq);
     if (ctx->h_ctl->done) break;
   }
   BFSX_HIP(hipEventRecord(ctx->ev1, s));
