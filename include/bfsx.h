@@ -239,9 +239,13 @@ int bfsx_generate_rmat_union(bfsx_ctx *ctx, uint32_t n_graphs, const int *scales
 /* ---- learned per-level scheduler (no counterpart in the reference) ------
  * An FP32 MLP 8 -> 32 -> 32 -> 4 (ReLU) predicts log2(device microseconds)
  * of each step kind (LLB, GLB, SLB, PULL) from frontier statistics; with
- * strategy BFSX_LEARNED each level runs the kind with the smallest
- * prediction (PULL only on symmetric graphs).  Without weights, or if the
- * network returns NaN, BFSX_LEARNED runs fixed LLB.  Parameters are 1476
+ * strategy BFSX_LEARNED each level after the root level (one vertex, which
+ * runs LLB) runs the kind with the smallest prediction (PULL only on
+ * symmetric graphs); the decision is made on the device at the end of the
+ * previous level, and a level whose frontier is within 25 % of the one the
+ * last decision saw, after the same kind of step, keeps that decision.
+ * Without weights, or if the network returns NaN, BFSX_LEARNED runs fixed
+ * LLB.  Parameters are 1476
  * floats: W1[32][8], b1[32], W2[32][32], b2[32], W3[4][32], b3[4]. */
 #define BFSX_SCHED_PARAMS 1476
 /* Loads scheduler weights (n == BFSX_SCHED_PARAMS); w == NULL unloads. */
