@@ -76,3 +76,38 @@ def _has_gpu():
         return torch.cuda.device_count() > 0
     except Exception:
         return False
+
+
+def header_defines():
+    out = {}
+    for m in re.finditer(r"#define\s+(BFSX_\w+)\s+(\d+)u?\b", open(HEADER).read()):
+        out[m.group(1)] = int(m.group(2))
+    return out
+
+
+def test_binding_constants_match_the_header():
+    d = header_defines()
+    assert bfsx.GRAPH_HUB_FIRST == d["BFSX_GRAPH_HUB_FIRST"]
+    assert bfsx.GRAPH_PULL_COPY == d["BFSX_GRAPH_PULL_COPY"]
+    assert bfsx.STEP_NAMES.index("llb") == d["BFSX_LLB"]
+    assert bfsx.STEP_NAMES.index("pull") == d["BFSX_PULL"]
+    assert bfsx.SCHED_PARAMS == d["BFSX_SCHED_PARAMS"]
+    # the order names the binding passes to bfsx_order_adjacency
+    import inspect
+    src = inspect.getsource(bfsx.Engine.order_adjacency)
+    codes = dict(re.findall(r'"(\w+)": (\d+)', src))
+    assert int(codes["id"]) == d["BFSX_ORDER_ID"]
+    assert int(codes["hub_first"]) == d["BFSX_ORDER_HUB_FIRST"]
+    assert int(codes["pull_hub_first"]) == d["BFSX_ORDER_PULL_HUB_FIRST"]
+
+
+def test_bench_reads_the_committed_traffic_summaries():
+    sys.path.insert(0, ROOT)
+    import bench
+    t = bench.traffic_from_profiles("rmat26", "pull")
+    assert t is not None and 1e8 < t < 1e10
+    rl = bench.with_traffic({"step": "pull", "avg_launch_us": 200.0}, "rmat26")
+    assert rl["traffic"] == t
+    assert abs(rl["hbm_achieved"] - t / 200e-6 / 1e9) < 0.1
+    assert abs(rl["hbm_frac"] - rl["hbm_achieved"] / bench.HBM_PEAK_GBS) < 1e-3
+    assert bench.traffic_from_profiles("rmat26", "nonexistent") is None
