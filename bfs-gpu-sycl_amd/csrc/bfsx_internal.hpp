@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -41,6 +42,8 @@ struct LevelCtl {
   uint32_t seq;  // k_level_end launches so far (written last into the host mirror)
 };
 static_assert(sizeof(LevelCtl) % 8 == 0, "the level-end kernel copies the block as 8-byte words");
+static_assert(offsetof(LevelCtl, seq) >= sizeof(LevelCtl) - 8,
+              "seq lives in the last 8-byte word, which the level-end kernel writes last");
 
 // Bitmaps hold whole 1024-vertex chunks (32 words) so that a wave can own a
 // chunk; mark arrays hold one byte per vertex, padded the same way.
