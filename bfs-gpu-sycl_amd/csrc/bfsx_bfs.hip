@@ -1390,14 +1390,15 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
       ctl->done = n == 0;
       s_live = n != 0;
       if (s_live && Wg) {
-        // keep the last decision while the frontier stays within 25 % of
-        // the one it was computed for, after the same kind of step
+        // keep the last decision while the frontier (vertices and edges)
+        // stays within a factor 1.5 of the one it was computed for, after
+        // the same kind of step
         const unsigned long long ef = ctl->q_edges[produced];
         const uint32_t pull = k == BFSX_PULL;
         const uint32_t f0 = ctl->dec_F;
         const unsigned long long e0 = ctl->dec_EF;
-        const bool close = (uint64_t)n * 5 >= (uint64_t)f0 * 4 && (uint64_t)n * 4 <= (uint64_t)f0 * 5 &&
-                           ef * 5 >= e0 * 4 && ef * 4 <= e0 * 5;
+        const bool close = (uint64_t)n * 3 >= (uint64_t)f0 * 2 && (uint64_t)n * 2 <= (uint64_t)f0 * 3 &&
+                           ef * 3 >= e0 * 2 && ef * 2 <= e0 * 3;
         if (reuse && ctl->dec_valid && ctl->dec_pull == pull && close) {
           s_live = 0;
         } else {

@@ -36,7 +36,8 @@ struct LevelCtl {
   uint32_t conv;                   // slot cursor of a bitmap-to-queue conversion
   unsigned long long examined;     // adjacency entries read by bottom-up steps (running)
   // frontier of the last network evaluation (a level whose frontier is
-  // within 25 % of it, after the same kind of step, keeps that decision)
+  // within a factor 1.5 of it, after the same kind of step, keeps that
+  // decision)
   unsigned long long dec_EF;
   uint32_t dec_F, dec_pull, dec_valid;
   uint32_t seq;  // k_level_end launches so far (written last into the host mirror)

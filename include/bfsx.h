@@ -242,8 +242,9 @@ int bfsx_generate_rmat_union(bfsx_ctx *ctx, uint32_t n_graphs, const int *scales
  * strategy BFSX_LEARNED each level after the root level (one vertex, which
  * runs LLB) runs the kind with the smallest prediction (PULL only on
  * symmetric graphs); the decision is made on the device at the end of the
- * previous level, and a level whose frontier is within 25 % of the one the
- * last decision saw, after the same kind of step, keeps that decision.
+ * previous level, and a level whose frontier is within a factor 1.5 of the
+ * one the last decision saw, after the same kind of step, keeps that
+ * decision.
  * Without weights, or if the network returns NaN, BFSX_LEARNED runs fixed
  * LLB.  Parameters are 1476
  * floats: W1[32][8], b1[32], W2[32][32], b2[32], W3[4][32], b3[4]. */
