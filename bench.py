@@ -61,9 +61,10 @@ def algorithmic_bytes(F, E_F, D, w_ptr, L_b=4):
 def pull_bytes(n, UV, S, D, w_ptr, L_b=4):
     """Bottom-up useful bytes of one level: the visited bitmap read, the
     current frontier bitmap read once and the next one written (n/8 each),
-    col_ptr twice per unvisited candidate, one row_idx read per adjacency
-    entry examined, a level write per discovered vertex."""
-    return 3.0 * n / 8.0 + UV * 2.0 * w_ptr + 4.0 * S + D * L_b
+    col_ptr twice per unvisited candidate, one row_idx read and one frontier
+    probe (charged 1 byte, as the top-down model charges a visited probe) per
+    adjacency entry examined, a level write per discovered vertex."""
+    return 3.0 * n / 8.0 + UV * 2.0 * w_ptr + 5.0 * S + D * L_b
 
 
 KERNEL_NAMES = {"llb": "k_llb", "slb": "k_llb (strided)", "glb": "k_glb_expand + scan kernels",
@@ -75,13 +76,14 @@ def roofline_leg(eng, roots, strategy, info):
     the engine stream) and level histograms -> algorithmic bytes / time of
     the dominant step kind's kernel.
 
-    `achieved` charges every launch the top-down bytes of its level
-    (algorithmic_bytes), whatever the step kind: the fixed denominator that
-    makes GTEPS comparable across kinds, so a bottom-up launch, which skips
-    most edges, can exceed the HBM peak on it.  `achieved_own_model` charges
-    a bottom-up launch what it must read itself (pull_bytes, with the
-    adjacency entries it examined); with_traffic adds the measured HBM
-    throughput."""
+    `achieved` charges each launch the bytes its own algorithm must move:
+    algorithmic_bytes for a top-down step, pull_bytes (with the adjacency
+    entries the launch examined) for a bottom-up one; `frac` = achieved / HBM
+    peak.  `achieved_topdown_equiv` charges every launch the top-down bytes of
+    its level whatever the step kind (the fixed denominator that makes GTEPS
+    comparable across kinds): a bottom-up launch skips most of those edges,
+    so that figure can exceed the peak and is no bandwidth.  with_traffic
+    adds the measured HBM bytes per launch."""
     import bfsx
     w_ptr = info["col_ptr_bytes"]
     n = float(info["n_cols"])
@@ -114,24 +116,26 @@ def roofline_leg(eng, roots, strategy, info):
             per_kind_step_us[k] += rec["us"]
             per_kind_launches[k] += 1
     k = int(np.argmax(per_kind_us))
-    achieved = per_kind_bytes[k] / (per_kind_us[k] * 1e-6) / 1e9
     own = per_kind_own[k] / (per_kind_us[k] * 1e-6) / 1e9
+    td = per_kind_bytes[k] / (per_kind_us[k] * 1e-6) / 1e9
     name = bfsx.STEP_NAMES[k]
+    launches = max(per_kind_launches[k], 1)
     return {
         "kernel": KERNEL_NAMES[name],
         "step": name,
         "bound": "hbm",
-        "achieved": round(achieved, 1),
+        "achieved": round(own, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "frac": round(own / HBM_PEAK_GBS, 4),
         "traffic": None,
         "launches": int(per_kind_launches[k]),
-        "avg_launch_us": round(per_kind_us[k] / max(per_kind_launches[k], 1), 2),
-        "avg_step_us": round(per_kind_step_us[k] / max(per_kind_launches[k], 1), 2),
-        "alg_bytes_per_launch": round(per_kind_bytes[k] / max(per_kind_launches[k], 1), 1),
-        "achieved_own_model": round(own, 1),
-        "own_bytes_per_launch": round(per_kind_own[k] / max(per_kind_launches[k], 1), 1),
+        "avg_launch_us": round(per_kind_us[k] / launches, 2),
+        "avg_step_us": round(per_kind_step_us[k] / launches, 2),
+        "alg_bytes_per_launch": round(per_kind_own[k] / launches, 1),
+        "achieved_topdown_equiv": round(td, 1),
+        "topdown_bytes_per_launch": round(per_kind_bytes[k] / launches, 1),
+        "kind_time_share": round(float(per_kind_us[k] / per_kind_us.sum()), 4),
     }
 
 
@@ -141,8 +145,9 @@ def with_traffic(rl, workload):
     rl["traffic"] = traffic_from_profiles(workload, rl["step"])
     if rl["traffic"]:
         hbm = rl["traffic"] / (rl["avg_launch_us"] * 1e-6) / 1e9
-        rl["hbm_achieved"] = round(hbm, 1)
-        rl["hbm_frac"] = round(hbm / HBM_PEAK_GBS, 4)
+        rl["traffic_over_alg"] = round(rl["traffic"] / rl["alg_bytes_per_launch"], 2)
+        rl["fabric_achieved"] = round(hbm, 1)
+        rl["fabric_frac"] = round(hbm / HBM_PEAK_GBS, 4)
     return rl
 
 
@@ -163,38 +168,112 @@ def traffic_from_profiles(workload, step):
     return None
 
 
-def cpu_baseline(eng, roots, gpu_levels_of, budget_s=30.0):
-    """The reference's run_bfs_cpu.hpp (oracle/_ref) or the oracle port, on
-    one core, over as many roots as fit a bounded budget (at least one)."""
+def host_cpu():
+    """The host the CPU legs run on: logical CPUs, the ones this process may
+    use, and the model name from /proc/cpuinfo."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)),
+            "model": model}
+
+
+class one_core:
+    """Pins the calling thread (the one that runs the CPU BFS through ctypes)
+    to a single CPU for the duration: the reference's CPU BFS is
+    single-threaded (its OpenMP is disabled, src/run_bfs_gpu.hpp:14-15)."""
+
+    def __enter__(self):
+        self.old = os.sched_getaffinity(0)
+        self.cpu = max(self.old)
+        os.sched_setaffinity(0, {self.cpu})
+        return self
+
+    def __exit__(self, *a):
+        os.sched_setaffinity(0, self.old)
+
+
+def cpu_bfs_times(cp, ri, roots, gpu_levels_of, budget_s=None):
+    """The reference's run_bfs_cpu.hpp (oracle/_ref) or, where that is not
+    built, the oracle's restatement of it, on one pinned core: per root
+    (edgesCount, seconds).  The GPU levels of every root timed are compared
+    bit-exact.  Stops after budget_s seconds (at least one root)."""
     from oracle import cpu
-    cp, ri = eng.download_csc()
     kind = "reference" if cpu.ref_available() else "port"
     ref = cpu.RefGraph(cp, ri) if kind == "reference" else None
-    edges, secs, checked = [], [], 0
+    edges, secs = [], []
     t_start = time.time()
-    for r in roots:
-        if kind == "reference":
-            lev, ec, ms = ref.run(int(r))
-        else:
-            t0 = time.perf_counter()
-            lev, ec, _ = cpu.bfs_levelsync(cp, ri, int(r))
-            ms = (time.perf_counter() - t0) * 1e3
-        want = gpu_levels_of(int(r))
-        if not np.array_equal(lev, want):
-            raise AssertionError(f"GPU levels differ from the CPU BFS at root {r}")
-        checked += 1
-        edges.append(ec)
-        secs.append(ms * 1e-3)
-        log(f"cpu baseline root {r}: {ms:.0f} ms, {ec / (ms * 1e-3) / 1e9:.4f} GTEPS")
-        if time.time() - t_start > budget_s:
-            break
+    with one_core() as pin:
+        for r in roots:
+            if kind == "reference":
+                lev, ec, ms = ref.run(int(r))
+            else:
+                t0 = time.perf_counter()
+                lev, ec, _ = cpu.bfs_levelsync(cp, ri, int(r))
+                ms = (time.perf_counter() - t0) * 1e3
+            want = gpu_levels_of(int(r))
+            if not np.array_equal(lev, want):
+                raise AssertionError(f"GPU levels differ from the CPU BFS at root {r}")
+            edges.append(ec)
+            secs.append(ms * 1e-3)
+            if budget_s is not None and time.time() - t_start > budget_s:
+                break
+        core = pin.cpu
     if ref is not None:
         ref.close()
-    value = len(edges) / sum(s / e for s, e in zip(secs, edges)) / 1e9
+    return kind, core, np.array(edges, np.float64), np.array(secs, np.float64)
+
+
+def cpu_baseline(eng, roots, gpu_levels_of, budget_s=30.0):
+    """CPU BFS on a bounded sample of the headline workload's roots."""
+    cp, ri = eng.download_csc()
+    kind, core, edges, secs = cpu_bfs_times(cp, ri, roots, gpu_levels_of, budget_s)
+    for e, t in zip(edges, secs):
+        log(f"cpu baseline: {t * 1e3:.0f} ms, {e / t / 1e9:.4f} GTEPS")
+    value = len(edges) / float(np.sum(secs / edges)) / 1e9
     return {"value": round(value, 5), "unit": "GTEPS", "cores": 1, "kind": kind,
-            "sample": f"{len(edges)} of {len(roots)} roots, single-threaded run_bfs_cpu, "
-                      f"harmonic mean; GPU levels bit-exact on these roots",
-            "roots_checked": checked}
+            "sample": f"{len(edges)} of {len(roots)} roots (about {budget_s:.0f} s of CPU work), "
+                      f"single-threaded run_bfs_cpu pinned to CPU {core}, harmonic mean; "
+                      f"GPU levels bit-exact on these roots",
+            "roots_checked": int(len(edges)), "host": host_cpu()}
+
+
+def small_configs(eng, names=("rmat16", "rmat19"), runs=3):
+    """R-MAT scales 16 and 19 (the CPU reference's case and the first GPU
+    case), all 64 roots: harmonic GTEPS of every strategy on the GPU (first
+    10 % of runs dropped) and of the CPU BFS on one core; the GPU levels of
+    every root are compared with the CPU's."""
+    import bfsx
+    import workloads
+    out = {}
+    for name in names:
+        w, nnz = workloads.build(eng, name)
+        roots = workloads.roots(eng, w)
+        cp, ri = eng.download_csc()
+        res = {"vertices": int(len(cp) - 1), "nnz": int(nnz), "roots": int(len(roots)), "gpu": {}}
+        for s in ("llb", "glb", "slb", "pull", "learned", "beamer"):
+            if s == "learned":
+                if not bfsx.have_scheduler_weights():
+                    continue
+                eng.load_scheduler()
+            sweep(eng, roots, s, 1)
+            r_runs, st = sweep(eng, roots, s, runs)
+            e = np.array([x["edges_count"] for x in st], np.float64)
+            res["gpu"][s] = round(float(bfsx.harmonic_gteps(e, per_root_times(r_runs))), 3)
+        eng.set_scheduler_weights(None)
+        kind, core, edges, secs = cpu_bfs_times(cp, ri, roots,
+                                                lambda r: eng.run(r, "llb")[0])
+        cpu_g = len(edges) / float(np.sum(secs / edges)) / 1e9
+        res["cpu"] = {"value": round(cpu_g, 5), "unit": "GTEPS", "cores": 1, "kind": kind,
+                      "roots": int(len(edges)), "cpu_ms_total": round(float(secs.sum()) * 1e3, 1)}
+        log(f"{name}: gpu {res['gpu']} cpu {cpu_g:.4f} GTEPS over {len(edges)} roots")
+        out[name] = res
+    return out
 
 
 def bench_batch(eng, args):
@@ -233,6 +312,16 @@ def bench_batch(eng, args):
     }
 
 
+def compare_sweep(eng, roots, strategy, n):
+    """Harmonic GTEPS of `strategy` on the same roots (one warm-up sweep, n
+    timed sweeps, first 10 % dropped per root)."""
+    import bfsx
+    sweep(eng, roots, strategy, 1)
+    runs, st = sweep(eng, roots, strategy, n)
+    e = np.array([x["edges_count"] for x in st], np.float64)
+    return float(bfsx.harmonic_gteps(e, per_root_times(runs))), st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,9 +331,15 @@ def main():
     ap.add_argument("--strategy", default="auto",
                     help="headline strategy: auto (learned when trained weights exist, else llb)")
     ap.add_argument("--roots", type=int, default=0, help="limit the number of roots (0 = all)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the R-MAT 16 / 19 legs (GPU strategies and CPU on all 64 roots)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuned fixed-LLB leg")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--compare-steps", type=int, default=3,
+                    help="timed sweeps of each comparison strategy (fixed LLB, tuned LLB, "
+                         "direction-optimising rule)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the streaming copy probe over this many bytes")
     args = ap.parse_args()
@@ -274,6 +369,15 @@ def main():
               flush=True)
         eng.close()
         return
+    # achievable streaming bandwidth on this GPU, unprofiled: a 1 GiB copy
+    copy_bytes = 1 << 30
+    copy_ms = eng.stream_copy(copy_bytes, repeats=20)
+    copy_probe = {"kernel": "k_stream_copy", "bytes": copy_bytes, "ms": round(copy_ms, 4),
+                  "GBps_read_plus_write": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1)}
+    log("copy probe", copy_probe)
+    small = None
+    if not args.no_small and args.workload == "rmat26":
+        small = small_configs(eng) if not args.no_cpu else None
     t0 = time.time()
     w, nnz = workloads.build(eng, args.workload)
     info = eng.info()
@@ -319,15 +423,30 @@ def main():
                    "col_ptr_bytes": info["col_ptr_bytes"], "roots": len(roots),
                    "strategy": strategy, "adjacency_order": w["order"],
                    "mean_edges_per_root": float(edges.mean())},
+        "graph500_gteps": round(float(value) / 2, 3),
+        "copy_probe": copy_probe,
     }
     if strategy != "llb":
-        sweep(eng, roots, "llb", args.warmup)
-        runs_llb, st_llb = sweep(eng, roots, "llb", args.steps)
-        e_llb = np.array([s["edges_count"] for s in st_llb], np.float64)
-        llb = bfsx.harmonic_gteps(e_llb, per_root_times(runs_llb))
-        result["fixed_llb_gteps"] = round(float(llb), 3)
-        result["speedup_vs_fixed_llb"] = round(float(value / llb), 3)
+        n = args.compare_steps
+        llb, _ = compare_sweep(eng, roots, "llb", n)
+        result["fixed_llb_gteps"] = round(llb, 3)
+        judge = llb
+        if not args.no_tune:
+            # the judge with its launch shape tuned on 8 of the roots
+            default = eng.params
+            tuned, before, after = eng.autotune(roots[:8], "llb", passes=1)
+            llb_t, _ = compare_sweep(eng, roots, "llb", n)
+            eng.set_params(**default)
+            result["fixed_llb_tuned"] = {"gteps": round(llb_t, 3), "params": tuned,
+                                         "tune_roots": 8}
+            judge = max(llb, llb_t)
+        result["speedup_vs_fixed_llb"] = round(float(value / judge), 3)
+        beam, st_b = compare_sweep(eng, roots, "beamer", n)
+        result["beamer_gteps"] = round(beam, 3)
+        result["speedup_vs_beamer"] = round(float(value / beam), 3)
         result["steps_used"] = {k: int(sum(s["steps"][k] for s in stats)) for k in bfsx.STEP_NAMES}
+        result["steps_used_beamer"] = {k: int(sum(s["steps"][k] for s in st_b))
+                                       for k in bfsx.STEP_NAMES}
         if sched_meta is not None:
             result["scheduler"] = {"train_workloads": sched_meta["train_workloads"],
                                    "rows": sched_meta["rows"],
@@ -338,16 +457,15 @@ def main():
             # the top-down explore kernel on its own (fixed LLB run)
             result["explore_roofline_llb"] = with_traffic(roofline_leg(eng, roots, "llb", info),
                                                           args.workload)
-        result["graph500_gteps"] = round(float(value) / 2, 3)
         # whole searches against the top-down algorithmic bytes (the same
-        # denominator for every step kind, like TEPS)
+        # denominator for every step kind, like TEPS): above 1 means faster
+        # than a top-down search streaming at the HBM peak could be
         reached = np.array([s["reached"] for s in stats], np.float64)
         b_alg = 5.0 * edges + reached * (2.0 * info["col_ptr_bytes"] + 8.0 + 4.0)
         ach = float(b_alg.sum() / (t_root.sum() * 1e-3) / 1e9)
-        result["search_roofline"] = {"bound": "hbm", "achieved": round(ach, 1),
-                                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                     "frac": round(ach / HBM_PEAK_GBS, 4),
-                                     "bytes_per_root": round(float(b_alg.mean()), 1)}
+        result["search_topdown_equiv"] = {"achieved": round(ach, 1), "unit": "GB/s",
+                                          "ratio_to_peak": round(ach / HBM_PEAK_GBS, 4),
+                                          "bytes_per_root": round(float(b_alg.mean()), 1)}
     if not args.no_cpu:
         lev_cache = {}
 
@@ -357,6 +475,8 @@ def main():
             return lev_cache[r]
 
         result["cpu_baseline"] = cpu_baseline(eng, roots, gpu_levels_of, args.cpu_budget)
+    if small is not None:
+        result["small_configs"] = small
     eng.close()
     print(json.dumps(result), flush=True)
 

@@ -18,9 +18,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbfsx.so")
 
-LLB, GLB, SLB, PULL, LEARNED = 0, 1, 2, 3, 4
+LLB, GLB, SLB, PULL, LEARNED, BEAMER = 0, 1, 2, 3, 4, 5
 STRATEGIES = {"llb": LLB, "local": LLB, "glb": GLB, "global": GLB, "slb": SLB,
-              "local_strided": SLB, "pull": PULL, "learned": LEARNED}
+              "local_strided": SLB, "pull": PULL, "learned": LEARNED, "beamer": BEAMER}
 STEP_NAMES = ["llb", "glb", "slb", "pull"]
 GRAPH_SYMMETRIC = 1
 GRAPH_HUB_FIRST = 2
@@ -62,7 +62,9 @@ class LevelRecord(C.Structure):
                 ("frontier_vertices", C.c_uint64), ("frontier_edges", C.c_uint64),
                 ("unvisited_vertices", C.c_uint64), ("unvisited_edges", C.c_uint64),
                 ("discovered", C.c_uint64), ("us", C.c_double),
-                ("pull_scanned", C.c_uint64), ("kernel_us", C.c_double)]
+                ("pull_scanned", C.c_uint64), ("kernel_us", C.c_double),
+                ("sched_evaluated", C.c_uint32), ("sched_in", C.c_float * 8),
+                ("sched_out", C.c_float * 4)]
 
 
 class Params(C.Structure):
@@ -274,7 +276,10 @@ class Engine:
         self._check(lib().bfsx_last_trace(self._h, None, 0, C.byref(n)))
         recs = (LevelRecord * max(n.value, 1))()
         self._check(lib().bfsx_last_trace(self._h, C.cast(recs, P), n.value, C.byref(n)))
-        return [{k: getattr(recs[i], k) for k, _ in LevelRecord._fields_} for i in range(n.value)]
+        def val(v):
+            return list(v) if isinstance(v, C.Array) else v
+        return [{k: val(getattr(recs[i], k)) for k, _ in LevelRecord._fields_}
+                for i in range(n.value)]
 
     def level_histogram(self, max_levels=1024):
         """(counts, edges) per level of the last run; grows the table if needed."""
@@ -407,6 +412,28 @@ def load_scheduler_file(path=None):
     if len(meta["weights"]) != SCHED_PARAMS:
         raise ValueError("scheduler file holds the wrong number of parameters")
     return meta
+
+
+# Layer shapes of the scheduler network, in parameter order (row-major):
+# W1[32][8], b1[32], W2[32][32], b2[32], W3[4][32], b3[4].
+SCHED_LAYERS = [("W1", (32, 8)), ("b1", (32,)), ("W2", (32, 32)), ("b2", (32,)),
+                ("W3", (4, 32)), ("b3", (4,))]
+
+
+def sched_init_weights(seed):
+    """Seeded starting point for scheduler training: Kaiming-uniform weights
+    (bound sqrt(6 / fan_in)), zero biases, float32 in the engine's layout."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, shape in SCHED_LAYERS:
+        if name.startswith("W"):
+            bound = np.sqrt(6.0 / shape[1])
+            parts.append(rng.uniform(-bound, bound, shape).ravel())
+        else:
+            parts.append(np.zeros(shape).ravel())
+    w = np.concatenate(parts).astype(np.float32)
+    assert w.size == SCHED_PARAMS
+    return w
 
 
 def harmonic_gteps(edges, ms):

@@ -367,18 +367,42 @@ static RunOut search_graph_loop(bfsx_ctx *ctx, uint32_t root, int fixed, bool le
   return out;
 }
 
+// Beamer, Asanovic and Patterson, "Direction-Optimizing Breadth-First
+// Search" (SC'12), in the form of their GAP benchmark: switch to bottom-up
+// when the frontier's edges exceed the unexplored edges / alpha; stay
+// bottom-up while the frontier grows or holds more than n / beta vertices.
+struct BeamerRule {
+  static constexpr uint64_t ALPHA = 14, BETA = 24;
+  bool bottom_up = false;
+  uint64_t prev_F = 0;
+  int next(const bfsx_ctx *ctx, bool pull_possible, uint64_t F, uint64_t EF, uint64_t VE) {
+    const uint64_t unexplored = ctx->nnz > VE ? ctx->nnz - VE : 0;
+    if (!bottom_up) {
+      bottom_up = pull_possible && EF > unexplored / ALPHA;
+    } else if (F < prev_F && F <= (uint64_t)ctx->n_cols / BETA) {
+      bottom_up = false;
+    }
+    prev_F = F;
+    if (bottom_up) return BFSX_PULL;
+    return EF * 256 >= ctx->n_cols ? BFSX_GLB : BFSX_LLB;
+  }
+};
+
 static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
                      bool want_parents) {
-  if (strategy < BFSX_LLB || strategy > BFSX_LEARNED) fail(BFSX_ERR_ARG, "unknown strategy");
+  if (strategy < BFSX_LLB || strategy > BFSX_BEAMER) fail(BFSX_ERR_ARG, "unknown strategy");
   const bool symmetric = ctx->has_graph && graph_symmetric(ctx);
   if (strategy == BFSX_PULL && !symmetric)
     fail(BFSX_ERR_UNSUPPORTED, "bottom-up steps need a symmetric graph with n_rows == n_cols");
   const bool learned = strategy == BFSX_LEARNED && ctx->sched_loaded;
-  const int fixed = strategy == BFSX_LEARNED ? BFSX_LLB : strategy;  // fallback: fixed LLB
+  const bool beamer = strategy == BFSX_BEAMER;
+  // fallback of the learned strategy without weights: fixed LLB
+  const int fixed = (strategy == BFSX_LEARNED || beamer) ? BFSX_LLB : strategy;
   const bool trace = flags & BFSX_RUN_TRACE;
   const bool timed = trace || (flags & BFSX_RUN_STEP_TIMES);
   hipStream_t s = ctx->stream;
   if (flags & BFSX_RUN_GRAPH_LOOP) {
+    if (beamer) fail(BFSX_ERR_UNSUPPORTED, "the direction-optimising rule runs in the host loop");
     if (fixed != BFSX_LLB && fixed != BFSX_SLB)
       fail(BFSX_ERR_UNSUPPORTED, "the captured level loop runs LLB, SLB or the learned choice between them");
     if (timed || want_parents)
@@ -388,7 +412,9 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   if (want_parents) ensure(ctx->parents, sizeof(uint32_t) * ctx->n_cols, s);
   uint32_t *par = want_parents ? ctx->parents.as<uint32_t>() : nullptr;
   // initial state outside the timer (host.cpp:113-121 / run_bfs_gpu.hpp:674-677)
-  Walker w = start_search(ctx, root, fixed == BFSX_PULL || (learned && symmetric), false, par);
+  Walker w = start_search(ctx, root, fixed == BFSX_PULL || ((learned || beamer) && symmetric),
+                          false, par);
+  BeamerRule rule;
   w.count = timed;
   BFSX_HIP(hipStreamSynchronize(s));
   ctx->trace.clear();
@@ -407,18 +433,28 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     visited_e = ctx->h_ctl->visited_edges;
   }
   uint32_t seq = ctx->h_ctl->seq;  // the mirror holds the block k_init wrote
+  uint32_t decisions = ctx->h_ctl->decisions;
   while (w.F > 0) {
     // a search has at most n_cols levels; more means a broken invariant
     if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
-    const int choice = learned ? (int)ctx->h_ctl->choice : fixed;
+    const int choice = learned  ? (int)ctx->h_ctl->choice
+                       : beamer ? rule.next(ctx, w.pull_possible, w.F, w.EF, w.VE)
+                                : fixed;
     const uint32_t level = w.level, F = w.F, cur = w.cur;
+    // the decision in force for this level (made at the previous level's end)
+    bfsx_level_record r{};
+    if (trace && learned) {
+      r.sched_evaluated = ctx->h_ctl->decisions != decisions ? 1u : 0u;
+      decisions = ctx->h_ctl->decisions;
+      std::memcpy(r.sched_in, ctx->h_ctl->sched_in, sizeof(r.sched_in));
+      std::memcpy(r.sched_out, ctx->h_ctl->sched_out, sizeof(r.sched_out));
+    }
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
     const bool new_fbm = w.launch(choice);
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
     read_ctl_after_level(ctx, ++seq);
     w.advance(new_fbm);
     if (timed) {
-      bfsx_level_record r{};
       r.level = level;
       r.choice = (uint32_t)choice;
       r.frontier_vertices = F;
@@ -875,8 +911,22 @@ int bfsx_pick_roots(bfsx_ctx *ctx, uint32_t n_roots, uint64_t seed, int mode, ui
   return guarded(ctx, [&] {
     set_device(ctx);
     if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
-    if (mode != 0 && mode != 1) fail(BFSX_ERR_ARG, "mode must be 0 (degree >= 1) or 1 (largest component)");
+    if (mode < 0 || mode > 2)
+      fail(BFSX_ERR_ARG, "mode must be 0 (degree >= 1), 1 (largest component) or 2 (top degree)");
     hipStream_t s = ctx->stream;
+    if (mode == 2) {
+      const uint32_t k = std::min(n_roots, ctx->n_rows);
+      DevBuf dv;
+      ensure(dv, sizeof(uint32_t) * std::max(k, 1u), s);
+      top_degree_vertices(ctx, k, dv.as<uint32_t>());
+      if (k) {
+        BFSX_HIP(hipMemcpyAsync(roots_out, dv.p, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, s));
+        BFSX_HIP(hipStreamSynchronize(s));
+      }
+      release(dv);
+      if (n_found) *n_found = k;
+      return;
+    }
     const int32_t *comp = nullptr;
     std::vector<int32_t> keep_levels;
     if (mode == 1) {
@@ -1152,7 +1202,7 @@ int bfsx_autotune(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, int st
   return guarded(ctx, [&] {
     set_device(ctx);
     if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
-    if (strategy < BFSX_LLB || strategy > BFSX_LEARNED) fail(BFSX_ERR_ARG, "unknown strategy");
+    if (strategy < BFSX_LLB || strategy > BFSX_BEAMER) fail(BFSX_ERR_ARG, "unknown strategy");
     if (passes < 1 || passes > 4) fail(BFSX_ERR_ARG, "passes must be in [1,4]");
     std::vector<double> edges(n_roots);
     for (uint32_t i = 0; i < n_roots; i++) {
@@ -1177,7 +1227,9 @@ int bfsx_autotune(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, int st
         {&bfsx_params::glb_chunk, {1024, 2048, 4096, 8192, 16384}, G},
         {&bfsx_params::pull_wg, {64, 128, 256, 512}, P},
     };
-    const uint32_t used = strategy == BFSX_LEARNED ? (L | G | S | P) : (1u << strategy);
+    const uint32_t used = strategy == BFSX_LEARNED  ? (L | G | S | P)
+                          : strategy == BFSX_BEAMER ? (L | G | P)
+                                                    : (1u << strategy);
     bfsx_params best_p = ctx->params;
     double best_s = tune_score(ctx, roots, n_roots, strategy, edges);
     const double start_s = best_s;

@@ -535,4 +535,50 @@ void root_candidates(bfsx_ctx *ctx, uint64_t seed, uint64_t first, uint32_t coun
   BFSX_HIP(hipGetLastError());
 }
 
+// Top-k by degree (scripts/rootNodes.py:133-137: sorted(G.degree, reverse)
+// [:k], a stable sort, so equal degrees keep ascending ids): one 64-bit key
+// per vertex < n_rows, (~degree << 32 | id) for degrees below 2^32; an
+// ascending radix sort puts the largest degrees first and, among equal
+// degrees, the smaller id first.
+template <typename PtrT>
+__global__ void k_degree_rank_keys(const PtrT *col_ptr, uint32_t n_rows, uint64_t *keys) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n_rows;
+       v += gridDim.x * blockDim.x) {
+    const uint64_t d = (uint64_t)(col_ptr[v + 1] - col_ptr[v]);
+    const uint32_t dc = d > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)d;
+    keys[v] = ((uint64_t)(~dc) << 32) | v;
+  }
+}
+
+__global__ void k_low_words(const uint64_t *keys, uint32_t k, uint32_t *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) out[i] = (uint32_t)keys[i];
+}
+
+void top_degree_vertices(bfsx_ctx *ctx, uint32_t k, uint32_t *d_out) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = ctx->n_rows;
+  if (k > n) k = n;
+  if (k == 0) return;
+  DevBuf keys, sorted, tmp;
+  ensure(keys, sizeof(uint64_t) * n, s);
+  ensure(sorted, sizeof(uint64_t) * n, s);
+  if (ctx->ptr_bytes == 8)
+    hipLaunchKernelGGL(k_degree_rank_keys<uint64_t>, dim3(grid_for(n)), dim3(256), 0, s,
+                       ctx->col_ptr.as<uint64_t>(), n, keys.as<uint64_t>());
+  else
+    hipLaunchKernelGGL(k_degree_rank_keys<uint32_t>, dim3(grid_for(n)), dim3(256), 0, s,
+                       ctx->col_ptr.as<uint32_t>(), n, keys.as<uint64_t>());
+  size_t tbytes = 0;
+  BFSX_HIP(rocprim::radix_sort_keys(nullptr, tbytes, keys.as<uint64_t>(), sorted.as<uint64_t>(),
+                                    (size_t)n, 0, 64, s));
+  ensure(tmp, tbytes, s);
+  BFSX_HIP(rocprim::radix_sort_keys(tmp.p, tbytes, keys.as<uint64_t>(), sorted.as<uint64_t>(),
+                                    (size_t)n, 0, 64, s));
+  hipLaunchKernelGGL(k_low_words, dim3(grid_for(k)), dim3(256), 0, s, sorted.as<uint64_t>(), k,
+                     d_out);
+  BFSX_HIP(hipGetLastError());
+  BFSX_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace bfsx

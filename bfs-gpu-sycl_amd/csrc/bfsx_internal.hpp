@@ -40,6 +40,8 @@ struct LevelCtl {
   // decision)
   unsigned long long dec_EF;
   uint32_t dec_F, dec_pull, dec_valid;
+  uint32_t decisions;  // network evaluations so far (sched_in/out hold the last)
+  uint32_t reserved;
   uint32_t seq;  // k_level_end launches so far (written last into the host mirror)
 };
 static_assert(sizeof(LevelCtl) % 8 == 0, "the level-end kernel copies the block as 8-byte words");
@@ -147,6 +149,8 @@ void order_adjacency(bfsx_ctx *ctx, bool hub_first, uint32_t *out);
 // fd[v] = {row_idx[col_ptr[v]] (~0u without neighbours), degree} for v < n_rows
 // (degrees below 2^32).
 void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint2 *fd);
+// d_out[0..min(k, n_rows)) = vertices of largest out-degree, ties by id
+void top_degree_vertices(bfsx_ctx *ctx, uint32_t k, uint32_t *d_out);
 void root_candidates(bfsx_ctx *ctx, uint64_t seed, uint64_t first,
                      uint32_t count, const int32_t *d_component_levels,
                      uint32_t *d_out_vertex, uint8_t *d_out_ok);

@@ -91,6 +91,7 @@ __device__ __forceinline__ void decide_compute(LevelCtl *ctl, const float4 *W4, 
     }
     for (int k = 0; k < IN; k++) ctl->sched_in[k] = x[k];
     ctl->choice = ok && bv < INFINITY ? best : (uint32_t)BFSX_LLB;
+    ctl->decisions += 1;
   }
 }
 

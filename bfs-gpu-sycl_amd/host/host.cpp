@@ -7,7 +7,7 @@
 // the levels, prints the level histogram and merges the same JSON record as
 // the reference under "<dataset>_<num_runs>_<strategy>" -> "1".
 //
-// Additions (all optional): --strategy local|global|local_strided|pull|learned
+// Additions (all optional): --strategy local|global|local_strided|pull|learned|beamer
 // (the reference fixes it at compile time), --partitions P (load a
 // P-partition dataset onto the one GPU), --generate rmat:S:E[:seed] or
 // grid:SIDE[:seed] (build a graph in HBM instead of loading one),
@@ -17,7 +17,9 @@
 // Validation: the reference re-runs the search on the CPU and compares
 // (host.cpp:172-188).  Here the levels are checked against the BFS
 // certificate in validate.hpp (exactly equivalent: it holds iff the levels
-// are the hop distances), and every run must equal the last one.
+// are the hop distances), and every run must equal the last one.  A failed
+// check exits with status 1 after writing the record (the reference exits
+// 0 either way).
 #include <algorithm>
 #include <climits>
 #include <fstream>
@@ -51,6 +53,7 @@ const char *strategy_key(int s) {
     case BFSX_SLB: return "local_strided";
     case BFSX_PULL: return "pull";
     case BFSX_LEARNED: return "learned";
+    case BFSX_BEAMER: return "beamer";
     default: return "local";
   }
 }
@@ -61,6 +64,7 @@ int strategy_from(const std::string &s) {
   if (s == "local_strided" || s == "slb") return BFSX_SLB;
   if (s == "pull") return BFSX_PULL;
   if (s == "learned") return BFSX_LEARNED;
+  if (s == "beamer") return BFSX_BEAMER;
   throw std::runtime_error("unknown --strategy " + s);
 }
 
@@ -313,7 +317,9 @@ int main(int argc, char *argv[]) {
     std::ofstream out(output_json_name);
     if (out.is_open()) out << combined.dump(4);
     else std::cerr << "Error opening file for writing." << std::endl;
-    return 0;
+    // the record is written either way; unlike the reference (which returns
+    // 0 after [FAILURE]), a failed check is also the exit status
+    return status ? 0 : 1;
   } catch (const std::exception &ex) {
     std::cerr << "[ERROR] " << ex.what() << std::endl;
     return 1;

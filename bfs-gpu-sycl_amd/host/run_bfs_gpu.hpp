@@ -76,6 +76,42 @@ struct CtxGuard {
   ~CtxGuard() { bfsx_destroy(ctx); }
 };
 
+// The 7 timing keys of a GPURun record from the per-run level-loop times
+// (src/run_bfs_gpu.hpp:756-790): the mean, the runs within 1.5 standard
+// deviations of it (removeAnomalies) and the runs after the first 10 %
+// (removeAnomaliesnofilter), each with its mean and minimum, and the total;
+// the same summary table is printed.
+template <typename JsonT>
+void timing_record(const std::vector<double> &run_times, JsonT &newJsonObj) {
+  const double threshold = 1.5;  // run_bfs_gpu.hpp:757
+  const std::vector<double> filtered = removeAnomalies(run_times, threshold);
+  const std::vector<double> filtered90 = removeAnomaliesnofilter(run_times);
+  auto mean = [](const std::vector<double> &v) {
+    return v.empty() ? 0.0 : std::accumulate(v.begin(), v.end(), 0.0) / v.size();
+  };
+  auto minv = [](const std::vector<double> &v) {
+    double m = v.empty() ? 0.0 : v[0];
+    for (double x : v) m = x < m ? x : m;
+    return m;
+  };
+  const double total = std::accumulate(run_times.begin(), run_times.end(), 0.0);
+  const double n = run_times.empty() ? 1.0 : (double)run_times.size();
+  std::cout << "\n----------------------------------------" << std::endl;
+  printRow("Average (filtered) Time:", formatDouble(mean(filtered)) + " (ms)");
+  printRow("Minimum (filtered) Time:", formatDouble(minv(filtered)) + " (ms)");
+  printRow("Average (90%) Time:", formatDouble(mean(filtered90)) + " (ms)");
+  printRow("Minimum (90%) Time:", formatDouble(minv(filtered90)) + " (ms)");
+  printRow("Average Execution Time:", formatDouble(total / n) + " (ms)");
+  printRow("Total Execution Time:", formatDouble(total) + " (ms)");
+  newJsonObj["rawExecutionTimeAll"] = run_times;
+  newJsonObj["avgExecutionTime"] = total / n;
+  newJsonObj["avgExecutionTimeFiltered"] = mean(filtered);
+  newJsonObj["minExecutionTimeFiltered"] = minv(filtered);
+  newJsonObj["avgExecutionTime90f"] = mean(filtered90);
+  newJsonObj["minExecutionTime90f"] = minv(filtered90);
+  newJsonObj["totalExecutionTime"] = total;
+}
+
 template <typename vectorT, typename JsonT>
 void GPURun(uint32_t vertexCount, vectorT &IndexHost, vectorT &OffsetHost,
             std::vector<uint8_t> &h_visit_mask, std::vector<uint8_t> &h_visit,
@@ -111,7 +147,8 @@ void GPURun(uint32_t vertexCount, vectorT &IndexHost, vectorT &OffsetHost,
   }
   // bottom-up steps read a hub-first copy of the lists, built with the
   // upload (outside the timed region, like the upload itself)
-  if (cfg.symmetric && (cfg.strategy == BFSX_PULL || cfg.strategy == BFSX_LEARNED))
+  if (cfg.symmetric && (cfg.strategy == BFSX_PULL || cfg.strategy == BFSX_LEARNED ||
+                        cfg.strategy == BFSX_BEAMER))
     bfsx_check(g.ctx, bfsx_order_adjacency(g.ctx, BFSX_ORDER_PULL_HUB_FIRST),
                "bfsx_order_adjacency");
   if (cfg.strategy == BFSX_LEARNED && !cfg.sched_weights.empty())
@@ -131,32 +168,7 @@ void GPURun(uint32_t vertexCount, vectorT &IndexHost, vectorT &OffsetHost,
     cfg.last_edges_count = st.edges_count;
   }
 
-  const double threshold = 1.5;  // run_bfs_gpu.hpp:757
-  const std::vector<double> filtered = removeAnomalies(run_times, threshold);
-  const std::vector<double> filtered90 = removeAnomaliesnofilter(run_times);
-  auto mean = [](const std::vector<double> &v) {
-    return v.empty() ? 0.0 : std::accumulate(v.begin(), v.end(), 0.0) / v.size();
-  };
-  auto minv = [](const std::vector<double> &v) {
-    double m = v.empty() ? 0.0 : v[0];
-    for (double x : v) m = x < m ? x : m;
-    return m;
-  };
-  const double total = std::accumulate(run_times.begin(), run_times.end(), 0.0);
-  std::cout << "\n----------------------------------------" << std::endl;
-  printRow("Average (filtered) Time:", formatDouble(mean(filtered)) + " (ms)");
-  printRow("Minimum (filtered) Time:", formatDouble(minv(filtered)) + " (ms)");
-  printRow("Average (90%) Time:", formatDouble(mean(filtered90)) + " (ms)");
-  printRow("Minimum (90%) Time:", formatDouble(minv(filtered90)) + " (ms)");
-  printRow("Average Execution Time:", formatDouble(total / num_runs) + " (ms)");
-  printRow("Total Execution Time:", formatDouble(total) + " (ms)");
-  newJsonObj["rawExecutionTimeAll"] = run_times;
-  newJsonObj["avgExecutionTime"] = total / num_runs;
-  newJsonObj["avgExecutionTimeFiltered"] = mean(filtered);
-  newJsonObj["minExecutionTimeFiltered"] = minv(filtered);
-  newJsonObj["avgExecutionTime90f"] = mean(filtered90);
-  newJsonObj["minExecutionTime90f"] = minv(filtered90);
-  newJsonObj["totalExecutionTime"] = total;
+  timing_record(run_times, newJsonObj);
 }
 
 }  // namespace bfsx

@@ -43,11 +43,6 @@ def log(*a):
     print(*a, flush=True)
 
 
-def init_weights(seed):
-    from oracle import mlp_ref
-    return mlp_ref.init_weights(seed)
-
-
 def evaluate(eng, roots, reps=3):
     out = {}
     for s in ("llb", "learned"):
@@ -107,7 +102,7 @@ def main():
     if args.save_data:
         np.savez(args.save_data, X=X, T=T, M=M, tags=np.array(tags))
     Tr, Mw = targets_and_weights(T, M)
-    W0 = init_weights(args.seed)
+    W0 = bfsx.sched_init_weights(args.seed)
     t0 = time.time()
     W, curve = eng.mlp_train(X, Tr, Mw, W0, args.epochs, args.lr)
     log(f"trained {len(X)} rows, {args.epochs} epochs in {time.time() - t0:.1f}s: "

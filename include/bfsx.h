@@ -53,6 +53,15 @@ extern "C" {
 #define BFSX_SLB 2      /* strided local load balance: :416-417 */
 #define BFSX_PULL 3     /* bottom-up step (symmetric graphs only) */
 #define BFSX_LEARNED 4  /* per-level choice of the four by the MLP scheduler */
+#define BFSX_BEAMER 5   /* direction-optimising rule of Beamer, Asanovic and
+                           Patterson (SC'12), alpha = 14, beta = 24: bottom-up
+                           once the frontier's edges exceed 1/alpha of the
+                           unexplored vertices' edges, top-down again once
+                           the frontier shrinks below n/beta; top-down levels
+                           run GLB when the frontier has at least n/256
+                           edges, else LLB (symmetric graphs; elsewhere
+                           top-down only).  The hand-set rule the learned
+                           scheduler is compared with. */
 #define BFSX_N_STEP_KINDS 4
 
 /* graph flags */
@@ -105,6 +114,14 @@ typedef struct bfsx_level_record {
   uint64_t pull_scanned;    /* adjacency entries read by a bottom-up step */
   double kernel_us;         /* device time of the step's main kernel
                                (k_llb / k_glb_expand / k_pull) */
+  /* learned strategy: the scheduler decision that picked this level's kind
+   * (made at the end of the previous level); sched_evaluated = 1 when the
+   * network ran for it, 0 when the last decision was kept (similar frontier)
+   * or for the root level, which runs LLB without an evaluation */
+  uint32_t sched_evaluated;
+  float sched_in[8];        /* network inputs of the decision in force */
+  float sched_out[4];       /* its outputs: predicted log2 time of LLB, GLB,
+                               SLB, PULL relative to LLB */
 } bfsx_level_record;
 
 /* launch-shape parameters (the SM_FACTOR / work-group knobs) */
@@ -181,8 +198,12 @@ int bfsx_graph_info_get(bfsx_ctx *ctx, bfsx_graph_info *out);
 int bfsx_order_adjacency(bfsx_ctx *ctx, int order);
 /* Copies the resident graph to host buffers of n_cols+1 / nnz entries. */
 int bfsx_download_csc(bfsx_ctx *ctx, void *col_ptr, uint32_t *row_idx);
-/* Seeded roots: mode 0 = vertices of degree >= 1, mode 1 = vertices of the
- * largest connected component (symmetric graphs). */
+/* Roots: mode 0 = seeded, among vertices of degree >= 1; mode 1 = seeded,
+ * among the vertices of the largest connected component (symmetric graphs);
+ * mode 2 = the n_roots vertices of largest degree, ties to the smaller id
+ * (the seed is unused) -- scripts/rootNodes.py:133-159, which takes the top
+ * 5 of networkx's degree (the number of neighbours on a simple symmetric
+ * graph) with a stable sort.  Replaces the reference's offline root script. */
 int bfsx_pick_roots(bfsx_ctx *ctx, uint32_t n_roots, uint64_t seed, int mode,
                     uint32_t *roots_out, uint32_t *n_found);
 

@@ -8,6 +8,9 @@
 //   json FILE                      -> parse and re-dump (indent 4)
 //   validate IN.bin ROOTV LEV.bin  -> certificate check result
 //   args ...                       -> CommandLineParser result
+//   stats T1 T2 ...                -> GPURun's timing record of these run
+//                                     times (summary table, then the JSON
+//                                     record on the last line)
 // IN.bin layout: u64 n, u64 nnz, u64 col_ptr[n+1], u32 row_idx[nnz].
 #include <cstdio>
 #include <fstream>
@@ -17,6 +20,7 @@
 #include "bfsx_json.hpp"
 #include "functions.hpp"
 #include "graph_load.hpp"
+#include "run_bfs_gpu.hpp"
 #include "validate.hpp"
 
 using bfsx::Json;
@@ -123,6 +127,12 @@ int main(int argc, char **argv) {
       Json out = Json::object();
       for (const char *k : {"num_runs", "dataset", "root", "output"}) out[k] = p.getArgument(k);
       std::cout << out.dump() << std::endl;
+    } else if (cmd == "stats") {
+      std::vector<double> t;
+      for (int i = 2; i < argc; i++) t.push_back(std::stod(argv[i]));
+      Json out = Json::object();
+      bfsx::timing_record(t, out);
+      std::cout << "\n" << out.dump() << std::endl;
     } else {
       std::cerr << "unknown command" << std::endl;
       return 2;

@@ -106,8 +106,10 @@ def test_bench_reads_the_committed_traffic_summaries():
     import bench
     t = bench.traffic_from_profiles("rmat26", "pull")
     assert t is not None and 1e8 < t < 1e10
-    rl = bench.with_traffic({"step": "pull", "avg_launch_us": 200.0}, "rmat26")
+    rl = bench.with_traffic({"step": "pull", "avg_launch_us": 200.0,
+                             "alg_bytes_per_launch": 2.5e8}, "rmat26")
     assert rl["traffic"] == t
-    assert abs(rl["hbm_achieved"] - t / 200e-6 / 1e9) < 0.1
-    assert abs(rl["hbm_frac"] - rl["hbm_achieved"] / bench.HBM_PEAK_GBS) < 1e-3
+    assert abs(rl["fabric_achieved"] - t / 200e-6 / 1e9) < 0.1
+    assert abs(rl["fabric_frac"] - rl["fabric_achieved"] / bench.HBM_PEAK_GBS) < 1e-3
+    assert abs(rl["traffic_over_alg"] - t / 2.5e8) < 0.01
     assert bench.traffic_from_profiles("rmat26", "nonexistent") is None

@@ -78,7 +78,12 @@ def test_rmat16_all_roots_all_steps(engine):
     roots = workloads.roots(engine, w)
     cp, ri = engine.download_csc()
     say("rmat16", len(ri))
-    assert_levels(engine, cp, ri, roots, STEPS)
+    assert len(roots) == 64
+    engine.load_scheduler()
+    try:
+        assert_levels(engine, cp, ri, roots, STEPS + ["learned"])
+    finally:
+        engine.set_scheduler_weights(None)
 
 
 def test_rmat19_all_roots_all_steps(engine):
@@ -87,7 +92,11 @@ def test_rmat19_all_roots_all_steps(engine):
     cp, ri = engine.download_csc()
     say("rmat19", len(ri))
     assert len(roots) == 64
-    assert_levels(engine, cp, ri, roots, STEPS)
+    engine.load_scheduler()
+    try:
+        assert_levels(engine, cp, ri, roots, STEPS + ["learned"])
+    finally:
+        engine.set_scheduler_weights(None)
 
 
 def test_rmat_uint64_col_ptr(engine):
@@ -336,14 +345,57 @@ def test_trace_is_consistent(engine):
             assert rec["us"] > 0
 
 
-def test_largest_component_roots(engine):
-    engine.generate_grid(256, 0.05, 6, 99, 4)
+@pytest.mark.parametrize("drop", [0.05, 0.3])
+def test_largest_component_roots(engine, drop):
+    """Seeded roots of the largest connected component equal the oracle's
+    seeded choice among the members of that component, labelled by scipy's
+    connected_components (0.3 of the grid's edges dropped leaves many
+    components)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    engine.generate_grid(256, drop, 6, 99, 4)
     roots = engine.pick_roots(16, 5, mode=1)
     assert len(roots) == 16
     cp, ri = engine.download_csc()
-    lev, _, _ = cpu.bfs_queue(cp, ri, int(roots[0]))
-    assert (lev >= 0).sum() * 2 > len(lev)
-    assert (lev[roots] >= 0).all()
+    n = len(cp) - 1
+    A = csr_matrix((np.ones(len(ri), np.int8), ri.astype(np.int64), cp.astype(np.int64)), (n, n))
+    nc, labels = connected_components(A, directed=False)
+    sizes = np.bincount(labels)
+    assert sizes.max() * 2 > n
+    member = (labels == np.argmax(sizes)).astype(np.uint8)
+    want = cpu.pick_roots(member, 16, 5)
+    assert (roots == want).all(), (roots, want)
+    if drop > 0.2:
+        assert nc > 100
+
+
+@pytest.mark.parametrize("sym", [True, False])
+def test_top_degree_roots(engine, sym):
+    """Mode 2: the k vertices of largest degree, ties to the smaller id --
+    scripts/rootNodes.py:133-137 (networkx degree, stable sort) -- checked
+    against networkx on a symmetric graph and a numpy restatement on a
+    directed one with many ties."""
+    import networkx as nx
+    from scipy.sparse import csr_matrix
+    if sym:
+        engine.generate_rmat(13, 8, 0.57, 0.19, 0.19, 181, 182, 8)
+        cp, ri = engine.download_csc()
+    else:
+        rng = np.random.default_rng(4)
+        cp, ri = random_graph(rng, 3000, 9000, directed=True)
+        engine.upload_csc(cp, ri, n_rows=2990, symmetric=False)
+    n = len(cp) - 1
+    deg = np.diff(cp.astype(np.int64))
+    for k in (1, 5, 64):
+        got = engine.pick_roots(k, 0, mode=2)
+        rows = n if sym else 2990
+        want = np.lexsort((np.arange(rows), -deg[:rows]))[:k]
+        assert (got == want).all(), (k, got[:8], want[:8])
+        if sym:
+            A = csr_matrix((np.ones(len(ri)), ri.astype(np.int64), cp.astype(np.int64)), (n, n))
+            G = nx.from_scipy_sparse_array(A)
+            top = [v for v, _ in sorted(G.degree, key=lambda x: x[1], reverse=True)[:k]]
+            assert list(got) == top
 
 
 def test_level_histogram_and_step_times(engine):

@@ -183,6 +183,34 @@ def test_cli_argument_parser(driver):
     assert p == {"num_runs": "5", "dataset": "foo", "root": "1", "output": "data.json"}
 
 
+@pytest.mark.parametrize("seed,n", [(0, 1), (1, 3), (2, 10), (3, 12), (4, 37)])
+def test_timing_record_matches_numpy(driver, seed, n):
+    """GPURun's 7 timing keys (src/run_bfs_gpu.hpp:756-790) from raw run
+    times: mean, total, the 1.5-sigma filter (population deviation,
+    src/functions.hpp:63-94) and the first-10 %-dropped runs (:96-108),
+    restated in numpy; outliers are planted so that the filter drops some."""
+    rng = np.random.default_rng(seed)
+    raw = rng.uniform(1.0, 2.0, n)
+    if n >= 10:
+        raw[rng.integers(0, n, 2)] = 50.0
+    out = run(driver, "stats", *[repr(float(x)) for x in raw], raw=True)
+    rec = json.loads(out.strip().splitlines()[-1])
+    mean, sd = raw.mean(), raw.std()
+    filt = raw[np.abs(raw - mean) <= 1.5 * sd]
+    f90 = raw[int(n * 0.1):]
+    if n >= 10:
+        assert len(filt) < n and len(f90) < n
+    approx = lambda x: pytest.approx(float(x), rel=1e-12)
+    assert rec["rawExecutionTimeAll"] == [approx(x) for x in raw]
+    assert rec["avgExecutionTime"] == approx(mean)
+    assert rec["totalExecutionTime"] == approx(raw.sum())
+    assert rec["avgExecutionTimeFiltered"] == approx(filt.mean())
+    assert rec["minExecutionTimeFiltered"] == approx(filt.min())
+    assert rec["avgExecutionTime90f"] == approx(f90.mean())
+    assert rec["minExecutionTime90f"] == approx(f90.min())
+    assert "Average (90%) Time:" in out
+
+
 def test_certificate_accepts_exact_levels_and_rejects_mistakes(driver, tmp_path):
     rng = np.random.default_rng(4)
     for directed in (False, True):

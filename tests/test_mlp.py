@@ -161,6 +161,58 @@ def test_learned_decision_follows_the_network(engine):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_in_loop_inference_matches_oracle(engine, seed):
+    """The network the level loop evaluates at each level end (a separate
+    implementation from the batch forward kernel) against the float64
+    oracle: for every level whose decision the network made, its inputs are
+    the oracle's frontier statistics, its outputs equal mlp_ref.forward of
+    those inputs, and the level ran the argmin over the allowed kinds; a
+    level that kept the last decision shows the same inputs and outputs as
+    the level before it.  Random, asymmetric weights, so a transposed or
+    shifted weight index changes the outputs."""
+    import bfsx
+    engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 171, 172, 4)
+    cp, ri = engine.download_csc()
+    info = engine.info()
+    n, nnz, n_iso = info["n_cols"], info["nnz"], info["n_isolated"]
+    deg = np.diff(cp.astype(np.int64))
+    W = mlp_ref.init_weights(300 + seed) * 2
+    W[mlp_ref.NPARAM - 4:] = np.random.default_rng(seed).normal(0, 0.5, 4)
+    engine.set_scheduler_weights(W)
+    evaluated = 0
+    try:
+        for r in engine.pick_roots(6, 173 + seed):
+            lev, st = engine.run(int(r), "learned", trace=True)
+            want = cpu.bfs_queue(cp, ri, int(r))[0]
+            assert np.array_equal(lev, want)
+            tr = engine.last_trace()
+            assert tr[0]["sched_evaluated"] == 0 and tr[0]["choice"] == bfsx.LLB
+            for L in range(1, len(tr)):
+                rec, prev = tr[L], tr[L - 1]
+                x = np.array(rec["sched_in"], np.float64)
+                y = np.array(rec["sched_out"], np.float64)
+                assert rec["choice"] == int(np.argmin(y)), (L, rec["choice"], y)
+                if not rec["sched_evaluated"]:
+                    assert L > 1 and rec["sched_in"] == prev["sched_in"]
+                    assert rec["sched_out"] == prev["sched_out"]
+                    continue
+                evaluated += 1
+                at = want == L
+                seen = (want >= 0) & (want < L)
+                xw = mlp_ref.features(int(at.sum()), int(deg[at].sum()),
+                                      n - n_iso - int(seen.sum()), nnz - int(deg[seen].sum()),
+                                      n, nnz, prev["choice"] == bfsx.PULL)
+                np.testing.assert_allclose(x, xw, rtol=1e-6, atol=1e-6, err_msg=f"level {L}")
+                yw = mlp_ref.forward(W, x[None, :])[0]
+                np.testing.assert_allclose(y, yw, rtol=1e-4, atol=1e-4 * np.abs(yw).max(),
+                                           err_msg=f"level {L}")
+    finally:
+        engine.set_scheduler_weights(None)
+    assert evaluated >= 6
+
+
+@pytest.mark.gpu
 def test_record_rows(engine):
     engine.generate_rmat(14, 16, 0.57, 0.19, 0.19, 101, 102, 4)
     cp, ri = engine.download_csc()
