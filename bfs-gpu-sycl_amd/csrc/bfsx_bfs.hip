@@ -1353,6 +1353,7 @@ __global__ void k_init(const PtrT *col_ptr, uint32_t n_rows, uint32_t n_cols, ui
     c.visited_vertices = 1;
     c.visited_edges = d;
     c.choice = BFSX_LLB;
+    c.seq = ctl->seq;  // the level-end counter runs on across searches
     *ctl = c;
   }
 }
@@ -1368,7 +1369,7 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
                                                   uint32_t produced, int stamp, int kind,
                                                   const float *Wg, uint64_t n_cols, uint64_t nnz,
                                                   uint64_t n_iso, uint32_t allow, int reuse,
-                                                  LevelCtl *mirror) {
+                                                  LevelMirror *mirror) {
   __shared__ int s_live;
   if (threadIdx.x == 0) {
     s_live = 0;
@@ -1417,18 +1418,18 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
     // keeps the last decision costs what a level without the scheduler does)
     if (s_live) decide_wave(ctl, Wg, produced, n_cols, nnz, n_iso, allow, 0);
   }
-  // The finished block into the host-mapped mirror (thread 0 made every
-  // write of this launch), the sequence number last: the host polls it
-  // instead of synchronising the stream.
+  // What the host needs for the next level, into the host-mapped words, each
+  // tagged with this level end's sequence number (thread 0 made every write
+  // of this launch): the host polls them instead of synchronising the stream.
   if (mirror && threadIdx.x == 0) {
     const uint32_t seq = ctl->seq + 1;
     ctl->seq = seq;
-    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(ctl);
-    unsigned long long *dst = reinterpret_cast<unsigned long long *>(mirror);
-    constexpr uint32_t WORDS = sizeof(LevelCtl) / 8;
-    for (uint32_t i = 0; i + 1 < WORDS; i++) dst[i] = src[i];
-    __threadfence_system();
-    dst[WORDS - 1] = src[WORDS - 1];  // holds seq
+    const unsigned long long t32 = (unsigned long long)seq << 32;
+    const unsigned long long t16 = (unsigned long long)(seq & 0xFFFFu) << 48;
+    mirror->w[0] = t32 | ctl->q_count[produced];
+    mirror->w[1] = t16 | (ctl->q_edges[produced] & MIRROR_48);
+    mirror->w[2] = t16 | (ctl->visited_edges & MIRROR_48);
+    mirror->w[3] = t32 | (ctl->choice & 0xFFu) | ((unsigned long long)(ctl->done ? 1u : 0u) << 8);
     __threadfence_system();
   }
 }

@@ -79,6 +79,8 @@ static void ensure_pull_state(bfsx_ctx *ctx) {
 }
 
 static void after_graph_change(bfsx_ctx *ctx) {
+  // edge counts travel in 48-bit fields of the level-end words
+  if (ctx->nnz > MIRROR_48) fail(BFSX_ERR_ARG, "graphs above 2^48 stored edges are not supported");
   release(ctx->row_idx_pull);  // a copy of the old lists
   release(ctx->pull_fd);
   ctx->gflags &= ~BFSX_GRAPH_PULL_COPY;
@@ -122,28 +124,53 @@ struct RunOut {
   uint32_t steps[BFSX_N_STEP_KINDS] = {0, 0, 0, 0};
 };
 
+// Full copy of the control block into h_ctl (stream synchronised).
 static void read_ctl(bfsx_ctx *ctx) {
   BFSX_HIP(hipMemcpyAsync(ctx->h_ctl, ctx->ctl.p, sizeof(LevelCtl), hipMemcpyDeviceToHost,
                           ctx->stream));
   BFSX_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-// After a launch that ends with k_level_end: that kernel leaves the control
-// block in the host-mapped mirror, sequence number last, so the host polls
-// the number (no copy, no stream synchronisation).  After a second without
-// it, the stream is synchronised, which reports any device error.
-static void read_ctl_after_level(bfsx_ctx *ctx, uint32_t seq) {
-  const volatile uint32_t *p = &ctx->h_ctl->seq;
+// ctx->view from the full copy in h_ctl; `produced` = the queue buffer the
+// last step filled.
+static void view_from_ctl(bfsx_ctx *ctx, uint32_t produced) {
+  const LevelCtl &c = *ctx->h_ctl;
+  ctx->view.F = c.q_count[produced];
+  ctx->view.EF = c.q_edges[produced];
+  ctx->view.VE = c.visited_edges;
+  ctx->view.choice = c.choice;
+  ctx->view.done = c.done;
+}
+
+// After a launch that ends with k_level_end: that kernel writes the values
+// the host needs into the host-mapped mirror, every word tagged with its
+// sequence number, so the host polls the words (no copy, no stream
+// synchronisation) until all four carry `seq` -- a word of the previous
+// level end next to a new one means the new ones have not all arrived.
+// After a second without them, the stream is synchronised, which reports any
+// device error, and the values come from a full copy of the control block.
+static void read_ctl_after_level(bfsx_ctx *ctx, uint32_t seq, uint32_t produced) {
+  const volatile unsigned long long *w = ctx->h_mirror->w;
+  const unsigned long long t32 = seq, t16 = seq & 0xFFFFu;
   const auto t0 = std::chrono::steady_clock::now();
-  for (uint32_t spin = 0; *p != seq; spin++) {
+  for (uint32_t spin = 0;; spin++) {
+    const unsigned long long w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+    if ((w0 >> 32) == t32 && (w1 >> 48) == t16 && (w2 >> 48) == t16 && (w3 >> 32) == t32) {
+      ctx->view.F = (uint32_t)w0;
+      ctx->view.EF = w1 & MIRROR_48;
+      ctx->view.VE = w2 & MIRROR_48;
+      ctx->view.choice = (uint32_t)(w3 & 0xFFu);
+      ctx->view.done = (uint32_t)((w3 >> 8) & 1u);
+      return;
+    }
     if ((spin & 1023u) == 1023u &&
         std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
-      BFSX_HIP(hipStreamSynchronize(ctx->stream));
-      if (*p != seq) read_ctl(ctx);
-      break;
+      read_ctl(ctx);  // synchronises the stream (reports device errors)
+      if (ctx->h_ctl->seq != seq) fail(BFSX_ERR_DEVICE, "level end did not arrive");
+      view_from_ctl(ctx, produced);
+      return;
     }
   }
-  std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 // Top-down levels whose frontier has at least this many edges mark and
@@ -256,9 +283,9 @@ struct Walker {
     if (new_fbm) fbm_cur ^= 1;
     fbm_valid = new_fbm;
     queue_valid = last != BFSX_PULL;
-    F = ctx->h_ctl->q_count[cur ^ 1];
-    EF = ctx->h_ctl->q_edges[cur ^ 1];
-    VE = ctx->h_ctl->visited_edges;
+    F = ctx->view.F;
+    EF = ctx->view.EF;
+    VE = ctx->view.VE;
     cur ^= 1;
     level++;
   }
@@ -281,14 +308,15 @@ static Walker start_search(bfsx_ctx *ctx, uint32_t root, bool pull_possible, boo
   }
   init_search(ctx, root, graph_symmetric(ctx), par);
   read_ctl(ctx);
+  view_from_ctl(ctx, 0);
   Walker w;
   w.ctx = ctx;
   w.pull_possible = pull_possible;
   w.track = true;
   w.stamp = stamp;
   w.par = par;
-  w.EF = ctx->h_ctl->q_edges[0];
-  w.VE = ctx->h_ctl->visited_edges;
+  w.EF = ctx->view.EF;
+  w.VE = ctx->view.VE;
   return w;
 }
 
@@ -345,21 +373,22 @@ static RunOut search_graph_loop(bfsx_ctx *ctx, uint32_t root, int fixed, bool le
   BFSX_HIP(hipEventRecord(ctx->ev0, s));
   // (the root level runs LLB, as in the host loop)
   const uint64_t max_launches = (uint64_t)ctx->n_cols / 2 + 2;
-  // each launch runs K level ends, each bumping the mirror's sequence number
+  // each launch runs K level ends, each bumping the level-end counter
   const uint32_t K = (ctx->params.graph_levels + 1) & ~1u;
   uint32_t seq = ctx->h_ctl->seq;
   for (uint64_t launches = 0;; launches++) {
     if (launches > max_launches) fail(BFSX_ERR_DEVICE, "captured level loop did not terminate");
     BFSX_HIP(hipGraphLaunch(exec, s));
     seq += K;
-    read_ctl_after_level(ctx, seq);
-    if (ctx->h_ctl->done) break;
+    read_ctl_after_level(ctx, seq, 0);  // (K is even: the last level fills queue 0)
+    if (ctx->view.done) break;
   }
   BFSX_HIP(hipEventRecord(ctx->ev1, s));
   BFSX_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0;
   BFSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   out.ms = ms;
+  read_ctl(ctx);  // per-kind level counts
   for (int k = 0; k < BFSX_N_STEP_KINDS; k++) {
     out.steps[k] = ctx->h_ctl->kind_count[k];
     out.n_levels += out.steps[k];
@@ -432,12 +461,12 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     f_edges = ctx->h_ctl->q_edges[0];
     visited_e = ctx->h_ctl->visited_edges;
   }
-  uint32_t seq = ctx->h_ctl->seq;  // the mirror holds the block k_init wrote
+  uint32_t seq = ctx->h_ctl->seq;  // level ends so far (h_ctl: the block k_init wrote)
   uint32_t decisions = ctx->h_ctl->decisions;
   while (w.F > 0) {
     // a search has at most n_cols levels; more means a broken invariant
     if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
-    const int choice = learned  ? (int)ctx->h_ctl->choice
+    const int choice = learned  ? (int)ctx->view.choice
                        : beamer ? rule.next(ctx, w.pull_possible, w.F, w.EF, w.VE)
                                 : fixed;
     const uint32_t level = w.level, F = w.F, cur = w.cur;
@@ -452,7 +481,10 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
     const bool new_fbm = w.launch(choice);
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
-    read_ctl_after_level(ctx, ++seq);
+    read_ctl_after_level(ctx, ++seq, cur ^ 1);
+    // per-level records read the whole block (queue sizes, running totals,
+    // scheduler inputs and outputs)
+    if (timed) read_ctl(ctx);
     w.advance(new_fbm);
     if (timed) {
       r.level = level;
@@ -505,7 +537,8 @@ static void snapshot(bfsx_ctx *ctx, bool save) {
   } else {
     BFSX_HIP(hipMemcpyAsync(ctx->levels.p, ctx->snap_levels.p, lb, hipMemcpyDeviceToDevice, s));
     BFSX_HIP(hipMemcpyAsync(ctx->vis.p, ctx->snap_vis.p, vb, hipMemcpyDeviceToDevice, s));
-    BFSX_HIP(hipMemcpyAsync(ctx->ctl.p, ctx->snap_ctl.p, sizeof(LevelCtl), hipMemcpyDeviceToDevice, s));
+    // (the level-end counter stays as it is)
+    BFSX_HIP(hipMemcpyAsync(ctx->ctl.p, ctx->snap_ctl.p, CTL_RESET_BYTES, hipMemcpyDeviceToDevice, s));
   }
 }
 
@@ -584,6 +617,7 @@ static uint32_t record(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, u
                               hipMemcpyHostToDevice, ctx->stream));
       const bool new_fbm = w.launch(take);  // its level end decides the next level
       read_ctl(ctx);
+      view_from_ctl(ctx, w.cur ^ 1);
       w.advance(new_fbm);
       rows++;
     }
@@ -697,9 +731,13 @@ int bfsx_create(int device, bfsx_ctx **out) {
     BFSX_HIP(hipEventCreate(&ctx->ev_k0));
     BFSX_HIP(hipEventCreate(&ctx->ev_k1));
     ensure(ctx->ctl, sizeof(LevelCtl), ctx->stream);
-    BFSX_HIP(hipHostMalloc((void **)&ctx->h_ctl, sizeof(LevelCtl),
+    BFSX_HIP(hipMemsetAsync(ctx->ctl.p, 0, sizeof(LevelCtl), ctx->stream));  // level-end counter 0
+    BFSX_HIP(hipStreamSynchronize(ctx->stream));
+    BFSX_HIP(hipHostMalloc((void **)&ctx->h_ctl, sizeof(LevelCtl), hipHostMallocDefault));
+    BFSX_HIP(hipHostMalloc((void **)&ctx->h_mirror, sizeof(LevelMirror),
                            hipHostMallocMapped | hipHostMallocCoherent));
-    BFSX_HIP(hipHostGetDevicePointer((void **)&ctx->d_mirror, ctx->h_ctl, 0));
+    std::memset(ctx->h_mirror, 0, sizeof(LevelMirror));
+    BFSX_HIP(hipHostGetDevicePointer((void **)&ctx->d_mirror, ctx->h_mirror, 0));
     ctx->params = default_params();
   });
   if (rc != BFSX_OK) {
@@ -725,6 +763,7 @@ void bfsx_destroy(bfsx_ctx *ctx) {
   for (DevBuf *b : {&ctx->sched_w, &ctx->snap_levels, &ctx->snap_vis, &ctx->snap_ctl})
     release(*b);
   if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
+  if (ctx->h_mirror) (void)hipHostFree(ctx->h_mirror);
   for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->ev_step0, ctx->ev_step1, ctx->ev_k0, ctx->ev_k1})
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -42,11 +42,37 @@ struct LevelCtl {
   uint32_t dec_F, dec_pull, dec_valid;
   uint32_t decisions;  // network evaluations so far (sched_in/out hold the last)
   uint32_t reserved;
-  uint32_t seq;  // k_level_end launches so far (written last into the host mirror)
+  // Level ends so far on this context: only the level-end kernel changes it
+  // (+1 each), so the tag it puts on the host-mapped words never repeats;
+  // every other writer of the block keeps it (CTL_RESET_BYTES).
+  uint32_t seq;
 };
-static_assert(sizeof(LevelCtl) % 8 == 0, "the level-end kernel copies the block as 8-byte words");
-static_assert(offsetof(LevelCtl, seq) >= sizeof(LevelCtl) - 8,
-              "seq lives in the last 8-byte word, which the level-end kernel writes last");
+static_assert(offsetof(LevelCtl, seq) + sizeof(uint32_t) == sizeof(LevelCtl),
+              "seq is the last field: resets and restores cover the bytes before it");
+constexpr size_t CTL_RESET_BYTES = offsetof(LevelCtl, seq);
+
+// What the host reads at a level end without synchronising the stream: four
+// 8-byte words in host-mapped memory, written by the level-end kernel.
+// Device writes to host memory can reach it out of order even across a
+// system-scope fence, so every word carries the level end's sequence number
+// and the host takes the words only once all four carry the number it waits
+// for (an 8-byte aligned store arrives whole).
+//   w[0] = seq << 32 | frontier size (count of the produced queue)
+//   w[1] = (seq & 0xFFFF) << 48 | frontier edges (48 bits)
+//   w[2] = (seq & 0xFFFF) << 48 | edges of the visited vertices (48 bits)
+//   w[3] = seq << 32 | next step kind (8 bits) | done << 8
+struct LevelMirror {
+  unsigned long long w[4];
+};
+constexpr unsigned long long MIRROR_48 = (1ull << 48) - 1;
+
+// The host's copy of those values (from the mirror, or from a full copy of
+// the control block).
+struct LevelView {
+  uint32_t F = 0;
+  unsigned long long EF = 0, VE = 0;
+  uint32_t choice = 0, done = 0;
+};
 
 // Bitmaps hold whole 1024-vertex chunks (32 words) so that a wave can own a
 // chunk; mark arrays hold one byte per vertex, padded the same way.
@@ -93,8 +119,10 @@ struct bfsx_ctx {
   bfsx::DevBuf fsum;     // bottom-up step: frontier summary bitmap (PULL_SUM_WORDS_MAX)
   bfsx::DevBuf bkt, bkt_cnt, bkt_tot, bkt_base;  // GLB bucketed discovery
   bfsx::DevBuf scratch;  // generic scratch (reductions, root picking)
-  bfsx::LevelCtl *h_ctl = nullptr;  // pinned, host-mapped mirror
-  bfsx::LevelCtl *d_mirror = nullptr;  // device address of h_ctl, written by k_level_end
+  bfsx::LevelCtl *h_ctl = nullptr;        // pinned: full copies of the control block
+  bfsx::LevelMirror *h_mirror = nullptr;  // pinned, host-mapped: tagged level-end words
+  bfsx::LevelMirror *d_mirror = nullptr;  // device address of h_mirror (k_level_end writes it)
+  bfsx::LevelView view;                   // the values of the last level end the host read
   bool mark_dirty = false;          // a mark step did not reach its compaction
 
   // learned scheduler weights and the snapshot used to time alternatives

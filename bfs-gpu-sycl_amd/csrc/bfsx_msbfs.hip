@@ -459,7 +459,7 @@ static MsResult ms_run_t(bfsx_ctx *ctx, const uint32_t *d_src, const uint16_t *d
                      n_src, seen, cur, lv);
   BFSX_HIP(hipMemcpyAsync(ctx->queue[0].p, distinct_src.data(),
                           sizeof(uint32_t) * distinct_src.size(), hipMemcpyHostToDevice, s));
-  BFSX_HIP(hipMemsetAsync(ctl, 0, sizeof(LevelCtl), s));
+  BFSX_HIP(hipMemsetAsync(ctl, 0, CTL_RESET_BYTES, s));  // (keeps the level-end counter)
   // vertices that bottom-up levels hand to k_ms_pull_hubs
   uint32_t n_phubs = 0;
   if (symmetric) {

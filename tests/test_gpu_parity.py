@@ -430,6 +430,69 @@ def test_level_histogram_and_step_times(engine):
             assert lo <= t["pull_scanned"] <= hi, (k, lo, t["pull_scanned"], hi)
 
 
+def test_short_searches_then_long_ones(engine):
+    """Searches of one level (isolated root) and of many levels alternate on
+    one context, with and without per-level traces, in the host loop and the
+    captured loop: each equals the oracle (the level-end words the host polls
+    must never carry a tag an earlier search already left there)."""
+    cp, ri = path_graph(300)
+    cp = np.concatenate([cp, [cp[-1]]]).astype(cp.dtype)  # vertex 300: isolated
+    engine.upload_csc(cp, ri, symmetric=True)
+    for trace in (False, True, False):
+        for s in ("llb", "slb", "glb", "pull"):
+            for r in (300, 0, 300, 150, 300, 300, 7):
+                want, _, _ = cpu.bfs_queue(cp, ri, r)
+                lev, st = engine.run(r, s, trace=trace)
+                assert np.array_equal(lev, want), (trace, s, r)
+                assert st["n_levels"] == int(want.max()) + 1
+                if s in ("llb", "slb") and not trace:
+                    lev, _ = engine.run(r, s, graph_loop=True)
+                    assert np.array_equal(lev, want), ("loop", s, r)
+
+
+def test_back_to_back_runs_into_pinned_buffers(engine):
+    """Searches issued back to back, levels copied into pinned host buffers
+    (DMA writes to host memory interleaved with the level loop's host-mapped
+    level-end words): every run equals the oracle.  The level-end words are
+    tagged with their sequence number because such writes can reach the host
+    out of order; untagged, about 1 run in 200 of this loop ended a level
+    early, having read a new sequence number next to the previous level's
+    frontier size."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    hip.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+    hip.hipHostFree.argtypes = [C.c_void_p]
+    engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 91, 1091, 4)
+    cp, ri = engine.download_csc()
+    n = len(cp) - 1
+    root = int(np.argmax(np.diff(cp.astype(np.int64))))
+    want, _, _ = cpu.bfs_queue(cp, ri, root)
+    lib = bfsx.lib()
+    bufs = []
+    try:
+        for _ in range(8):
+            p = C.c_void_p()
+            assert hip.hipHostMalloc(C.byref(p), 4 * n, 0) == 0
+            bufs.append(p)
+        views = [np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_int32)), (n,)) for p in bufs]
+        st = bfsx.RunStats()
+        for s in ("slb", "llb", "glb"):
+            sid = bfsx.strategy_id(s)
+            for i in range(480):
+                rc = lib.bfsx_run(engine._h, root, sid, 0, bufs[i % 8], None, C.byref(st))
+                assert rc == 0
+                if i % 8 == 7:
+                    for v in views:
+                        bad = np.flatnonzero(v != want)
+                        assert bad.size == 0, (s, i, bad.size, bad[:5])
+    finally:
+        for p in bufs:
+            hip.hipHostFree(p)
+
+
 def test_stream_copy(engine):
     ms = engine.stream_copy(1 << 26, 4)
     assert ms > 0

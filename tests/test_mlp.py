@@ -198,8 +198,9 @@ def test_in_loop_inference_matches_oracle(engine, seed):
                     assert rec["sched_out"] == prev["sched_out"]
                     continue
                 evaluated += 1
+                # (decided at the end of level L-1: level L is discovered)
                 at = want == L
-                seen = (want >= 0) & (want < L)
+                seen = (want >= 0) & (want <= L)
                 xw = mlp_ref.features(int(at.sum()), int(deg[at].sum()),
                                       n - n_iso - int(seen.sum()), nnz - int(deg[seen].sum()),
                                       n, nnz, prev["choice"] == bfsx.PULL)
