@@ -126,6 +126,8 @@ SIGNATURES = {
                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "bfsx_get_params": (C.c_int, [P, C.POINTER(Params)]),
     "bfsx_set_params": (C.c_int, [P, C.POINTER(Params)]),
+    "bfsx_run_noalloc": (C.c_int, [C.c_int, P, P, C.c_int, P, C.c_uint32, C.c_uint32,
+                                   C.c_uint64, C.c_uint32, P, C.POINTER(C.c_uint32)]),
 }
 
 

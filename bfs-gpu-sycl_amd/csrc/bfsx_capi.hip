@@ -694,7 +694,9 @@ const char *bfsx_status_string(int st) {
   }
 }
 
-const char *bfsx_last_error(const bfsx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *bfsx_last_error(const bfsx_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : bfsx::noalloc_last_error();
+}
 
 int bfsx_create(int device, bfsx_ctx **out) {
   if (!out) return BFSX_ERR_ARG;

@@ -246,6 +246,9 @@ void gen_rmat_pairs_offset(bfsx_ctx *ctx, int scale, uint64_t m, uint32_t tA, ui
                            uint32_t tABC, uint64_t seed_graph, uint64_t seed_perm,
                            uint32_t offset, uint32_t *d_u, uint32_t *d_v);
 
+// ---- allocation-free search (bfsx_noalloc.hip) ----------------------------
+const char *noalloc_last_error();  // the calling thread's last failure message
+
 // ---- learned scheduler (bfsx_sched.hip) -----------------------------------
 // Features of the frontier in queue `idx` -> MLP -> ctl->choice.
 void decide(bfsx_ctx *ctx, uint32_t idx, bool first, uint32_t allow_mask);

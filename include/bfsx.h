@@ -156,6 +156,8 @@ typedef struct bfsx_params {
 /* ---- context ------------------------------------------------------------ */
 int bfsx_create(int device, bfsx_ctx **out);
 void bfsx_destroy(bfsx_ctx *ctx);
+/* message of the context's last failure; with NULL, that of the calling
+ * thread's last bfsx_run_noalloc */
 const char *bfsx_last_error(const bfsx_ctx *ctx);
 const char *bfsx_status_string(int status);
 int bfsx_device_info(bfsx_ctx *ctx, char *name, size_t name_len,
@@ -306,6 +308,31 @@ int bfsx_set_params(bfsx_ctx *ctx, const bfsx_params *in);
 int bfsx_autotune(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, int strategy,
                   uint32_t passes, bfsx_params *best, double *gteps_before,
                   double *gteps_after);
+
+/* ---- allocation-free search (no counterpart in the reference) -----------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  One BFS
+ * from `root` over a CSC graph the caller already holds in device memory
+ * (col_ptr: n_cols+1 entries of col_ptr_bytes, row_idx: nnz entries; only
+ * vertices < n_rows are expanded), writing int32 levels (-1 unreached) to the
+ * caller's device array `levels` (n_cols entries); *n_levels (optional, host)
+ * receives the number of levels, as bfsx_run's stats.n_levels.  Levels equal
+ * bfsx_run's.  The call allocates nothing: the search state (visited bitmap,
+ * two frontier queues, counters; BFSX_NOALLOC_SCRATCH_BYTES) lives in a fixed
+ * device array of the library, one per GPU, held until the process ends, so
+ * graphs are limited to BFSX_NOALLOC_MAX_VERTICES vertices (larger ones are
+ * rejected with BFSX_ERR_ARG).  Every call re-initialises that state; calls
+ * that overlap on one GPU race on it.  Kernels run on `stream` (a
+ * hipStream_t; NULL = the null stream) and the call returns when the search
+ * has finished; the calling thread's current device is restored.  A
+ * col_ptr or row_idx entry outside the graph is skipped by the kernels and
+ * reported as BFSX_ERR_ARG.  The message of the last failure on the calling
+ * thread: bfsx_last_error(NULL). */
+#define BFSX_NOALLOC_MAX_VERTICES (1u << 22)
+#define BFSX_NOALLOC_SCRATCH_BYTES 34079488u
+int bfsx_run_noalloc(int device, void *stream, const void *col_ptr, int col_ptr_bytes,
+                     const uint32_t *row_idx, uint32_t n_cols, uint32_t n_rows, uint64_t nnz,
+                     uint32_t root, int32_t *levels, uint32_t *n_levels);
 
 #ifdef __cplusplus
 }
