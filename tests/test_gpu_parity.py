@@ -61,6 +61,24 @@ def test_grid_builder_matches_oracle(engine):
     assert nnz == len(wri) and (cp == wcp).all() and (ri == wri).all()
 
 
+def test_builder_matches_committed_checksums(engine):
+    """The GPU builder's R-MAT 16, R-MAT 19 and 256 x 256 grid (id-ordered
+    lists) have the nnz and SHA-256 digests committed in
+    tests/golden/generated_checksums.json."""
+    import json
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_generated_checksums as mk
+    want = json.load(open(os.path.join(here, "golden", "generated_checksums.json")))
+    for name, wl, over in mk.GRAPHS:
+        workloads.build(engine, wl, order=None, **over)
+        got = mk.digest(*engine.download_csc())
+        for k in ("nnz", "vertices", "col_ptr_bytes", "col_ptr_sha256", "row_idx_sha256"):
+            assert got[k] == want[name][k], (name, k)
+
+
 def test_roots_match_oracle(engine):
     w, _ = workloads.build(engine, "rmat16")
     roots = workloads.roots(engine, w)

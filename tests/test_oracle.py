@@ -293,3 +293,26 @@ def test_pick_roots_seeded_and_eligible():
     assert len(r) == 20 and len(set(r.tolist())) == 20
     assert (elig[r] == 1).all()
     assert (r == cpu.pick_roots(elig, 20, 42)).all()
+
+
+# ---------------------------------------------------------------------------
+# the synthetic-graph recipe against its committed checksums
+# ---------------------------------------------------------------------------
+def test_generated_graphs_match_committed_checksums():
+    """R-MAT 16, R-MAT 19 and a 256 x 256 grid built by the oracle's
+    restatement of the recipe have the nnz and SHA-256 digests committed in
+    tests/golden/generated_checksums.json (made by
+    tests/golden/make_generated_checksums.py)."""
+    import json
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_generated_checksums as mk
+    want = json.load(open(os.path.join(here, "golden", "generated_checksums.json")))
+    assert set(want) == {g[0] for g in mk.GRAPHS}
+    for name in want:
+        _, cp, ri = mk.build(name)
+        got = mk.digest(cp, ri)
+        for k in ("nnz", "vertices", "col_ptr_bytes", "col_ptr_sha256", "row_idx_sha256"):
+            assert got[k] == want[name][k], (name, k)
