@@ -276,6 +276,47 @@ def small_configs(eng, names=("rmat16", "rmat19"), runs=3):
     return out
 
 
+def bfsx_weights():
+    import bfsx
+    return bfsx.have_scheduler_weights()
+
+
+def sequential_searches(eng, sources, off, args):
+    """The batch's searches one at a time, each on its own graph, learned
+    schedule: the 256 roots on R-MAT scale 22 alone, each small graph's root
+    on that graph alone (same seeds, so the same vertex ids up to the
+    packing offset).  Aggregate = all edges / all search time (the mean of
+    the timed runs per search).  Rebuilds the graphs, so it runs after the
+    batch."""
+    import workloads
+    n = max(1, min(args.steps, 3))
+    edges, ms = 0.0, 0.0
+    eng.load_scheduler()
+    try:
+        workloads.build(eng, "rmat22")
+        big = np.asarray(sources[:256], np.uint32)
+        _, st = eng.run_batch(big, "learned")  # warm-up
+        runs, st = sweep(eng, big, "learned", n)
+        edges += float(sum(x["edges_count"] for x in st))
+        ms += float(np.sum(per_root_times(runs)))
+        a, b, c = workloads.RMAT_ABC
+        for i, (scale, ef) in enumerate(workloads.small_graph_shapes(len(off) - 2)):
+            eng.generate_rmat(scale, ef, a, b, c, 4000 + i, 5000 + i, 4)
+            eng.order_adjacency(workloads.ORDER)
+            r = np.array([int(sources[256 + i]) - int(off[i + 1])], np.uint32)
+            eng.run_batch(r, "learned")
+            runs, st = sweep(eng, r, "learned", n)
+            edges += float(st[0]["edges_count"])
+            ms += float(np.sum(per_root_times(runs)))
+    finally:
+        eng.set_scheduler_weights(None)
+    out = {"gteps": round(edges / (ms * 1e-3) / 1e9, 2), "ms_total": round(ms, 3),
+           "searches": int(len(sources)), "edges": edges,
+           "graphs": "R-MAT 22 alone for its 256 roots, each small graph alone for its root"}
+    log(f"sequential learned searches: {out}")
+    return out
+
+
 def bench_batch(eng, args):
     """R-MAT scale 22 with 256 searches in one bitset batch, packed with 64
     small R-MAT graphs searched in the same launches; aggregate GTEPS =
@@ -286,6 +327,11 @@ def bench_batch(eng, args):
     info = eng.info()
     log(f"built batch: V={info['n_cols']} nnz={info['nnz']} graphs={len(off) - 1} "
         f"in {time.time() - t0:.1f}s")
+    # the batch's launch parameters tuned on this batch (outside the timed
+    # region): relabelling, direction divisor, hub threshold, levels per
+    # captured launch
+    tuned, tb, ta = eng.autotune_multi(sources, bits, passes=1)
+    log(f"batch autotune: {tb:.1f} -> {ta:.1f} GTEPS", {k: tuned[k] for k in tuned if k.startswith("ms_")})
     for _ in range(args.warmup):
         eng.run_multi(sources, bits, n_bits, levels=False)
     times, st = [], None
@@ -296,6 +342,7 @@ def bench_batch(eng, args):
     wall = time.perf_counter() - t_wall
     t = float(np.mean(times[int(len(times) * 0.1):]))
     value = st["edges_count"] / (t * 1e-3) / 1e9
+    seq = sequential_searches(eng, sources, off, args) if bfsx_weights() else None
     return {
         "metric": "aggregate GTEPS of one batched launch: 256 searches on R-MAT scale 22 "
                   "plus 64 packed R-MAT graphs (scales 12-18), uint8 levels",
@@ -309,6 +356,9 @@ def bench_batch(eng, args):
                    "levels": st["n_levels"], "push_levels": st["push_levels"],
                    "pull_levels": st["pull_levels"], "batch_ms": round(t, 3),
                    "edges_all_searches": st["edges_count"]},
+        "tuned": {k: v for k, v in tuned.items() if k.startswith("ms_")},
+        "sequential_learned": seq,
+        "speedup_vs_sequential": round(value / seq["gteps"], 3) if seq else None,
     }
 
 

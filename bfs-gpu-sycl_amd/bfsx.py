@@ -72,7 +72,9 @@ class Params(C.Structure):
                 ("glb_blocks_per_cu", C.c_uint32), ("glb_chunk", C.c_uint32),
                 ("hub_degree", C.c_uint32), ("pull_wg", C.c_uint32),
                 ("graph_levels", C.c_uint32), ("mark_ratio_q10", C.c_uint32),
-                ("bucket_ratio_q10", C.c_uint32)]
+                ("bucket_ratio_q10", C.c_uint32), ("ms_relabel", C.c_uint32),
+                ("ms_push_divisor", C.c_uint32), ("ms_pull_hub", C.c_uint32),
+                ("ms_graph_levels", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -124,6 +126,9 @@ SIGNATURES = {
                                     C.c_uint32, C.POINTER(C.c_uint32)]),
     "bfsx_autotune": (C.c_int, [P, P, C.c_uint32, C.c_int, C.c_uint32, C.POINTER(Params),
                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "bfsx_autotune_multi": (C.c_int, [P, P, P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.POINTER(Params), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
     "bfsx_get_params": (C.c_int, [P, C.POINTER(Params)]),
     "bfsx_set_params": (C.c_int, [P, C.POINTER(Params)]),
     "bfsx_run_noalloc": (C.c_int, [C.c_int, P, P, C.c_int, P, C.c_uint32, C.c_uint32,
@@ -310,6 +315,18 @@ class Engine:
         before, after = C.c_double(0), C.c_double(0)
         self._check(lib().bfsx_autotune(self._h, _ptr(roots), len(roots), strategy_id(strategy),
                                         passes, C.byref(p), C.byref(before), C.byref(after)))
+        return p.as_dict(), before.value, after.value
+
+    def autotune_multi(self, sources, bits, passes=1):
+        """Tunes the batched search's parameters on these searches; returns
+        (params dict, aggregate GTEPS before, after).  The params stay installed."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        b = np.ascontiguousarray(bits, np.uint16)
+        p = Params()
+        before, after = C.c_double(0), C.c_double(0)
+        self._check(lib().bfsx_autotune_multi(self._h, _ptr(src), _ptr(b), len(src),
+                                              int(b.max()) + 1 if len(b) else 1, passes,
+                                              C.byref(p), C.byref(before), C.byref(after)))
         return p.as_dict(), before.value, after.value
 
     # ---- batched searches -----------------------------------------------

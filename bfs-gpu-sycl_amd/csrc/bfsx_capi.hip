@@ -50,6 +50,7 @@ void drop_loop_graph(bfsx_ctx *ctx);
 
 static void free_search_state(bfsx_ctx *ctx) {
   drop_loop_graph(ctx);  // the captured loop holds pointers into this state
+  ms_drop(ctx);
   for (DevBuf *b : {&ctx->levels, &ctx->parents, &ctx->vis, &ctx->fbm[0], &ctx->fbm[1],
                     &ctx->queue[0], &ctx->queue[1], &ctx->glb_scan, &ctx->glb_start,
                     &ctx->glb_tile, &ctx->glb_chunk_lo, &ctx->mark, &ctx->fsum, &ctx->bkt,
@@ -100,6 +101,10 @@ static bfsx_params default_params() {
   p.graph_levels = 8;
   p.mark_ratio_q10 = 16;
   p.bucket_ratio_q10 = 512;
+  p.ms_relabel = 1;
+  p.ms_push_divisor = 14;
+  p.ms_pull_hub = 256;
+  p.ms_graph_levels = 8;
   return p;
 }
 
@@ -113,6 +118,11 @@ static void validate_params(const bfsx_params &p) {
   if (p.glb_blocks_per_cu < 1 || p.glb_blocks_per_cu > 64) fail(BFSX_ERR_ARG, "glb_blocks_per_cu must be in [1,64]");
   if (p.glb_chunk < 256 || p.glb_chunk > (1u << 22)) fail(BFSX_ERR_ARG, "glb_chunk must be in [256, 2^22]");
   if (p.graph_levels < 1 || p.graph_levels > 64) fail(BFSX_ERR_ARG, "graph_levels must be in [1,64]");
+  if (p.ms_relabel > 1) fail(BFSX_ERR_ARG, "ms_relabel must be 0 or 1");
+  if (p.ms_push_divisor < 1 || p.ms_push_divisor > 1024) fail(BFSX_ERR_ARG, "ms_push_divisor must be in [1,1024]");
+  if (p.ms_pull_hub < 32 || p.ms_pull_hub > 65536) fail(BFSX_ERR_ARG, "ms_pull_hub must be in [32,65536]");
+  if (p.ms_graph_levels < 2 || p.ms_graph_levels > 64 || (p.ms_graph_levels & 1))
+    fail(BFSX_ERR_ARG, "ms_graph_levels must be even and in [2,64]");
 }
 
 // ---------------------------------------------------------------------------
@@ -149,7 +159,7 @@ static void view_from_ctl(bfsx_ctx *ctx, uint32_t produced) {
 // level end next to a new one means the new ones have not all arrived.
 // After a second without them, the stream is synchronised, which reports any
 // device error, and the values come from a full copy of the control block.
-static void read_ctl_after_level(bfsx_ctx *ctx, uint32_t seq, uint32_t produced) {
+void read_level_end(bfsx_ctx *ctx, uint32_t seq, uint32_t produced) {
   const volatile unsigned long long *w = ctx->h_mirror->w;
   const unsigned long long t32 = seq, t16 = seq & 0xFFFFu;
   const auto t0 = std::chrono::steady_clock::now();
@@ -380,7 +390,7 @@ static RunOut search_graph_loop(bfsx_ctx *ctx, uint32_t root, int fixed, bool le
     if (launches > max_launches) fail(BFSX_ERR_DEVICE, "captured level loop did not terminate");
     BFSX_HIP(hipGraphLaunch(exec, s));
     seq += K;
-    read_ctl_after_level(ctx, seq, 0);  // (K is even: the last level fills queue 0)
+    read_level_end(ctx, seq, 0);  // (K is even: the last level fills queue 0)
     if (ctx->view.done) break;
   }
   BFSX_HIP(hipEventRecord(ctx->ev1, s));
@@ -481,7 +491,7 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
     const bool new_fbm = w.launch(choice);
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
-    read_ctl_after_level(ctx, ++seq, cur ^ 1);
+    read_level_end(ctx, ++seq, cur ^ 1);
     // per-level records read the whole block (queue sizes, running totals,
     // scheduler inputs and outputs)
     if (timed) read_ctl(ctx);
@@ -1113,7 +1123,7 @@ int bfsx_run_multi(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits,
   return guarded(ctx, [&] {
     set_device(ctx);
     // the queue buffers and control block are shared with single searches
-    const MsResult r = ms_run(ctx, sources, bits, n_sources, n_bits, levels_out, 14);
+    const MsResult r = ms_run(ctx, sources, bits, n_sources, n_bits, levels_out);
     if (stats) {
       stats->ms = r.ms;
       stats->edges_count = r.edges;
@@ -1294,6 +1304,61 @@ int bfsx_autotune(bfsx_ctx *ctx, const uint32_t *roots, uint32_t n_roots, int st
     }
     ctx->params = best_p;
     drop_loop_graph(ctx);
+    if (best) *best = best_p;
+    if (gteps_before) *gteps_before = start_s;
+    if (gteps_after) *gteps_after = best_s;
+  });
+}
+
+int bfsx_autotune_multi(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits,
+                        uint32_t n_sources, uint32_t n_bits, uint32_t passes, bfsx_params *best,
+                        double *gteps_before, double *gteps_after) {
+  if (!ctx || !sources || !bits || n_sources == 0) return BFSX_ERR_ARG;
+  return guarded(ctx, [&] {
+    set_device(ctx);
+    if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
+    if (passes < 1 || passes > 4) fail(BFSX_ERR_ARG, "passes must be in [1,4]");
+    struct Knob {
+      uint32_t bfsx_params::*field;
+      std::vector<uint32_t> values;
+    };
+    const std::vector<Knob> knobs = {
+        {&bfsx_params::ms_relabel, {0, 1}},
+        {&bfsx_params::ms_push_divisor, {4, 8, 14, 20, 32}},
+        {&bfsx_params::ms_pull_hub, {64, 128, 256, 512, 1024}},
+        {&bfsx_params::ms_graph_levels, {2, 4, 8, 16}},
+    };
+    // aggregate throughput of the batch (best of two runs)
+    auto score = [&]() {
+      double t = 1e300;
+      unsigned long long e = 0;
+      for (int rep = 0; rep < 2; rep++) {
+        const MsResult r = ms_run(ctx, sources, bits, n_sources, n_bits, nullptr);
+        t = std::min(t, r.ms);
+        e = r.edges;
+      }
+      return t > 0 ? (double)e / (t * 1e-3) / 1e9 : 0.0;
+    };
+    bfsx_params best_p = ctx->params;
+    double best_s = score();
+    const double start_s = best_s;
+    for (uint32_t pass = 0; pass < passes; pass++) {
+      for (const Knob &k : knobs) {
+        for (uint32_t v : k.values) {
+          if (best_p.*(k.field) == v) continue;
+          bfsx_params p = best_p;
+          p.*(k.field) = v;
+          validate_params(p);
+          ctx->params = p;
+          const double sc = score();
+          if (sc > best_s * 1.01) {
+            best_s = sc;
+            best_p = p;
+          }
+        }
+      }
+    }
+    ctx->params = best_p;
     if (best) *best = best_p;
     if (gteps_before) *gteps_before = start_s;
     if (gteps_after) *gteps_after = best_s;

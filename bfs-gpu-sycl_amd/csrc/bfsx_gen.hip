@@ -20,6 +20,7 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 
 #include "bfsx_internal.hpp"
@@ -533,6 +534,110 @@ void root_candidates(bfsx_ctx *ctx, uint64_t seed, uint64_t first, uint32_t coun
                        ctx->col_ptr.as<uint32_t>(), ctx->n_rows, ctx->n_cols, seed, first,
                        count, d_comp, d_out_v, d_ok);
   BFSX_HIP(hipGetLastError());
+}
+
+// ---- degree relabelling (batched searches) ----------------------------------
+// new id i = rank of v when vertices are sorted by (degree descending, id
+// ascending); the relabelled lists are ascending in new ids, i.e. hub first.
+// On a power-law graph the hubs, which most lists reach, then sit in the
+// lowest ids, so the per-vertex bitsets a bottom-up level gathers from them
+// share a few L2-resident megabytes.
+template <typename PtrT>
+__global__ void k_relabel_degrees(const PtrT *col_ptr, const uint32_t *order, uint32_t n,
+                                  PtrT *d) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x)
+    d[i] = i < n ? col_ptr[order[i] + 1] - col_ptr[order[i]] : PtrT(0);
+}
+
+// key of edge k of old column c: (rank[c] << B) | rank[row_idx[k]]; a wave per column
+template <typename PtrT>
+__global__ void k_relabel_keys(const PtrT *col_ptr, uint32_t n, const uint32_t *row_idx,
+                               const uint32_t *rank, int B, uint64_t *keys) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t c = blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x >> 6); c < n;
+       c += nwaves) {
+    const PtrT b = col_ptr[c], e = col_ptr[c + 1];
+    const uint64_t hi = (uint64_t)rank[c] << B;
+    for (PtrT k = b + lane; k < e; k += 64) keys[k] = hi | rank[row_idx[k]];
+  }
+}
+
+__global__ void k_low_bits(const uint64_t *keys, uint64_t m, uint64_t mask, uint32_t *out) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < m;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    out[k] = (uint32_t)(keys[k] & mask);
+}
+
+template <typename PtrT>
+static void relabel_t(bfsx_ctx *ctx, uint32_t *rank, uint32_t *order, PtrT *new_cp,
+                      uint32_t *new_ri) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n = ctx->n_cols;
+  const uint64_t nnz = ctx->nnz;
+  const int B = bits_for(n) > 0 ? bits_for(n) : 1;
+  const uint64_t mask = (1ull << B) - 1;
+  const PtrT *cp = ctx->col_ptr.as<PtrT>();
+  DevBuf keys, sorted, tmp, deg;
+  struct Releaser {
+    DevBuf *b[4];
+    ~Releaser() {
+      for (DevBuf *x : b) release(*x);
+    }
+  } releaser{{&keys, &sorted, &tmp, &deg}};
+  // vertex order
+  ensure(keys, sizeof(uint64_t) * n, s);
+  ensure(sorted, sizeof(uint64_t) * n, s);
+  hipLaunchKernelGGL(k_degree_keys<PtrT>, dim3(grid_for(n)), dim3(256), 0, s, cp, n, n,
+                     ctx->max_degree, B, keys.as<uint64_t>());
+  size_t tbytes = 0;
+  const int vend = B + bits_for((uint64_t)ctx->max_degree + 1);
+  BFSX_HIP(rocprim::radix_sort_keys(nullptr, tbytes, keys.as<uint64_t>(), sorted.as<uint64_t>(),
+                                    (size_t)n, 0, vend, s));
+  ensure(tmp, tbytes, s);
+  BFSX_HIP(rocprim::radix_sort_keys(tmp.p, tbytes, keys.as<uint64_t>(), sorted.as<uint64_t>(),
+                                    (size_t)n, 0, vend, s));
+  hipLaunchKernelGGL(k_ranks, dim3(grid_for(n)), dim3(256), 0, s, sorted.as<uint64_t>(), n, mask,
+                     order, rank);
+  // relabelled col_ptr: exclusive scan of the degrees in the new order
+  ensure(deg, sizeof(PtrT) * ((uint64_t)n + 1), s);
+  hipLaunchKernelGGL(k_relabel_degrees<PtrT>, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, s,
+                     cp, order, n, deg.as<PtrT>());
+  tbytes = 0;
+  BFSX_HIP(rocprim::exclusive_scan(nullptr, tbytes, deg.as<PtrT>(), new_cp, PtrT(0),
+                                   (size_t)n + 1, rocprim::plus<PtrT>(), s));
+  ensure(tmp, tbytes, s);
+  BFSX_HIP(rocprim::exclusive_scan(tmp.p, tbytes, deg.as<PtrT>(), new_cp, PtrT(0),
+                                   (size_t)n + 1, rocprim::plus<PtrT>(), s));
+  BFSX_HIP(hipStreamSynchronize(s));
+  release(keys);
+  release(sorted);
+  // relabelled lists: sort the edges by (new column, new neighbour)
+  if (nnz) {
+    ensure(keys, sizeof(uint64_t) * nnz, s);
+    ensure(sorted, sizeof(uint64_t) * nnz, s);
+    hipLaunchKernelGGL(k_relabel_keys<PtrT>, dim3(grid_for((uint64_t)n * 64)), dim3(256), 0, s,
+                       cp, n, ctx->row_idx.as<uint32_t>(), rank, B, keys.as<uint64_t>());
+    tbytes = 0;
+    BFSX_HIP(rocprim::radix_sort_keys(nullptr, tbytes, keys.as<uint64_t>(),
+                                      sorted.as<uint64_t>(), (size_t)nnz, 0, 2 * B, s));
+    ensure(tmp, tbytes, s);
+    BFSX_HIP(rocprim::radix_sort_keys(tmp.p, tbytes, keys.as<uint64_t>(), sorted.as<uint64_t>(),
+                                      (size_t)nnz, 0, 2 * B, s));
+    hipLaunchKernelGGL(k_low_bits, dim3(grid_for(nnz)), dim3(256), 0, s, sorted.as<uint64_t>(),
+                       nnz, mask, new_ri);
+  }
+  BFSX_HIP(hipGetLastError());
+  BFSX_HIP(hipStreamSynchronize(s));
+}
+
+void relabel_by_degree(bfsx_ctx *ctx, uint32_t *rank, uint32_t *order, void *new_cp,
+                       uint32_t *new_ri) {
+  if (ctx->n_rows != ctx->n_cols) fail(BFSX_ERR_UNSUPPORTED, "relabelling needs n_rows == n_cols");
+  if (ctx->ptr_bytes == 8)
+    relabel_t<uint64_t>(ctx, rank, order, static_cast<uint64_t *>(new_cp), new_ri);
+  else
+    relabel_t<uint32_t>(ctx, rank, order, static_cast<uint32_t *>(new_cp), new_ri);
 }
 
 // Top-k by degree (scripts/rootNodes.py:133-137: sorted(G.degree, reverse)

@@ -41,6 +41,9 @@ struct LevelCtl {
   unsigned long long dec_EF;
   uint32_t dec_F, dec_pull, dec_valid;
   uint32_t decisions;  // network evaluations so far (sched_in/out hold the last)
+  // batched searches: hub-list length of a top-down level, and whether the
+  // previous level ran bottom-up (choice holds this level's direction)
+  uint32_t ms_hub_count, ms_prev_pull;
   uint32_t reserved;
   // Level ends so far on this context: only the level-end kernel changes it
   // (+1 each), so the tag it puts on the host-mapped words never repeats;
@@ -137,6 +140,13 @@ struct bfsx_ctx {
 
   // batched multi-source search state
   bfsx::DevBuf ms_seen, ms_cur, ms_nxt, ms_lv, ms_hubs, ms_phubs, ms_act;
+  // the graph relabelled by degree for batched searches (made on first use)
+  bfsx::DevBuf ms_rcp, ms_rri, ms_rank, ms_order;
+  bool ms_relabelled = false;
+  // captured level loop of the batched search, rebuilt when its key changes
+  hipGraph_t ms_graph = nullptr;
+  hipGraphExec_t ms_exec = nullptr;
+  uint64_t ms_key = 0;
 
   // captured level loop (BFSX_RUN_GRAPH_LOOP), rebuilt when its key changes
   hipGraph_t loop_graph = nullptr;
@@ -177,6 +187,12 @@ void order_adjacency(bfsx_ctx *ctx, bool hub_first, uint32_t *out);
 // fd[v] = {row_idx[col_ptr[v]] (~0u without neighbours), degree} for v < n_rows
 // (degrees below 2^32).
 void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint2 *fd);
+// The graph with vertices renamed by (degree descending, id ascending):
+// rank[v] = new id of v, order[i] = old id of new vertex i, new_cp / new_ri
+// the relabelled CSC (lists ascending in new ids), col_ptr of the graph's
+// width.  Needs n_rows == n_cols.
+void relabel_by_degree(bfsx_ctx *ctx, uint32_t *rank, uint32_t *order, void *new_cp,
+                       uint32_t *new_ri);
 // d_out[0..min(k, n_rows)) = vertices of largest out-degree, ties by id
 void top_degree_vertices(bfsx_ctx *ctx, uint32_t k, uint32_t *d_out);
 void root_candidates(bfsx_ctx *ctx, uint64_t seed, uint64_t first,
@@ -229,6 +245,10 @@ void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
 void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
                unsigned long long mark_min_E);
 void level_begin(bfsx_ctx *ctx);
+// Waits for the level end with sequence number `seq` (the tagged host-mapped
+// words) and leaves its values in ctx->view; `produced` = the queue buffer
+// the level filled (for the synchronising fallback).
+void read_level_end(bfsx_ctx *ctx, uint32_t seq, uint32_t produced);
 void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out);
 void level_histogram(bfsx_ctx *ctx, uint32_t max_levels, unsigned long long *d_counts,
                      unsigned long long *d_edges);
@@ -241,7 +261,8 @@ struct MsResult {
   unsigned long long edges = 0;
 };
 MsResult ms_run(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits, uint32_t n_src,
-                uint32_t n_bits, uint8_t *levels_out, uint32_t push_divisor);
+                uint32_t n_bits, uint8_t *levels_out);
+void ms_drop(bfsx_ctx *ctx);  // the relabelled graph and the captured loop
 void gen_rmat_pairs_offset(bfsx_ctx *ctx, int scale, uint64_t m, uint32_t tA, uint32_t tAB,
                            uint32_t tABC, uint64_t seed_graph, uint64_t seed_perm,
                            uint32_t offset, uint32_t *d_u, uint32_t *d_v);

@@ -16,6 +16,21 @@
 // queue with one atomic per wave.  A source set may hold several vertices
 // for one bit; the result is then the distance to the nearest of them, which
 // is how disjoint graphs packed into one CSC run in the same launch.
+//
+// The level loop is device-driven: K levels are captured once into a
+// hipGraph and replayed.  Each level's kernels read the frontier size, the
+// level and the direction from the control block; a one-wave kernel at the
+// start of a level picks the direction (top-down while the frontier's edges
+// times ms_push_divisor stay below nnz), one at the end counts, sets the
+// termination flag and writes the level-end words the host polls once per
+// launch.  After the last level every kernel of the remaining captured
+// levels returns at once.
+//
+// On symmetric graphs the searches run on a copy with vertices renamed by
+// degree (hubs first, made once per graph outside the timed region): a
+// bottom-up level gathers the 32-byte bitsets of neighbours, most of which
+// are hubs, and those then share a few L2-resident megabytes.  Sources are
+// renamed on the way in and levels on the way out.
 #include <algorithm>
 
 #include "bfsx_internal.hpp"
@@ -52,22 +67,37 @@ __device__ __forceinline__ void ms_write_levels(uint8_t *lv, uint32_t NB, uint32
 
 struct MsArgs {
   const void *col_ptr;
-  const uint32_t *row_idx;
+  const uint32_t *row_idx;       // lists of top-down levels
+  const uint32_t *row_idx_pull;  // lists of bottom-up levels (hub first)
   uint32_t n_rows, n_cols;
   uint32_t *seen, *cur, *nxt;
   uint8_t *lv;
   uint32_t *q_in, *q_out, *q_count_out;
-  uint32_t F;
-  uint32_t level;  // level being written (the new bits' level)
+  uint32_t qi;     // queue buffer holding this level's active vertices
+  uint32_t F;      // (set from the control block at kernel start)
+  uint32_t level;  // level being written (the new bits' level), likewise
   uint32_t *overflow;
   unsigned long long *q_edges_out;  // out-degree sum of the next active queue
-  // one bit per vertex: in some search's frontier (cur != 0); act_cur is
-  // null on the first level.  Bottom-up levels probe it (a few MiB, L2
-  // resident) before reading a neighbour's bitset, whose 32 bytes cost a
-  // whole line.
+  uint32_t pull_hub;  // bottom-up: longer lists go to k_ms_pull_hubs
+  LevelCtl *ctl;
+  // one bit per vertex: in some search's frontier (cur != 0); not used on
+  // the first level.  Bottom-up levels probe it (a few MiB, L2 resident)
+  // before reading a neighbour's bitset, whose 32 bytes cost a whole line.
   const uint32_t *act_cur;
   uint32_t *act_next;
 };
+
+// Reads this level's state from the control block into a; false when the
+// search has ended or the level runs in the other direction (uniform over
+// the launch, so every thread returns together).
+__device__ __forceinline__ bool ms_level(MsArgs &a, uint32_t pull) {
+  const LevelCtl *c = a.ctl;
+  if (c->done || c->choice != pull) return false;
+  a.F = c->q_count[a.qi];
+  a.level = c->level + 1;
+  if (c->level == 0) a.act_cur = nullptr;
+  return true;
+}
 
 // Sets the act_next bits of the G = 64 / W consecutive vertices of a wave
 // (first one u0) whose group found new bits (m: ballot of nb != 0).
@@ -133,10 +163,9 @@ __device__ __forceinline__ void ms_track(const PtrT *col_ptr, uint32_t n_rows, b
 // Active vertices with more than MS_HUB edges go to a list that every wave
 // then splits (k_ms_push_hubs); the rest are expanded by one wave each.
 constexpr uint32_t MS_HUB = 2048;
-// Bottom-up levels hand vertices with more neighbours than this to
+// Bottom-up levels hand vertices with more neighbours than a.pull_hub to
 // k_ms_pull_hubs (a work-group each): a W-lane group scanning a long list
 // alone would hold its wave for hundreds of rounds.
-constexpr uint32_t MS_PULL_HUB = 256;
 
 template <typename PtrT, int W>
 __device__ __forceinline__ void ms_push_edges(const MsArgs &a, const PtrT *col_ptr, uint32_t v,
@@ -154,7 +183,9 @@ __device__ __forceinline__ void ms_push_edges(const MsArgs &a, const PtrT *col_p
 }
 
 template <typename PtrT, int W>
-__global__ __launch_bounds__(256) void k_ms_push(MsArgs a, uint32_t *hubs, uint32_t *n_hubs) {
+__global__ __launch_bounds__(256) void k_ms_push(MsArgs a, uint32_t *hubs) {
+  if (!ms_level(a, 0)) return;
+  uint32_t *n_hubs = &a.ctl->ms_hub_count;
   // a group of W lanes per edge: lane j handles word j
   const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
   constexpr uint32_t G = 64 / W;  // groups per wave
@@ -173,21 +204,24 @@ __global__ __launch_bounds__(256) void k_ms_push(MsArgs a, uint32_t *hubs, uint3
 }
 
 template <typename PtrT, int W>
-__global__ __launch_bounds__(256) void k_ms_push_hubs(MsArgs a, const uint32_t *hubs,
-                                                      const uint32_t *n_hubs) {
+__global__ __launch_bounds__(256) void k_ms_push_hubs(MsArgs a, const uint32_t *hubs) {
+  if (!ms_level(a, 0)) return;
   const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
   constexpr uint32_t G = 64 / W;
   const uint32_t grp = (threadIdx.x & 63) / W;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t nwaves = gridDim.x * 4;
-  const uint32_t nh = *n_hubs;
+  const uint32_t nh = a.ctl->ms_hub_count;
   for (uint32_t h = 0; h < nh; h++)
     ms_push_edges<PtrT, W>(a, col_ptr, hubs[h], (PtrT)(wave * G + grp), (PtrT)nwaves * G);
 }
 
 // ---- after a top-down level: keep unseen bits, levels, next active queue ----
+// Also clears this level's frontier words (cur), which become the next
+// level's nxt: a following top-down level finds them zero.
 template <typename PtrT, int W>
 __global__ __launch_bounds__(256) void k_ms_finalize(MsArgs a, uint32_t n_groups_total) {
+  if (!ms_level(a, 0)) return;
   const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
   constexpr uint32_t G = 64 / W;
   const uint32_t lane = threadIdx.x & 63, j = lane % W, grp = lane / W;
@@ -205,6 +239,7 @@ __global__ __launch_bounds__(256) void k_ms_finalize(MsArgs a, uint32_t n_groups
         if (nb) a.seen[o] = s | nb;
         a.nxt[o] = nb;
       }
+      if (a.cur[o]) a.cur[o] = 0u;
     }
     if (nb) {
       if (a.level > 254) atomicOr(a.overflow, 1u);
@@ -222,6 +257,8 @@ __global__ __launch_bounds__(256) void k_ms_finalize(MsArgs a, uint32_t n_groups
 // ---- bottom-up: every vertex with unseen bits ORs its neighbours' frontier ----
 template <typename PtrT, int W>
 __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_total) {
+  if (!ms_level(a, 1)) return;
+  a.row_idx = a.row_idx_pull;
   const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
   constexpr uint32_t G = 64 / W;
   const uint32_t lane = threadIdx.x & 63, j = lane % W, grp = lane / W;
@@ -237,7 +274,7 @@ __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_tot
     // skips; vertices with more than MS_PULL_HUB neighbours are k_ms_pull_hubs'
     const bool row = valid && u < a.n_rows;
     const PtrT b = row ? col_ptr[u] : PtrT(0), e = row ? col_ptr[u + 1] : PtrT(0);
-    const bool hub = (uint64_t)(e - b) > MS_PULL_HUB;
+    const bool hub = (uint64_t)(e - b) > a.pull_hub;
     bool todo = row && !hub && s != ~0u;
     const bool group_todo = (__ballot(todo) & gmask) != 0;
     uint32_t acc = 0;
@@ -282,6 +319,8 @@ __global__ __launch_bounds__(256) void k_ms_pull(MsArgs a, uint32_t n_groups_tot
 // round as in k_ms_pull), partial words ORed in LDS.
 template <typename PtrT, int W>
 __global__ __launch_bounds__(256) void k_ms_pull_hubs(MsArgs a, const uint32_t *hubs) {
+  if (!ms_level(a, 1)) return;
+  a.row_idx = a.row_idx_pull;
   const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
   constexpr uint32_t NG = 256 / W;  // groups per work-group
   constexpr int PER = 8 / W;
@@ -338,13 +377,60 @@ __global__ __launch_bounds__(256) void k_ms_pull_hubs(MsArgs a, const uint32_t *
   }
 }
 
-// vertices with more than MS_PULL_HUB neighbours (once per batch, outside the timer)
+// vertices with more than pull_hub neighbours (once per batch, outside the timer)
 template <typename PtrT>
-__global__ void k_ms_hub_list(const PtrT *col_ptr, uint32_t n_rows, uint32_t *hubs,
-                              uint32_t *n_hubs) {
+__global__ void k_ms_hub_list(const PtrT *col_ptr, uint32_t n_rows, uint32_t pull_hub,
+                              uint32_t *hubs, uint32_t *n_hubs) {
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_rows;
        u += gridDim.x * blockDim.x)
-    if ((uint64_t)(col_ptr[u + 1] - col_ptr[u]) > MS_PULL_HUB) hubs[atomicAdd(n_hubs, 1u)] = u;
+    if ((uint64_t)(col_ptr[u + 1] - col_ptr[u]) > pull_hub) hubs[atomicAdd(n_hubs, 1u)] = u;
+}
+
+// ---- level start and end (one wave each) -------------------------------------
+// Direction of the level, reset of the output queue's counters.
+__global__ void k_ms_begin(LevelCtl *ctl, uint32_t qi, uint64_t nnz, uint32_t divisor,
+                           int symmetric) {
+  if (threadIdx.x != 0 || ctl->done) return;
+  const uint32_t qo = qi ^ 1;
+  const bool push = !symmetric || ctl->q_edges[qi] * (unsigned long long)divisor < nnz;
+  ctl->ms_prev_pull = ctl->choice;
+  ctl->choice = push ? 0u : 1u;
+  ctl->ms_hub_count = 0;
+  ctl->q_count[qo] = 0;
+  ctl->q_edges[qo] = 0;
+}
+
+// Clears the next-active map, and the nxt words when a top-down level
+// follows a bottom-up one (a bottom-up level writes every nxt word; a
+// top-down level's finalize leaves the next top-down level's nxt zero).
+__global__ void k_ms_clear(const LevelCtl *ctl, uint32_t *act_next, size_t act_words,
+                           uint32_t *nxt, size_t nxt_words) {
+  if (ctl->done) return;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  for (size_t i = i0; i < act_words; i += stride) act_next[i] = 0u;
+  if (ctl->choice == 0 && ctl->ms_prev_pull)
+    for (size_t i = i0; i < nxt_words; i += stride) nxt[i] = 0u;
+}
+
+// Counts, termination, and the tagged level-end words for the host.
+__global__ void k_ms_end(LevelCtl *ctl, uint32_t qi, LevelMirror *mirror) {
+  if (threadIdx.x != 0) return;
+  const uint32_t qo = qi ^ 1;
+  if (!ctl->done) {
+    ctl->kind_count[ctl->choice ? 1 : 0]++;
+    ctl->level += 1;
+    ctl->done = ctl->q_count[qo] == 0;
+  }
+  const uint32_t seq = ctl->seq + 1;
+  ctl->seq = seq;
+  const unsigned long long t32 = (unsigned long long)seq << 32;
+  const unsigned long long t16 = (unsigned long long)(seq & 0xFFFFu) << 48;
+  mirror->w[0] = t32 | ctl->q_count[qo];
+  mirror->w[1] = t16 | (ctl->q_edges[qo] & MIRROR_48);
+  mirror->w[2] = t16 | (ctl->overflow ? 1ull : 0ull);
+  mirror->w[3] = t32 | (ctl->choice & 0xFFu) | ((unsigned long long)(ctl->done ? 1u : 0u) << 8);
+  __threadfence_system();
 }
 
 // ---- set-up ------------------------------------------------------------------
@@ -423,6 +509,34 @@ __global__ void k_ms_edges(const PtrT *col_ptr, const uint32_t *seen, uint32_t n
   if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
 }
 
+// ---- renaming sources and levels ---------------------------------------------
+__global__ void k_ms_map(uint32_t *v, uint32_t n, const uint32_t *rank) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = rank[v[i]];
+}
+
+// lv_out[v] = lv[rank[v]] (NB bytes per vertex, as 16-byte words)
+__global__ void k_ms_unrename(const uint4 *lv, uint32_t words_per_vertex, uint32_t n,
+                              const uint32_t *rank, uint4 *lv_out) {
+  const uint64_t total = (uint64_t)n * words_per_vertex;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = i / words_per_vertex, k = i % words_per_vertex;
+    lv_out[i] = lv[(uint64_t)rank[v] * words_per_vertex + k];
+  }
+}
+
+// sum of out-degree of the distinct sources (the first level's frontier edges)
+template <typename PtrT>
+__global__ void k_ms_src_edges(const PtrT *col_ptr, uint32_t n_rows, const uint32_t *q,
+                               uint32_t F, unsigned long long *out) {
+  unsigned long long d = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F; i += gridDim.x * blockDim.x)
+    if (q[i] < n_rows) d += (unsigned long long)(col_ptr[q[i] + 1] - col_ptr[q[i]]);
+  for (int k = 32; k > 0; k >>= 1) d += __shfl_xor(d, k, 64);
+  if ((threadIdx.x & 63) == 0 && d) atomicAdd(out, d);
+}
+
 // ---- host side ------------------------------------------------------------------
 static unsigned ms_grid(uint64_t items, unsigned per_block, unsigned cap) {
   uint64_t g = (items + per_block - 1) / per_block;
@@ -430,36 +544,102 @@ static unsigned ms_grid(uint64_t items, unsigned per_block, unsigned cap) {
   return g > cap ? cap : (unsigned)g;
 }
 
+static void ms_drop_loop(bfsx_ctx *ctx) {
+  if (ctx->ms_exec) (void)hipGraphExecDestroy(ctx->ms_exec);
+  if (ctx->ms_graph) (void)hipGraphDestroy(ctx->ms_graph);
+  ctx->ms_exec = nullptr;
+  ctx->ms_graph = nullptr;
+  ctx->ms_key = 0;
+}
+
+void ms_drop(bfsx_ctx *ctx) {
+  ms_drop_loop(ctx);
+  for (DevBuf *b : {&ctx->ms_rcp, &ctx->ms_rri, &ctx->ms_rank, &ctx->ms_order}) release(*b);
+  ctx->ms_relabelled = false;
+}
+
+// The graph the batch searches: the relabelled copy (made on first use) or
+// the resident one.
+struct MsGraph {
+  const void *col_ptr;
+  const uint32_t *row_idx, *row_idx_pull;
+  const uint32_t *rank;  // old id -> searched id, or null
+  const uint32_t *order; // searched id -> old id, or null
+};
+
+static MsGraph ms_graph_of(bfsx_ctx *ctx, bool symmetric) {
+  MsGraph g{ctx->col_ptr.p, ctx->row_idx.as<uint32_t>(), ctx->row_idx.as<uint32_t>(), nullptr,
+            nullptr};
+  if (ctx->row_idx_pull.p) g.row_idx_pull = ctx->row_idx_pull.as<uint32_t>();
+  if (!symmetric || !ctx->params.ms_relabel) return g;
+  if (!ctx->ms_relabelled) {
+    hipStream_t s = ctx->stream;
+    const uint64_t n = ctx->n_cols;
+    ensure(ctx->ms_rank, sizeof(uint32_t) * std::max<uint64_t>(n, 1), s);
+    ensure(ctx->ms_order, sizeof(uint32_t) * std::max<uint64_t>(n, 1), s);
+    ensure(ctx->ms_rcp, (size_t)ctx->ptr_bytes * (n + 1), s);
+    ensure(ctx->ms_rri, sizeof(uint32_t) * std::max<uint64_t>(ctx->nnz, 1), s);
+    relabel_by_degree(ctx, ctx->ms_rank.as<uint32_t>(), ctx->ms_order.as<uint32_t>(),
+                      ctx->ms_rcp.p, ctx->ms_rri.as<uint32_t>());
+    ctx->ms_relabelled = true;
+  }
+  // relabelled lists ascend in new ids, which puts hubs first: one copy
+  // serves both directions
+  g.col_ptr = ctx->ms_rcp.p;
+  g.row_idx = g.row_idx_pull = ctx->ms_rri.as<uint32_t>();
+  g.rank = ctx->ms_rank.as<uint32_t>();
+  g.order = ctx->ms_order.as<uint32_t>();
+  return g;
+}
+
 template <typename PtrT, int W>
 static MsResult ms_run_t(bfsx_ctx *ctx, const uint32_t *d_src, const uint16_t *d_bit,
                          uint32_t n_src, const std::vector<uint32_t> &distinct_src,
-                         uint32_t n_bits, uint8_t *levels_out, uint32_t push_divisor) {
+                         uint32_t n_bits, uint8_t *levels_out) {
   hipStream_t s = ctx->stream;
   const uint64_t n = ctx->n_cols;
   const uint32_t NB = 32 * W;
+  const bool symmetric = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols;
+  const MsGraph G = ms_graph_of(ctx, symmetric);
+  const PtrT *gcp = static_cast<const PtrT *>(G.col_ptr);
   ensure(ctx->ms_seen, sizeof(uint32_t) * n * W, s);
   ensure(ctx->ms_cur, sizeof(uint32_t) * n * W, s);
   ensure(ctx->ms_nxt, sizeof(uint32_t) * n * W, s);
   ensure(ctx->ms_lv, (size_t)n * NB, s);
   ensure(ctx->ms_hubs, sizeof(uint32_t) * (n + 1), s);
-  ensure(ctx->ms_act, sizeof(uint32_t) * 2 * (n / 32 + 2), s);
+  const size_t act_words = n / 32 + 2;
+  ensure(ctx->ms_act, sizeof(uint32_t) * 2 * act_words, s);
   ensure(ctx->scratch, 64, s);
-  const bool symmetric = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols;
   uint32_t *seen = ctx->ms_seen.as<uint32_t>();
-  uint32_t *cur = ctx->ms_cur.as<uint32_t>();
-  uint32_t *nxt = ctx->ms_nxt.as<uint32_t>();
+  uint32_t *buf[2] = {ctx->ms_cur.as<uint32_t>(), ctx->ms_nxt.as<uint32_t>()};
+  uint32_t *act[2] = {ctx->ms_act.as<uint32_t>(), ctx->ms_act.as<uint32_t>() + act_words};
   uint8_t *lv = ctx->ms_lv.as<uint8_t>();
   LevelCtl *ctl = ctx->ctl.as<LevelCtl>();
   const unsigned cap = (unsigned)ctx->cu_count * 16;
-  // initial state (outside the timer)
+  const uint32_t pull_hub = ctx->params.ms_pull_hub;
+
+  // initial state (outside the timer): sources renamed, bitsets, levels,
+  // the first queue and its edge count, the control block (its level-end
+  // counter kept)
+  uint32_t *src = const_cast<uint32_t *>(d_src);
+  uint32_t *q0 = ctx->queue[0].as<uint32_t>();
+  const uint32_t F0 = (uint32_t)distinct_src.size();
+  BFSX_HIP(hipMemcpyAsync(q0, distinct_src.data(), sizeof(uint32_t) * F0, hipMemcpyHostToDevice, s));
+  if (G.rank) {
+    hipLaunchKernelGGL(k_ms_map, dim3((n_src + 255) / 256), dim3(256), 0, s, src, n_src, G.rank);
+    hipLaunchKernelGGL(k_ms_map, dim3((F0 + 255) / 256), dim3(256), 0, s, q0, F0, G.rank);
+  }
   hipLaunchKernelGGL(k_ms_init_words<W>, dim3(ms_grid(n * W, 256, cap)), dim3(256), 0, s, seen,
-                     cur, nxt, (uint32_t)n, n_bits);
+                     buf[0], buf[1], (uint32_t)n, n_bits);
   BFSX_HIP(hipMemsetAsync(lv, 0xFF, (size_t)n * NB, s));
-  hipLaunchKernelGGL(k_ms_sources<W>, dim3((n_src + 255) / 256), dim3(256), 0, s, d_src, d_bit,
-                     n_src, seen, cur, lv);
-  BFSX_HIP(hipMemcpyAsync(ctx->queue[0].p, distinct_src.data(),
-                          sizeof(uint32_t) * distinct_src.size(), hipMemcpyHostToDevice, s));
-  BFSX_HIP(hipMemsetAsync(ctl, 0, CTL_RESET_BYTES, s));  // (keeps the level-end counter)
+  hipLaunchKernelGGL(k_ms_sources<W>, dim3((n_src + 255) / 256), dim3(256), 0, s, src, d_bit,
+                     n_src, seen, buf[0], lv);
+  hipLaunchKernelGGL(k_fill_u32, dim3(ms_grid(2 * act_words, 256, cap)), dim3(256), 0, s,
+                     act[0], 2 * act_words, 0u);
+  BFSX_HIP(hipMemsetAsync(ctl, 0, CTL_RESET_BYTES, s));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, &ctl->q_count[0], (size_t)1, F0);
+  hipLaunchKernelGGL((k_ms_src_edges<PtrT>), dim3(ms_grid(F0, 256, 64)), dim3(256), 0, s, gcp,
+                     ctx->n_rows, q0, F0, &ctl->q_edges[0]);
   // vertices that bottom-up levels hand to k_ms_pull_hubs
   uint32_t n_phubs = 0;
   if (symmetric) {
@@ -467,111 +647,142 @@ static MsResult ms_run_t(bfsx_ctx *ctx, const uint32_t *d_src, const uint16_t *d
     uint32_t *d_np = ctx->scratch.as<uint32_t>() + 12;
     BFSX_HIP(hipMemsetAsync(d_np, 0, sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_ms_hub_list<PtrT>, dim3(ms_grid(ctx->n_rows, 256, cap)), dim3(256), 0, s,
-                       static_cast<const PtrT *>(ctx->col_ptr.p), ctx->n_rows,
-                       ctx->ms_phubs.as<uint32_t>(), d_np);
+                       gcp, ctx->n_rows, pull_hub, ctx->ms_phubs.as<uint32_t>(), d_np);
     BFSX_HIP(hipGetLastError());
     BFSX_HIP(hipMemcpyAsync(&n_phubs, d_np, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   }
+  BFSX_HIP(hipGetLastError());
   BFSX_HIP(hipStreamSynchronize(s));
 
-  MsResult r;
-  BFSX_HIP(hipEventRecord(ctx->ev0, s));
-  uint32_t F = (uint32_t)distinct_src.size(), qi = 0, level = 0;
-  const double avg_deg = ctx->n_rows ? (double)ctx->nnz / ctx->n_rows : 0.0;
-  double f_edges = F * avg_deg;  // first level: estimate; then measured
-  uint32_t *hubs = ctx->ms_hubs.as<uint32_t>();
-  uint32_t *n_hubs = ctx->scratch.as<uint32_t>() + 8;
+  // K device-driven levels, captured once per shape
+  const uint32_t K = ctx->params.ms_graph_levels;
   const uint32_t n_groups = (uint32_t)((n + (64 / W) - 1) / (64 / W));
-  const size_t act_words = n / 32 + 2;
-  uint32_t *act[2] = {ctx->ms_act.as<uint32_t>(), ctx->ms_act.as<uint32_t>() + act_words};
-  int ai = 0;  // act[ai] receives the next level's active vertices
-  while (F > 0) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(ms_grid(act_words, 256, cap)), dim3(256), 0, s, act[ai],
-                       act_words, 0u);
-    MsArgs a{};
-    a.act_cur = level == 0 ? nullptr : act[ai ^ 1];
-    a.act_next = act[ai];
-    a.col_ptr = ctx->col_ptr.p;
-    a.row_idx = ctx->row_idx.as<uint32_t>();
-    a.n_rows = ctx->n_rows;
-    a.n_cols = (uint32_t)n;
-    a.seen = seen;
-    a.cur = cur;
-    a.nxt = nxt;
-    a.lv = lv;
-    a.q_in = ctx->queue[qi].as<uint32_t>();
-    a.q_out = ctx->queue[qi ^ 1].as<uint32_t>();
-    a.q_count_out = &ctl->q_count[qi ^ 1];
-    a.F = F;
-    a.level = level + 1;
-    a.overflow = &ctl->overflow;
-    a.q_edges_out = &ctl->q_edges[qi ^ 1];
-    const bool push = !symmetric || f_edges * push_divisor < (double)ctx->nnz;
-    if (!push && ctx->row_idx_pull.p) a.row_idx = ctx->row_idx_pull.as<uint32_t>();
-    if (push) {
-      hipLaunchKernelGGL(k_fill_u32, dim3(ms_grid(n * W, 256, cap)), dim3(256), 0, s, nxt,
-                         (size_t)(n * W), 0u);
-      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, n_hubs, (size_t)1, 0u);
-      hipLaunchKernelGGL((k_ms_push<PtrT, W>), dim3(ms_grid((uint64_t)F * 64, 256, cap)),
-                         dim3(256), 0, s, a, hubs, n_hubs);
-      hipLaunchKernelGGL((k_ms_push_hubs<PtrT, W>), dim3(cap), dim3(256), 0, s, a,
-                         (const uint32_t *)hubs, (const uint32_t *)n_hubs);
-      hipLaunchKernelGGL((k_ms_finalize<PtrT, W>), dim3(ms_grid((uint64_t)n_groups * 64, 256, cap)),
-                         dim3(256), 0, s, a, n_groups);
-      r.push_levels++;
-    } else {
-      hipLaunchKernelGGL((k_ms_pull<PtrT, W>), dim3(ms_grid((uint64_t)n_groups * 64, 256, cap)),
-                         dim3(256), 0, s, a, n_groups);
-      if (n_phubs)
-        hipLaunchKernelGGL((k_ms_pull_hubs<PtrT, W>), dim3(n_phubs), dim3(256), 0, s, a,
-                           (const uint32_t *)ctx->ms_phubs.as<uint32_t>());
-      r.pull_levels++;
+  uint64_t key = 1469598103934665603ull;
+  for (uint64_t v : {(uint64_t)W, (uint64_t)sizeof(PtrT), (uint64_t)K, (uint64_t)n,
+                     (uint64_t)n_phubs, (uint64_t)pull_hub, (uint64_t)ctx->params.ms_push_divisor,
+                     (uint64_t)symmetric, (uint64_t)(uintptr_t)G.col_ptr,
+                     (uint64_t)(uintptr_t)G.row_idx_pull, (uint64_t)(uintptr_t)seen,
+                     (uint64_t)(uintptr_t)buf[0], (uint64_t)(uintptr_t)buf[1],
+                     (uint64_t)(uintptr_t)lv, (uint64_t)(uintptr_t)act[0],
+                     (uint64_t)(uintptr_t)ctx->ms_hubs.p, (uint64_t)(uintptr_t)ctx->ms_phubs.p,
+                     (uint64_t)(uintptr_t)ctx->queue[0].p, (uint64_t)(uintptr_t)ctx->queue[1].p})
+    key = (key ^ v) * 1099511628211ull;
+  if (!ctx->ms_exec || ctx->ms_key != key) {
+    ms_drop_loop(ctx);
+    BFSX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      for (uint32_t k = 0; k < K; k++) {
+        const uint32_t p = k & 1u;
+        MsArgs a{};
+        a.col_ptr = G.col_ptr;
+        a.row_idx = G.row_idx;
+        a.row_idx_pull = G.row_idx_pull;
+        a.n_rows = ctx->n_rows;
+        a.n_cols = (uint32_t)n;
+        a.seen = seen;
+        a.cur = buf[p];
+        a.nxt = buf[p ^ 1];
+        a.lv = lv;
+        a.qi = p;
+        a.q_in = ctx->queue[p].as<uint32_t>();
+        a.q_out = ctx->queue[p ^ 1].as<uint32_t>();
+        a.q_count_out = &ctl->q_count[p ^ 1];
+        a.q_edges_out = &ctl->q_edges[p ^ 1];
+        a.overflow = &ctl->overflow;
+        a.pull_hub = pull_hub;
+        a.ctl = ctl;
+        a.act_cur = act[p ^ 1];
+        a.act_next = act[p];
+        hipLaunchKernelGGL(k_ms_begin, dim3(1), dim3(64), 0, s, ctl, p, (uint64_t)ctx->nnz,
+                           ctx->params.ms_push_divisor, symmetric ? 1 : 0);
+        hipLaunchKernelGGL(k_ms_clear, dim3(cap), dim3(256), 0, s, (const LevelCtl *)ctl, act[p],
+                           act_words, buf[p ^ 1], (size_t)(n * W));
+        hipLaunchKernelGGL((k_ms_push<PtrT, W>), dim3(cap), dim3(256), 0, s, a,
+                           ctx->ms_hubs.as<uint32_t>());
+        hipLaunchKernelGGL((k_ms_push_hubs<PtrT, W>), dim3(cap), dim3(256), 0, s, a,
+                           (const uint32_t *)ctx->ms_hubs.as<uint32_t>());
+        hipLaunchKernelGGL((k_ms_finalize<PtrT, W>), dim3(ms_grid((uint64_t)n_groups * 64, 256, cap)),
+                           dim3(256), 0, s, a, n_groups);
+        if (symmetric) {
+          hipLaunchKernelGGL((k_ms_pull<PtrT, W>), dim3(ms_grid((uint64_t)n_groups * 64, 256, cap)),
+                             dim3(256), 0, s, a, n_groups);
+          if (n_phubs)
+            hipLaunchKernelGGL((k_ms_pull_hubs<PtrT, W>), dim3(n_phubs), dim3(256), 0, s, a,
+                               (const uint32_t *)ctx->ms_phubs.as<uint32_t>());
+        }
+        hipLaunchKernelGGL(k_ms_end, dim3(1), dim3(64), 0, s, ctl, p, ctx->d_mirror);
+      }
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
     }
-    BFSX_HIP(hipGetLastError());
-    // next frontier = nxt; reset the consumed queue counter and degree sum
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, &ctl->q_count[qi], (size_t)1, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s,
-                       reinterpret_cast<uint32_t *>(&ctl->q_edges[qi]), (size_t)2, 0u);
-    BFSX_HIP(hipMemcpyAsync(ctx->h_ctl, ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, s));
-    BFSX_HIP(hipStreamSynchronize(s));
-    if (ctx->h_ctl->overflow) fail(BFSX_ERR_OVERFLOW, "a level above 254 does not fit uint8 levels");
-    F = ctx->h_ctl->q_count[qi ^ 1];
-    f_edges = (double)ctx->h_ctl->q_edges[qi ^ 1];
-    std::swap(cur, nxt);
-    qi ^= 1;
-    ai ^= 1;
-    level++;
-    r.n_levels++;
+    BFSX_HIP(hipStreamEndCapture(s, &ctx->ms_graph));
+    BFSX_HIP(hipGraphInstantiate(&ctx->ms_exec, ctx->ms_graph, nullptr, nullptr, 0));
+    ctx->ms_key = key;
+  }
+
+  // the timed level loop: one graph launch per K levels, the termination
+  // flag read from the tagged level-end words
+  MsResult r;
+  BFSX_HIP(hipMemcpyAsync(ctx->h_ctl, ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, s));
+  BFSX_HIP(hipStreamSynchronize(s));
+  uint32_t seq = ctx->h_ctl->seq;
+  BFSX_HIP(hipEventRecord(ctx->ev0, s));
+  const uint64_t max_launches = n / K + 4;  // a search has at most n levels
+  for (uint64_t launches = 0;; launches++) {
+    if (launches > max_launches) fail(BFSX_ERR_DEVICE, "batched level loop did not terminate");
+    BFSX_HIP(hipGraphLaunch(ctx->ms_exec, s));
+    seq += K;
+    read_level_end(ctx, seq, 0);
+    if (ctx->view.done) break;
   }
   BFSX_HIP(hipEventRecord(ctx->ev1, s));
   BFSX_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0;
   BFSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   r.ms = ms;
+  BFSX_HIP(hipMemcpyAsync(ctx->h_ctl, ctl, sizeof(LevelCtl), hipMemcpyDeviceToHost, s));
+  BFSX_HIP(hipStreamSynchronize(s));
+  if (ctx->h_ctl->overflow) fail(BFSX_ERR_OVERFLOW, "a level above 254 does not fit uint8 levels");
+  r.push_levels = ctx->h_ctl->kind_count[0];
+  r.pull_levels = ctx->h_ctl->kind_count[1];
+  r.n_levels = r.push_levels + r.pull_levels;
+
   // edges summed over searches, then the per-search level rows
   unsigned long long *d_e = ctx->scratch.as<unsigned long long>();
   BFSX_HIP(hipMemsetAsync(d_e, 0, sizeof(unsigned long long), s));
   hipLaunchKernelGGL((k_ms_edges<PtrT, W>), dim3(ms_grid(ctx->n_rows, 256, cap)), dim3(256), 0, s,
-                     static_cast<const PtrT *>(ctx->col_ptr.p), seen, ctx->n_rows, n_bits, d_e);
+                     gcp, seen, ctx->n_rows, n_bits, d_e);
   unsigned long long edges = 0;
   BFSX_HIP(hipMemcpyAsync(&edges, d_e, sizeof(edges), hipMemcpyDeviceToHost, s));
   BFSX_HIP(hipStreamSynchronize(s));
   r.edges = edges;
   if (levels_out) {
-    DevBuf out;
+    DevBuf out, plain;
+    const uint8_t *rows = lv;
+    if (G.rank) {  // back to the caller's vertex ids
+      ensure(plain, (size_t)n * NB, s);
+      hipLaunchKernelGGL(k_ms_unrename, dim3(ms_grid(n * (NB / 16), 256, cap)), dim3(256), 0, s,
+                         reinterpret_cast<const uint4 *>(lv), NB / 16, (uint32_t)n, G.rank,
+                         plain.as<uint4>());
+      rows = plain.as<uint8_t>();
+    }
     ensure(out, (size_t)n_bits * n, s);
-    hipLaunchKernelGGL(k_ms_transpose, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, lv, NB,
+    hipLaunchKernelGGL(k_ms_transpose, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, rows, NB,
                        n_bits, (uint32_t)n, out.as<uint8_t>());
     BFSX_HIP(hipGetLastError());
     BFSX_HIP(hipMemcpyAsync(levels_out, out.p, (size_t)n_bits * n, hipMemcpyDeviceToHost, s));
     BFSX_HIP(hipStreamSynchronize(s));
     release(out);
+    release(plain);
   }
   return r;
 }
 
 MsResult ms_run(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits, uint32_t n_src,
-                uint32_t n_bits, uint8_t *levels_out, uint32_t push_divisor) {
+                uint32_t n_bits, uint8_t *levels_out) {
   if (!ctx->has_graph) fail(BFSX_ERR_STATE, "no graph uploaded");
   if (n_bits == 0 || n_bits > 256) fail(BFSX_ERR_ARG, "n_bits must be in [1, 256]");
   if (n_src == 0) fail(BFSX_ERR_ARG, "no sources");
@@ -597,9 +808,9 @@ MsResult ms_run(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits, ui
 #define BFSX_MS_CASE(WV)                                                                    \
   case WV:                                                                                  \
     r = wide ? ms_run_t<uint64_t, WV>(ctx, ds.as<uint32_t>(), db.as<uint16_t>(), n_src,     \
-                                      distinct, n_bits, levels_out, push_divisor)           \
+                                      distinct, n_bits, levels_out)                         \
              : ms_run_t<uint32_t, WV>(ctx, ds.as<uint32_t>(), db.as<uint16_t>(), n_src,     \
-                                      distinct, n_bits, levels_out, push_divisor);          \
+                                      distinct, n_bits, levels_out);                        \
     break;
     switch (W) {
       BFSX_MS_CASE(1)

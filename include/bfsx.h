@@ -151,6 +151,15 @@ typedef struct bfsx_params {
                                (0 = always, 0xFFFFFFFF = never; not with
                                mark_ratio_q10 = 0xFFFFFFFF or parents;
                                graphs of up to 2^27 vertices) */
+  /* batched searches (bfsx_run_multi) */
+  uint32_t ms_relabel;      /* 1: search a copy of the graph with vertices
+                               renamed by degree (hubs first; made once per
+                               graph, symmetric graphs only), 0: as stored */
+  uint32_t ms_push_divisor; /* a level runs top-down while its frontier's
+                               edges * this < nnz, else bottom-up (1..1024) */
+  uint32_t ms_pull_hub;     /* bottom-up levels give vertices with more
+                               neighbours than this a work-group each (32..65536) */
+  uint32_t ms_graph_levels; /* levels per captured hipGraph launch (2..64, even) */
 } bfsx_params;
 
 /* ---- context ------------------------------------------------------------ */
@@ -254,6 +263,13 @@ int bfsx_run_multi(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits,
 /* Builds n_graphs R-MAT graphs (quadrants 0.57/0.19/0.19/0.05) packed into
  * one symmetric CSC: graph g holds vertices [offsets[g], offsets[g+1]).
  * offsets has n_graphs + 1 entries. */
+/* Autotuner of the batched search: coordinate descent over ms_relabel,
+ * ms_push_divisor, ms_pull_hub and ms_graph_levels, scoring the aggregate
+ * throughput of bfsx_run_multi(sources, bits) on the resident graph; installs
+ * and returns the best parameters like bfsx_autotune. */
+int bfsx_autotune_multi(bfsx_ctx *ctx, const uint32_t *sources, const uint16_t *bits,
+                        uint32_t n_sources, uint32_t n_bits, uint32_t passes, bfsx_params *best,
+                        double *gteps_before, double *gteps_after);
 int bfsx_generate_rmat_union(bfsx_ctx *ctx, uint32_t n_graphs, const int *scales,
                              const int *edge_factors, const uint64_t *seeds_graph,
                              const uint64_t *seeds_perm, int col_ptr_bytes,

@@ -98,3 +98,41 @@ def test_packed_graphs_match_recipe_and_run_in_one_batch(engine):
         w8 = np.where(want < 0, 255, want).astype(np.uint8)
         assert (lev[g] == w8).all()
         assert (lev[g][:off[g]] == 255).all() and (lev[g][off[g + 1]:] == 255).all()
+
+
+@pytest.mark.parametrize("relabel,K,divisor,hub", [(0, 8, 14, 256), (1, 2, 14, 256),
+                                                    (1, 64, 4, 64), (1, 4, 32, 1024),
+                                                    (0, 16, 1, 32)])
+def test_batch_shapes_match_oracle(engine, relabel, K, divisor, hub):
+    """The batch on the graph as stored and on its degree-relabelled copy,
+    with 2..64 levels per captured hipGraph launch, different direction
+    divisors and hub thresholds: levels and edge counts equal the oracle's."""
+    engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 201, 202, 4)
+    engine.order_adjacency("pull_hub_first")
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(200, 203)
+    old = engine.params
+    try:
+        engine.set_params(ms_relabel=relabel, ms_graph_levels=K, ms_push_divisor=divisor,
+                          ms_pull_hub=hub)
+        st = check(engine, cp, ri, roots, np.arange(200), 200)
+        if divisor >= 14:
+            assert st["push_levels"] > 0 and st["pull_levels"] > 0
+        # the captured loop replayed, then another bitset width
+        check(engine, cp, ri, roots, np.arange(200), 200)
+        check(engine, cp, ri, roots[:33], np.arange(33), 33)
+    finally:
+        engine.set_params(**old)
+
+
+def test_batch_autotune_installs_valid_parameters(engine):
+    engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 211, 212, 8)
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(128, 213)
+    old = engine.params
+    try:
+        p, before, after = engine.autotune_multi(roots, np.arange(128), passes=1)
+        assert after >= before > 0 and engine.params == p
+        check(engine, cp, ri, roots, np.arange(128), 128)
+    finally:
+        engine.set_params(**old)
