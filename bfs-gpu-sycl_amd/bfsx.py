@@ -28,6 +28,7 @@ GRAPH_PULL_COPY = 4
 RUN_GRAPH_LOOP = 1
 RUN_TRACE = 2
 RUN_STEP_TIMES = 4
+RUN_DEVICE_LOOP = 8
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "out of memory",
           4: "invalid state", 5: "i/o error", 6: "overflow", 7: "unsupported"}
@@ -74,7 +75,7 @@ class Params(C.Structure):
                 ("graph_levels", C.c_uint32), ("mark_ratio_q10", C.c_uint32),
                 ("bucket_ratio_q10", C.c_uint32), ("ms_relabel", C.c_uint32),
                 ("ms_push_divisor", C.c_uint32), ("ms_pull_hub", C.c_uint32),
-                ("ms_graph_levels", C.c_uint32)]
+                ("ms_graph_levels", C.c_uint32), ("coop_groups", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -252,13 +253,14 @@ class Engine:
 
     # ---- search ---------------------------------------------------------
     def run(self, root, strategy="llb", trace=False, parents=False, levels=True,
-            graph_loop=False, step_times=False):
+            graph_loop=False, step_times=False, device_loop=False):
         g = self.info()
         lev = np.empty(g["n_cols"], np.int32) if levels else None
         par = np.empty(g["n_cols"], np.uint32) if parents else None
         st = RunStats()
         flags = ((RUN_TRACE if trace else 0) | (RUN_GRAPH_LOOP if graph_loop else 0)
-                 | (RUN_STEP_TIMES if step_times else 0))
+                 | (RUN_STEP_TIMES if step_times else 0)
+                 | (RUN_DEVICE_LOOP if device_loop else 0))
         self._check(lib().bfsx_run(self._h, int(root), strategy_id(strategy), flags, _ptr(lev),
                                    _ptr(par), C.byref(st)))
         out = [lev, st.as_dict()]
@@ -267,13 +269,14 @@ class Engine:
         return tuple(out)
 
     def run_batch(self, roots, strategy="llb", levels=False, trace=False, graph_loop=False,
-                  step_times=False):
+                  step_times=False, device_loop=False):
         roots = np.ascontiguousarray(roots, np.uint32)
         g = self.info()
         lev = np.empty((len(roots), g["n_cols"]), np.int32) if levels else None
         st = (RunStats * max(len(roots), 1))()
         flags = ((RUN_TRACE if trace else 0) | (RUN_GRAPH_LOOP if graph_loop else 0)
-                 | (RUN_STEP_TIMES if step_times else 0))
+                 | (RUN_STEP_TIMES if step_times else 0)
+                 | (RUN_DEVICE_LOOP if device_loop else 0))
         self._check(lib().bfsx_run_batch(self._h, _ptr(roots), len(roots), strategy_id(strategy),
                                          flags, _ptr(lev), C.cast(st, P)))
         return lev, [st[i].as_dict() for i in range(len(roots))]

@@ -27,6 +27,8 @@
 //         consecutive vertices, so bitmap words are written without atomics.
 #include <algorithm>
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "bfsx_internal.hpp"
 #include "bfsx_mlp.hpp"
 
@@ -1362,14 +1364,15 @@ __global__ void k_level_begin(LevelCtl *ctl) {
   if (threadIdx.x == 0) ctl->t_begin = (unsigned long long)wall_clock64();
 }
 
-// kind >= 0: the step kind that ran; kind < 0: the one in ctl->choice.  A
-// level with an empty frontier (after the end, in a captured loop) changes
-// nothing but the done flag.
-__global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consumed,
-                                                  uint32_t produced, int stamp, int kind,
-                                                  const float *Wg, uint64_t n_cols, uint64_t nnz,
-                                                  uint64_t n_iso, uint32_t allow, int reuse,
-                                                  LevelMirror *mirror) {
+// Bookkeeping at the end of a level, by one work-group (every thread calls
+// it; the scheduler's inference uses the first 64).  kind >= 0: the step
+// kind that ran; kind < 0: the one in ctl->choice.  A level with an empty
+// frontier (after the end, in a captured loop) changes nothing but the done
+// flag.
+__device__ __forceinline__ void level_end_body(LevelCtl *ctl, uint32_t consumed,
+                                               uint32_t produced, int stamp, int kind,
+                                               const float *Wg, uint64_t n_cols, uint64_t nnz,
+                                               uint64_t n_iso, uint32_t allow, int reuse) {
   __shared__ int s_live;
   if (threadIdx.x == 0) {
     s_live = 0;
@@ -1418,20 +1421,93 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
     // keeps the last decision costs what a level without the scheduler does)
     if (s_live) decide_wave(ctl, Wg, produced, n_cols, nnz, n_iso, allow, 0);
   }
-  // What the host needs for the next level, into the host-mapped words, each
-  // tagged with this level end's sequence number (thread 0 made every write
-  // of this launch): the host polls them instead of synchronising the stream.
-  if (mirror && threadIdx.x == 0) {
-    const uint32_t seq = ctl->seq + 1;
-    ctl->seq = seq;
-    const unsigned long long t32 = (unsigned long long)seq << 32;
-    const unsigned long long t16 = (unsigned long long)(seq & 0xFFFFu) << 48;
-    mirror->w[0] = t32 | ctl->q_count[produced];
-    mirror->w[1] = t16 | (ctl->q_edges[produced] & MIRROR_48);
-    mirror->w[2] = t16 | (ctl->visited_edges & MIRROR_48);
-    mirror->w[3] = t32 | (ctl->choice & 0xFFu) | ((unsigned long long)(ctl->done ? 1u : 0u) << 8);
-    __threadfence_system();
+}
+
+// What the host needs for the next level, into the host-mapped words, each
+// tagged with a new sequence number (by the thread that made, or saw, every
+// write to the control block): the host polls them instead of synchronising
+// the stream.  `produced` = the queue buffer holding the next frontier.
+__device__ __forceinline__ void write_level_end(LevelCtl *ctl, uint32_t produced,
+                                                LevelMirror *mirror) {
+  const uint32_t seq = ctl->seq + 1;
+  ctl->seq = seq;
+  const unsigned long long t32 = (unsigned long long)seq << 32;
+  const unsigned long long t16 = (unsigned long long)(seq & 0xFFFFu) << 48;
+  mirror->w[0] = t32 | ctl->q_count[produced];
+  mirror->w[1] = t16 | (ctl->q_edges[produced] & MIRROR_48);
+  mirror->w[2] = t16 | (ctl->visited_edges & MIRROR_48);
+  mirror->w[3] = t32 | (ctl->choice & 0xFFu) | ((unsigned long long)(ctl->done ? 1u : 0u) << 8) |
+                 ((unsigned long long)(ctl->level & MIRROR_LEVEL_MASK) << 9);
+  __threadfence_system();
+}
+
+__global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consumed,
+                                                  uint32_t produced, int stamp, int kind,
+                                                  const float *Wg, uint64_t n_cols, uint64_t nnz,
+                                                  uint64_t n_iso, uint32_t allow, int reuse,
+                                                  LevelMirror *mirror) {
+  level_end_body(ctl, consumed, produced, stamp, kind, Wg, n_cols, nnz, n_iso, allow, reuse);
+  if (mirror && threadIdx.x == 0) write_level_end(ctl, produced, mirror);
+}
+
+// ---------------------------------------------------------------------------
+// device loop: consecutive thin top-down levels in one cooperative launch
+// ---------------------------------------------------------------------------
+// Every work-group is resident (cooperative launch), so levels are separated
+// by grid-wide barriers instead of kernel launches and host round trips.  A
+// level runs while its frontier has fewer than max_edges edges (exact-once
+// discovery, the regime where a level costs microseconds) and its kind is
+// LLB or SLB (the fixed kind, or the scheduler's choice, made by work-group
+// 0 at each level end exactly as k_level_end makes it).  The loop leaves at
+// the first level that is heavy, or for which the scheduler picks GLB or
+// PULL, or when the frontier is empty, or after max_levels; work-group 0
+// then writes the tagged level-end words (with the level reached) for the
+// host, which runs the next level itself.
+struct CoopArgs {
+  uint32_t *queue[2];
+  uint32_t cur;                  // queue buffer holding the frontier at entry
+  int fixed;                     // BFSX_LLB or BFSX_SLB, or -1: ctl->choice
+  unsigned long long max_edges;  // frontiers with this many edges leave the loop
+  uint32_t max_levels;
+  const float *Wg;               // scheduler weights (learned) or null
+  uint64_t n_cols, nnz, n_iso;
+  uint32_t allow;                // kinds the scheduler may pick
+  LevelMirror *mirror;
+};
+
+template <typename PtrT, int WG>
+__global__ __launch_bounds__(WG) void k_coop_levels(KArgs base, LevelCtl *ctl, CoopArgs c) {
+  namespace cg = cooperative_groups;
+  cg::grid_group grid = cg::this_grid();
+  __shared__ LlbShared<PtrT, WG> sh;
+  uint32_t cur = c.cur;
+  for (uint32_t it = 0; it < c.max_levels; it++) {
+    // the same control block for every work-group (written before the
+    // launch or before the last barrier): the decision to stay is uniform
+    const uint32_t F = ctl->q_count[cur];
+    const unsigned long long EF = ctl->q_edges[cur];
+    const int choice = c.fixed >= 0 ? c.fixed : (int)ctl->choice;
+    if (F == 0 || EF >= c.max_edges || (choice != BFSX_LLB && choice != BFSX_SLB)) break;
+    KArgs a = base;
+    a.q_in = c.queue[cur];
+    a.q_out = c.queue[cur ^ 1];
+    a.F = F;
+    a.q_out_count = &ctl->q_count[cur ^ 1];
+    a.q_out_edges = &ctl->q_edges[cur ^ 1];
+    a.next_level = (int32_t)ctl->level + 1;
+    a.mark = nullptr;
+    const uint32_t n_tiles = (F + WG - 1) / WG;
+    unsigned long long dacc = 0;
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
+      llb_tile<PtrT, WG>(a, sh, tile, n_tiles, choice == BFSX_SLB, dacc);
+    block_atomic_add<WG>(dacc, a.q_out_edges);
+    grid.sync();
+    if (blockIdx.x == 0)
+      level_end_body(ctl, cur, cur ^ 1, 0, choice, c.Wg, c.n_cols, c.nnz, c.n_iso, c.allow, 1);
+    grid.sync();
+    cur ^= 1;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) write_level_end(ctl, cur, c.mirror);
 }
 
 // sum of out-degree over reached vertices < n_rows, reached count, max level
@@ -1896,6 +1972,64 @@ void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
                unsigned long long mark_min_E) {
   if (ctx->ptr_bytes == 8) dev_level_t<uint64_t>(ctx, cur, fixed, track, mark_min_E);
   else dev_level_t<uint32_t>(ctx, cur, fixed, track, mark_min_E);
+}
+
+template <typename PtrT, int WG>
+static bool coop_launch(bfsx_ctx *ctx, const KArgs &a, const CoopArgs &c) {
+  const void *f = reinterpret_cast<const void *>(&k_coop_levels<PtrT, WG>);
+  int per_cu = 0;
+  BFSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, WG, 0));
+  if (per_cu < 1) return false;
+  const unsigned groups = std::min<unsigned>(ctx->params.coop_groups, (unsigned)(ctx->cu_count * per_cu));
+  const dim3 grid(groups), block(WG);
+  KArgs ka = a;
+  LevelCtl *ctl = ctx->ctl.as<LevelCtl>();
+  CoopArgs ca = c;
+  void *args[] = {&ka, &ctl, &ca};
+  BFSX_HIP(hipLaunchCooperativeKernel(f, grid, block, args, 0, ctx->stream));
+  return true;
+}
+
+template <typename PtrT>
+static bool coop_levels_t(bfsx_ctx *ctx, const KArgs &a, const CoopArgs &c) {
+  switch (ctx->params.llb_wg) {
+    case 64: return coop_launch<PtrT, 64>(ctx, a, c);
+    case 128: return coop_launch<PtrT, 128>(ctx, a, c);
+    case 256: return coop_launch<PtrT, 256>(ctx, a, c);
+    case 512: return coop_launch<PtrT, 512>(ctx, a, c);
+    case 1024: return coop_launch<PtrT, 1024>(ctx, a, c);
+    default: fail(BFSX_ERR_ARG, "llb_wg must be 64, 128, 256, 512 or 1024");
+  }
+}
+
+bool coop_levels(bfsx_ctx *ctx, uint32_t cur, int fixed, uint32_t allow, bool learned,
+                 unsigned long long max_edges, uint32_t max_levels) {
+  if (!ctx->coop_ok) return false;
+  KArgs a{};
+  a.col_ptr = ctx->col_ptr.p;
+  a.row_idx = ctx->row_idx.as<uint32_t>();
+  a.n_rows = ctx->n_rows;
+  a.n_cols = ctx->n_cols;
+  a.vis = ctx->vis.as<uint32_t>();
+  a.levels = ctx->levels.as<int32_t>();
+  a.parents = nullptr;
+  a.hub_degree = ctx->params.hub_degree;
+  a.mark = nullptr;
+  a.examined = nullptr;
+  CoopArgs c{};
+  c.queue[0] = ctx->queue[0].as<uint32_t>();
+  c.queue[1] = ctx->queue[1].as<uint32_t>();
+  c.cur = cur;
+  c.fixed = fixed;
+  c.max_edges = max_edges;
+  c.max_levels = max_levels;
+  c.Wg = learned ? ctx->sched_w.as<float>() : nullptr;
+  c.n_cols = ctx->n_cols;
+  c.nnz = ctx->nnz;
+  c.n_iso = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols ? ctx->n_isolated : 0;
+  c.allow = allow;
+  c.mirror = ctx->d_mirror;
+  return ctx->ptr_bytes == 8 ? coop_levels_t<uint64_t>(ctx, a, c) : coop_levels_t<uint32_t>(ctx, a, c);
 }
 
 void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out) {

@@ -105,6 +105,7 @@ static bfsx_params default_params() {
   p.ms_push_divisor = 14;
   p.ms_pull_hub = 256;
   p.ms_graph_levels = 8;
+  p.coop_groups = 32;
   return p;
 }
 
@@ -119,6 +120,8 @@ static void validate_params(const bfsx_params &p) {
   if (p.glb_chunk < 256 || p.glb_chunk > (1u << 22)) fail(BFSX_ERR_ARG, "glb_chunk must be in [256, 2^22]");
   if (p.graph_levels < 1 || p.graph_levels > 64) fail(BFSX_ERR_ARG, "graph_levels must be in [1,64]");
   if (p.ms_relabel > 1) fail(BFSX_ERR_ARG, "ms_relabel must be 0 or 1");
+  if (p.coop_groups < 1 || p.coop_groups > 4096)
+    fail(BFSX_ERR_ARG, "coop_groups must be in [1,4096]");
   if (p.ms_push_divisor < 1 || p.ms_push_divisor > 1024) fail(BFSX_ERR_ARG, "ms_push_divisor must be in [1,1024]");
   if (p.ms_pull_hub < 32 || p.ms_pull_hub > 65536) fail(BFSX_ERR_ARG, "ms_pull_hub must be in [32,65536]");
   if (p.ms_graph_levels < 2 || p.ms_graph_levels > 64 || (p.ms_graph_levels & 1))
@@ -150,6 +153,7 @@ static void view_from_ctl(bfsx_ctx *ctx, uint32_t produced) {
   ctx->view.VE = c.visited_edges;
   ctx->view.choice = c.choice;
   ctx->view.done = c.done;
+  ctx->view.level = c.level & MIRROR_LEVEL_MASK;
 }
 
 // After a launch that ends with k_level_end: that kernel writes the values
@@ -159,7 +163,7 @@ static void view_from_ctl(bfsx_ctx *ctx, uint32_t produced) {
 // level end next to a new one means the new ones have not all arrived.
 // After a second without them, the stream is synchronised, which reports any
 // device error, and the values come from a full copy of the control block.
-void read_level_end(bfsx_ctx *ctx, uint32_t seq, uint32_t produced) {
+void read_level_end(bfsx_ctx *ctx, uint32_t seq, uint32_t produced, uint32_t level0) {
   const volatile unsigned long long *w = ctx->h_mirror->w;
   const unsigned long long t32 = seq, t16 = seq & 0xFFFFu;
   const auto t0 = std::chrono::steady_clock::now();
@@ -171,12 +175,15 @@ void read_level_end(bfsx_ctx *ctx, uint32_t seq, uint32_t produced) {
       ctx->view.VE = w2 & MIRROR_48;
       ctx->view.choice = (uint32_t)(w3 & 0xFFu);
       ctx->view.done = (uint32_t)((w3 >> 8) & 1u);
+      ctx->view.level = (uint32_t)((w3 >> 9) & MIRROR_LEVEL_MASK);
       return;
     }
     if ((spin & 1023u) == 1023u &&
         std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
       read_ctl(ctx);  // synchronises the stream (reports device errors)
       if (ctx->h_ctl->seq != seq) fail(BFSX_ERR_DEVICE, "level end did not arrive");
+      // after a device loop: the frontier's buffer follows the levels run
+      if (level0 != ~0u) produced ^= (ctx->h_ctl->level - level0) & 1u;
       view_from_ctl(ctx, produced);
       return;
     }
@@ -429,6 +436,11 @@ struct BeamerRule {
 
 static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
                      bool want_parents) {
+  if ((flags & BFSX_RUN_DEVICE_LOOP) &&
+      ((flags & (BFSX_RUN_TRACE | BFSX_RUN_STEP_TIMES | BFSX_RUN_GRAPH_LOOP)) || want_parents ||
+       strategy == BFSX_BEAMER))
+    fail(BFSX_ERR_UNSUPPORTED,
+         "the device loop runs without traces, parents, the captured loop or the Beamer rule");
   if (strategy < BFSX_LLB || strategy > BFSX_BEAMER) fail(BFSX_ERR_ARG, "unknown strategy");
   const bool symmetric = ctx->has_graph && graph_symmetric(ctx);
   if (strategy == BFSX_PULL && !symmetric)
@@ -473,12 +485,36 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   }
   uint32_t seq = ctx->h_ctl->seq;  // level ends so far (h_ctl: the block k_init wrote)
   uint32_t decisions = ctx->h_ctl->decisions;
+  const bool dev_loop = (flags & BFSX_RUN_DEVICE_LOOP) && ctx->coop_ok;
+  const unsigned long long thin = mark_min_edges(ctx);
   while (w.F > 0) {
     // a search has at most n_cols levels; more means a broken invariant
     if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
     const int choice = learned  ? (int)ctx->view.choice
                        : beamer ? rule.next(ctx, w.pull_possible, w.F, w.EF, w.VE)
                                 : fixed;
+    if (dev_loop && (choice == BFSX_LLB || choice == BFSX_SLB) && w.queue_valid && w.EF < thin) {
+      // a run of thin levels on the device, up to the first heavy level, GLB
+      // or PULL choice, or the end of the search
+      const uint32_t level0 = w.level;
+      // (at most 4096 levels per launch: the host launches again if needed)
+      if (coop_levels(ctx, w.cur, learned ? -1 : fixed, w.decide_allow, learned, thin, 4096)) {
+        read_level_end(ctx, ++seq, w.cur, level0);
+        const uint32_t ran = (ctx->view.level - level0) & MIRROR_LEVEL_MASK;
+        if (ran > 0) {
+          w.cur ^= ran & 1u;
+          w.level += ran;
+          w.F = ctx->view.F;
+          w.EF = ctx->view.EF;
+          w.VE = ctx->view.VE;
+          w.fbm_valid = false;
+          w.queue_valid = true;
+          w.last = choice;
+          out.n_levels += ran;
+          continue;
+        }
+      }
+    }
     const uint32_t level = w.level, F = w.F, cur = w.cur;
     // the decision in force for this level (made at the previous level's end)
     bfsx_level_record r{};
@@ -527,6 +563,10 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   float ms = 0;
   BFSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   out.ms = ms;
+  if (dev_loop) {  // the device loop's levels were counted per kind on the device
+    read_ctl(ctx);
+    for (int k = 0; k < BFSX_N_STEP_KINDS; k++) out.steps[k] = ctx->h_ctl->kind_count[k];
+  }
   return out;
 }
 
@@ -732,6 +772,9 @@ int bfsx_create(int device, bfsx_ctx **out) {
       ctx->lds_per_cu = (uint32_t)prop.maxSharedMemoryPerMultiProcessor;
     ctx->name = prop.gcnArchName;
     ctx->hbm_bytes = prop.totalGlobalMem;
+    int coop = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) == hipSuccess)
+      ctx->coop_ok = coop != 0;
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
       ctx->wall_clock_khz = khz;

@@ -63,11 +63,13 @@ constexpr size_t CTL_RESET_BYTES = offsetof(LevelCtl, seq);
 //   w[0] = seq << 32 | frontier size (count of the produced queue)
 //   w[1] = (seq & 0xFFFF) << 48 | frontier edges (48 bits)
 //   w[2] = (seq & 0xFFFF) << 48 | edges of the visited vertices (48 bits)
-//   w[3] = seq << 32 | next step kind (8 bits) | done << 8
+//   w[3] = seq << 32 | next step kind (8 bits) | done << 8 | level << 9
+//          (the level of the next frontier, 23 bits)
 struct LevelMirror {
   unsigned long long w[4];
 };
 constexpr unsigned long long MIRROR_48 = (1ull << 48) - 1;
+constexpr uint32_t MIRROR_LEVEL_MASK = (1u << 23) - 1;
 
 // The host's copy of those values (from the mirror, or from a full copy of
 // the control block).
@@ -75,6 +77,7 @@ struct LevelView {
   uint32_t F = 0;
   unsigned long long EF = 0, VE = 0;
   uint32_t choice = 0, done = 0;
+  uint32_t level = 0;  // level of the next frontier (23 bits)
 };
 
 // Bitmaps hold whole 1024-vertex chunks (32 words) so that a wave can own a
@@ -98,6 +101,7 @@ struct bfsx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int cu_count = 0;
+  bool coop_ok = false;            // cooperative launches supported
   uint32_t lds_per_block = 65536;  // LDS bytes one work-group may allocate
   uint32_t lds_per_cu = 65536;     // LDS bytes of one CU
   int wall_clock_khz = 100000;
@@ -245,10 +249,19 @@ void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
 void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
                unsigned long long mark_min_E);
 void level_begin(bfsx_ctx *ctx);
+// Runs consecutive thin top-down levels (frontier edges below max_edges,
+// kind LLB or SLB: `fixed`, or the scheduler's choice when fixed < 0) in one
+// cooperative launch, from the frontier in queue `cur`; the launch ends with
+// one set of level-end words (ctx->d_mirror) carrying the level reached.
+// False when the device cannot launch cooperatively.
+bool coop_levels(bfsx_ctx *ctx, uint32_t cur, int fixed, uint32_t allow, bool learned,
+                 unsigned long long max_edges, uint32_t max_levels);
 // Waits for the level end with sequence number `seq` (the tagged host-mapped
 // words) and leaves its values in ctx->view; `produced` = the queue buffer
 // the level filled (for the synchronising fallback).
-void read_level_end(bfsx_ctx *ctx, uint32_t seq, uint32_t produced);
+// After a device loop (level0 = the level at its launch) the buffer is
+// `produced` flipped once per level it ran.
+void read_level_end(bfsx_ctx *ctx, uint32_t seq, uint32_t produced, uint32_t level0 = ~0u);
 void reached_summary(bfsx_ctx *ctx, unsigned long long *d_out);
 void level_histogram(bfsx_ctx *ctx, uint32_t max_levels, unsigned long long *d_counts,
                      unsigned long long *d_edges);

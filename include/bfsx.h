@@ -75,6 +75,16 @@ extern "C" {
 #define BFSX_RUN_TRACE 2u       /* record per-level statistics and timings */
 #define BFSX_RUN_STEP_TIMES 4u  /* record per-level step times only (no
                                    extra per-vertex work in the kernels) */
+#define BFSX_RUN_DEVICE_LOOP 8u /* device-driven thin levels: runs of top-down
+                                   levels whose frontier edges stay below the
+                                   mark threshold (mark_ratio_q10) and whose
+                                   kind is LLB or SLB (fixed, or the learned
+                                   scheduler's choice, made on the device)
+                                   execute in one cooperative launch with
+                                   grid-wide barriers between levels; heavy
+                                   levels and GLB or PULL steps run as host-
+                                   issued steps.  Any strategy but BFSX_BEAMER;
+                                   not with traces or parents. */
 
 typedef struct bfsx_ctx bfsx_ctx;
 
@@ -160,6 +170,10 @@ typedef struct bfsx_params {
   uint32_t ms_pull_hub;     /* bottom-up levels give vertices with more
                                neighbours than this a work-group each (32..65536) */
   uint32_t ms_graph_levels; /* levels per captured hipGraph launch (2..64, even) */
+  uint32_t coop_groups;     /* BFSX_RUN_DEVICE_LOOP: work-groups of the
+                               cooperative launch (1..4096, capped by what
+                               fits resident; grid-wide barriers cost more
+                               the more groups take part) */
 } bfsx_params;
 
 /* ---- context ------------------------------------------------------------ */

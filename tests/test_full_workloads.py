@@ -31,14 +31,17 @@ def say(*a):
 
 
 def check_roots(engine, cp, ri, roots, modes):
-    """modes: [(strategy, graph_loop)]; the oracle runs 16 roots at a time."""
+    """modes: [(strategy, loop)], loop False (host loop), True (captured
+    hipGraph loop) or "device" (device loop); the oracle runs 16 roots at a
+    time."""
     with ThreadPoolExecutor(POOL) as pool:
         for lo in range(0, len(roots), POOL):
             batch = [int(r) for r in roots[lo:lo + POOL]]
             want = list(pool.map(lambda r: cpu.bfs_queue(cp, ri, r), batch))
             for r, (lev_w, ec_w, _) in zip(batch, want):
                 for s, loop in modes:
-                    lev, st = engine.run(r, s, graph_loop=loop)
+                    lev, st = engine.run(r, s, graph_loop=loop is True,
+                                         device_loop=loop == "device")
                     bad = np.flatnonzero(lev != lev_w)
                     assert bad.size == 0, (s, loop, r, bad[:5], lev[bad[:5]], lev_w[bad[:5]])
                     assert st["edges_count"] == ec_w, (s, loop, r)
@@ -55,7 +58,8 @@ def test_rmat26_all_64_roots(engine):
     say("rmat26", nnz)
     engine.load_scheduler()
     try:
-        check_roots(engine, cp, ri, roots, [("learned", False), ("llb", False)])
+        check_roots(engine, cp, ri, roots, [("learned", False), ("llb", False),
+                                            ("learned", "device")])
     finally:
         engine.set_scheduler_weights(None)
 
@@ -73,7 +77,8 @@ def test_grid4096_all_64_roots(engine):
     try:
         check_roots(engine, cp, ri, roots,
                     [(s, False) for s in ("llb", "slb", "glb", "pull", "learned")] +
-                    [(s, True) for s in ("llb", "slb", "learned")])
+                    [(s, True) for s in ("llb", "slb", "learned")] +
+                    [(s, "device") for s in ("llb", "learned")])
     finally:
         engine.set_scheduler_weights(None)
 

@@ -8,7 +8,7 @@ import bfsx
 import workloads
 from graphs import (binary_tree, csc_from_edges, path_graph, random_graph, star_graph,
                     symmetrize)
-from oracle import cpu
+from oracle import cpu, mlp_ref
 
 pytestmark = pytest.mark.gpu
 
@@ -564,6 +564,87 @@ def test_graph_loop_learned_and_mark_modes(engine):
                     assert st["steps"]["llb"] + st["steps"]["slb"] == int(want.max()) + 1
     finally:
         engine.set_scheduler_weights(None)
+        engine.set_params(**old)
+
+
+# ---------------------------------------------------------------------------
+# device loop (cooperative launch of thin levels)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("ratio", [16, 0, 0xFFFFFFFF])
+def test_device_loop_parity(engine, ratio):
+    """Runs of thin levels in one cooperative launch (mark threshold as set,
+    never thin, always thin), fixed kinds and the learned schedule: levels,
+    edge counts and per-kind level counts match the oracle and the host
+    loop, on an R-MAT graph, a grid with hundreds of levels, a path and a
+    directed graph with n_rows < n_cols."""
+    old = engine.params
+    try:
+        engine.set_params(mark_ratio_q10=ratio)
+        cases = []
+        engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 221, 222, 4)
+        engine.order_adjacency("pull_hub_first")
+        cases.append((engine.download_csc(), engine.pick_roots(4, 223), True, None))
+        engine.generate_grid(300, 0.05, 5, 224, 8)
+        cases.append((engine.download_csc(), engine.pick_roots(3, 225, 1), True, None))
+        cp, ri = path_graph(3000)
+        cases.append(((cp, ri), np.array([5, 1500], np.uint32), True, None))
+        rng = np.random.default_rng(12)
+        dcp, dri = random_graph(rng, 5000, 30000, directed=True)
+        cases.append(((dcp, dri), rng.integers(0, 5000, 3), False, 4990))
+        for (cp, ri), roots, sym, n_rows in cases:
+            engine.upload_csc(cp, ri, n_rows=n_rows, symmetric=sym)
+            if sym:
+                engine.order_adjacency("pull_hub_first")
+            for r in roots:
+                want, ec, _ = cpu.bfs_queue(cp, ri, int(r), n_rows=n_rows)
+                for s in ("llb", "slb", "glb") + (("pull",) if sym else ()):
+                    lev, st = engine.run(int(r), s, device_loop=True)
+                    assert np.array_equal(lev, want), (ratio, s, int(r))
+                    assert st["edges_count"] == ec
+                    assert st["n_levels"] == int(want.max()) + 1 == sum(st["steps"].values())
+                    assert st["steps"][s] == st["n_levels"]
+                if sym:
+                    for seed in range(2):
+                        engine.set_scheduler_weights(mlp_ref.init_weights(240 + seed) * 3)
+                        lev, st = engine.run(int(r), "learned", device_loop=True)
+                        assert np.array_equal(lev, want), ("learned", seed, int(r))
+                        assert st["n_levels"] == int(want.max()) + 1 == sum(st["steps"].values())
+                    engine.load_scheduler()
+                    lev, st = engine.run(int(r), "learned", device_loop=True)
+                    assert np.array_equal(lev, want) and st["edges_count"] == ec
+                    engine.set_scheduler_weights(None)
+    finally:
+        engine.set_scheduler_weights(None)
+        engine.set_params(**old)
+
+
+def test_device_loop_block_counts_and_tiny_graphs(engine):
+    old = engine.params
+    try:
+        engine.generate_grid(200, 0.05, 3, 231, 4)
+        cp, ri = engine.download_csc()
+        roots = engine.pick_roots(2, 232, 1)
+        for groups in (1, 7, 256, 4096):
+            engine.set_params(coop_groups=groups, llb_wg=1024 if groups == 7 else 256)
+            for r in roots:
+                want, _, _ = cpu.bfs_queue(cp, ri, int(r))
+                assert np.array_equal(engine.run(int(r), "slb", device_loop=True)[0], want)
+        engine.set_params(**old)
+        c1, r1 = csc_from_edges(1, [], [])
+        engine.upload_csc(c1, r1, symmetric=True)
+        lev, st = engine.run(0, "llb", device_loop=True)
+        assert list(lev) == [0] and st["n_levels"] == 1
+        c4, r4 = csc_from_edges(4, *symmetrize([0, 1], [1, 2]))
+        engine.upload_csc(c4, r4, symmetric=True)
+        assert list(engine.run(3, "llb", device_loop=True)[0]) == [-1, -1, -1, 0]
+        assert list(engine.run(0, "slb", device_loop=True)[0]) == [0, 1, 2, -1]
+        for bad in (dict(trace=True), dict(parents=True), dict(graph_loop=True)):
+            with pytest.raises(bfsx.BfsxError) as e:
+                engine.run(0, "llb", device_loop=True, **bad)
+            assert e.value.status == 7
+        with pytest.raises(bfsx.BfsxError):
+            engine.run(0, "beamer", device_loop=True)
+    finally:
         engine.set_params(**old)
 
 
