@@ -871,6 +871,39 @@ __device__ __forceinline__ void in_frontier4(const uint32_t *fbm, const uint32_t
   for (int i = 0; i < 4; i++) h[i] = may[i] && ((word[i] >> (w[i] & 31)) & 1u);
 }
 
+// Probe of a first neighbour f read from fd: with hub indexing (hbm set) a
+// flagged f names hub f & ~PULL_HUB_FLAG, whose frontier bit is in hbm (an
+// L2-resident bitmap of the hubs); otherwise f is a vertex id, probed as by
+// in_frontier.
+template <bool SUM>
+__device__ __forceinline__ bool in_frontier_fd(const uint32_t *fbm, const uint32_t *hbm,
+                                               const uint32_t *s_sum, uint32_t shift,
+                                               uint32_t f) {
+  const bool hub = hbm && (f & PULL_HUB_FLAG);
+  const uint32_t w = hub ? f & ~PULL_HUB_FLAG : f;
+  bool may = true;
+  if (SUM && !hub) {
+    const uint32_t b = w >> shift;
+    may = (s_sum[b >> 5] >> (b & 31)) & 1u;
+  }
+  const uint32_t *src = hub ? hbm : fbm;
+  const uint32_t word = src[may ? (w >> 5) : 0u];
+  return may && ((word >> (w & 31)) & 1u);
+}
+
+// hbm bit h = frontier bit of hub[h] (H a multiple of 64, and so is the
+// grid-stride: every wave runs whole rounds).
+__global__ void k_hub_bits(const uint32_t *fbm, const uint32_t *hub, uint32_t H, uint32_t *hbm) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H; h += stride) {
+    const uint32_t v = hub[h];
+    const unsigned long long m = __ballot((fbm[v >> 5] >> (v & 31)) & 1u);
+    if (lane == 0) hbm[h >> 5] = (uint32_t)m;
+    if (lane == 32) hbm[h >> 5] = (uint32_t)(m >> 32);
+  }
+}
+
 // Summary of a frontier bitmap: bit b = any frontier vertex in
 // [b << shift, (b + 1) << shift); shift >= 5, so a bit covers whole words.
 __global__ void k_fbm_summary(const uint32_t *fbm, uint32_t n_words, uint32_t shift,
@@ -918,7 +951,8 @@ template <typename PtrT, int WG, bool SUM>
 __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
                                              uint32_t *fbm_next, const uint32_t *g_sum,
                                              uint32_t sum_words, uint32_t shift,
-                                             const uint2 *fd) {
+                                             const uint2 *fd, const uint32_t *hbm,
+                                             const uint32_t *hub) {
   constexpr int NW = WG / 64;
   extern __shared__ uint32_t s_sum_mem[];  // sum_words entries when g_sum is set
   __shared__ uint32_t s_found[NW][64];
@@ -980,9 +1014,10 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         // With the hub-first copy, fd[u] = {first neighbour, degree} (a
         // dense array read in vertex order) holds u's highest-degree
         // neighbour: on dense frontiers that probe decides most candidates,
-        // which then cost neither a col_ptr nor a row_idx line.  (u is a
-        // vertex of this chunk below n_rows for every lane, so the loads
-        // need no branch.)
+        // which then cost neither a col_ptr nor a row_idx line.  A first
+        // neighbour among the top-degree vertices is named by its hub index
+        // and probed in the hub bitmap.  (u is a vertex of this chunk below
+        // n_rows for every lane, so the loads need no branch.)
         uint2 x[PULL_SLOTS];
 #pragma unroll
         for (int sl = 0; sl < PULL_SLOTS; sl++) x[sl] = fd[u_[sl]];
@@ -994,13 +1029,15 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
           f[sl] = deg_[sl] > 0 ? x[sl].x : 0u;
         }
 #pragma unroll
-        for (int sl = 0; sl < PULL_SLOTS; sl++) hf[sl] = in_frontier<SUM>(fbm_cur, s_sum, shift, f[sl]);
+        for (int sl = 0; sl < PULL_SLOTS; sl++)
+          hf[sl] = in_frontier_fd<SUM>(fbm_cur, hbm, s_sum, shift, f[sl]);
 #pragma unroll
         for (int sl = 0; sl < PULL_SLOTS; sl++) {
           if (deg_[sl] > 0) {
             if (hf[sl]) {
               found_[sl] = true;
-              par_[sl] = f[sl];
+              const bool by_hub = hbm && (f[sl] & PULL_HUB_FLAG);
+              par_[sl] = !by_hub ? f[sl] : a.parents ? hub[f[sl] & ~PULL_HUB_FLAG] : 0u;
             }
             if (a.examined) scanned += 1;
           }
@@ -1804,23 +1841,31 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
   const uint2 *fd = ctx->pull_fd.p && s.row_idx == ctx->row_idx_pull.as<uint32_t>()
                         ? ctx->pull_fd.as<uint2>()
                         : nullptr;
+  // hub indexing belongs to fd: the hubs' frontier bits, gathered first
+  const uint32_t *hbm = nullptr, *hub = nullptr;
+  if (fd && ctx->pull_hubs) {
+    hub = ctx->pull_hub.as<uint32_t>();
+    hbm = ctx->pull_hbm.as<uint32_t>();
+    hipLaunchKernelGGL(k_hub_bits, dim3(cap_grid(ctx->pull_hubs / 256 + 1, (unsigned)ctx->cu_count * 4)),
+                       dim3(256), 0, st, s.fbm_cur, hub, ctx->pull_hubs, ctx->pull_hbm.as<uint32_t>());
+  }
   KernelTimer kt(s, st);
   switch (wg) {
     case 64:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 64, true>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
-      else hipLaunchKernelGGL((k_pull<PtrT, 64, false>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 64, true>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
+      else hipLaunchKernelGGL((k_pull<PtrT, 64, false>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
       break;
     case 128:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 128, true>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
-      else hipLaunchKernelGGL((k_pull<PtrT, 128, false>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 128, true>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
+      else hipLaunchKernelGGL((k_pull<PtrT, 128, false>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
       break;
     case 256:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 256, true>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
-      else hipLaunchKernelGGL((k_pull<PtrT, 256, false>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 256, true>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
+      else hipLaunchKernelGGL((k_pull<PtrT, 256, false>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
       break;
     case 512:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 512, true>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
-      else hipLaunchKernelGGL((k_pull<PtrT, 512, false>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 512, true>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
+      else hipLaunchKernelGGL((k_pull<PtrT, 512, false>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
       break;
     default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
   }

@@ -79,11 +79,19 @@ static void ensure_pull_state(bfsx_ctx *ctx) {
   ensure(ctx->fsum, sizeof(uint32_t) * bfsx::PULL_SUM_WORDS_MAX, ctx->stream);
 }
 
+// the {first neighbour, degree} array and its hub index
+static void drop_pull_fd(bfsx_ctx *ctx) {
+  release(ctx->pull_fd);
+  release(ctx->pull_hub);
+  release(ctx->pull_hbm);
+  ctx->pull_hubs = 0;
+}
+
 static void after_graph_change(bfsx_ctx *ctx) {
   // edge counts travel in 48-bit fields of the level-end words
   if (ctx->nnz > MIRROR_48) fail(BFSX_ERR_ARG, "graphs above 2^48 stored edges are not supported");
   release(ctx->row_idx_pull);  // a copy of the old lists
-  release(ctx->pull_fd);
+  drop_pull_fd(ctx);
   ctx->gflags &= ~BFSX_GRAPH_PULL_COPY;
   free_search_state(ctx);
   graph_summary(ctx);
@@ -811,7 +819,7 @@ void bfsx_destroy(bfsx_ctx *ctx) {
   release(ctx->col_ptr);
   release(ctx->row_idx);
   release(ctx->row_idx_pull);
-  release(ctx->pull_fd);
+  drop_pull_fd(ctx);
   release(ctx->iso_bm);
   release(ctx->ctl);
   release(ctx->scratch);
@@ -952,6 +960,7 @@ int bfsx_graph_info_get(bfsx_ctx *ctx, bfsx_graph_info *out) {
     out->flags = ctx->gflags;
     out->max_degree = ctx->max_degree;
     out->n_isolated = ctx->n_isolated;
+    out->pull_hubs = ctx->pull_hubs;
   });
 }
 
@@ -970,12 +979,13 @@ int bfsx_order_adjacency(bfsx_ctx *ctx, int order) {
       if (ctx->max_degree < 0xFFFFFFFFull) {  // degrees fit the array's 32 bits
         ensure(ctx->pull_fd, sizeof(uint2) * std::max<uint64_t>(ctx->n_rows, 1), ctx->stream);
         first_neighbours(ctx, ctx->row_idx_pull.as<uint32_t>(), ctx->pull_fd.as<uint2>());
+        index_hub_neighbours(ctx);
       }
       ctx->gflags |= BFSX_GRAPH_PULL_COPY;
       return;
     }
     release(ctx->row_idx_pull);
-    release(ctx->pull_fd);
+    drop_pull_fd(ctx);
     ctx->gflags &= ~BFSX_GRAPH_PULL_COPY;
     order_adjacency(ctx, order == BFSX_ORDER_HUB_FIRST, ctx->row_idx.as<uint32_t>());
     if (order == BFSX_ORDER_HUB_FIRST) ctx->gflags |= BFSX_GRAPH_HUB_FIRST;

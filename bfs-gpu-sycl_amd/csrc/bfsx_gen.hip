@@ -496,6 +496,50 @@ void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint2 *fd) {
   BFSX_HIP(hipStreamSynchronize(ctx->stream));
 }
 
+// hrank[v] = h for the hub hub[h] (hrank was filled with ~0u)
+__global__ void k_hub_rank(const uint32_t *hub, uint32_t H, uint32_t *hrank) {
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H; h += gridDim.x * blockDim.x)
+    hrank[hub[h]] = h;
+}
+
+// first neighbours that are hubs: their index, flagged
+__global__ void k_fd_hub_index(uint2 *fd, uint32_t n_rows, const uint32_t *hrank) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n_rows;
+       v += gridDim.x * blockDim.x) {
+    const uint2 x = fd[v];
+    if (x.y == 0) continue;
+    const uint32_t h = hrank[x.x];
+    if (h != ~0u) fd[v].x = PULL_HUB_FLAG | h;
+  }
+}
+
+void index_hub_neighbours(bfsx_ctx *ctx) {
+  release(ctx->pull_hub);
+  release(ctx->pull_hbm);
+  ctx->pull_hubs = 0;
+  const uint32_t n = ctx->n_rows;
+  if (!ctx->pull_fd.p || ctx->n_cols >= PULL_HUB_FLAG || n < 64) return;
+  // about one vertex in 256, in whole waves of 64
+  uint64_t H = ((uint64_t)n / 256 + 63) & ~63ull;
+  H = std::max<uint64_t>(H, 64);
+  H = std::min<uint64_t>(H, PULL_HUBS_MAX);  // (and H <= n: n >= 64)
+  hipStream_t s = ctx->stream;
+  ensure(ctx->pull_hub, sizeof(uint32_t) * H, s);
+  ensure(ctx->pull_hbm, sizeof(uint32_t) * (H / 32), s);
+  top_degree_vertices(ctx, (uint32_t)H, ctx->pull_hub.as<uint32_t>());
+  DevBuf hrank;
+  ensure(hrank, sizeof(uint32_t) * ctx->n_cols, s);
+  BFSX_HIP(hipMemsetAsync(hrank.p, 0xFF, sizeof(uint32_t) * ctx->n_cols, s));
+  hipLaunchKernelGGL(k_hub_rank, dim3(grid_for(H)), dim3(256), 0, s,
+                     ctx->pull_hub.as<uint32_t>(), (uint32_t)H, hrank.as<uint32_t>());
+  hipLaunchKernelGGL(k_fd_hub_index, dim3(grid_for(n)), dim3(256), 0, s,
+                     ctx->pull_fd.as<uint2>(), n, hrank.as<uint32_t>());
+  BFSX_HIP(hipGetLastError());
+  BFSX_HIP(hipStreamSynchronize(s));
+  release(hrank);
+  ctx->pull_hubs = (uint32_t)H;
+}
+
 void order_adjacency(bfsx_ctx *ctx, bool hub_first, uint32_t *out) {
   if (ctx->ptr_bytes == 8) order_adjacency_t<uint64_t>(ctx, hub_first, out);
   else order_adjacency_t<uint32_t>(ctx, hub_first, out);

@@ -86,6 +86,10 @@ __host__ __device__ inline uint64_t bitmap_words(uint64_t n) { return ((n + 1023
 
 // Largest frontier summary of the bottom-up step (words; 32 KiB).
 constexpr uint32_t PULL_SUM_WORDS_MAX = 8192;
+// fd[v].x with this bit set names a hub by its index (graphs below 2^31
+// vertices); at most PULL_HUBS_MAX hubs (a 32 KiB bitmap).
+constexpr uint32_t PULL_HUB_FLAG = 0x80000000u;
+constexpr uint32_t PULL_HUBS_MAX = 1u << 18;
 __host__ __device__ inline uint64_t mark_bytes(uint64_t n) { return ((n + 1023) / 1024) * 1024; }
 
 // Growable device allocation.
@@ -119,6 +123,12 @@ struct bfsx_ctx {
   bfsx::DevBuf col_ptr, row_idx, iso_bm;
   bfsx::DevBuf row_idx_pull;  // hub-first copy read by bottom-up steps (optional)
   bfsx::DevBuf pull_fd;       // with it: {first (highest-degree) neighbour, degree} per vertex
+  // The pull_hubs vertices of largest degree ("hubs") are named in pull_fd by
+  // their index: fd[v].x = PULL_HUB_FLAG | h for the hub pull_hub[h].  Before
+  // every bottom-up step their frontier bits are gathered into pull_hbm
+  // (pull_hubs bits), a bitmap small enough to stay in L2.
+  bfsx::DevBuf pull_hub, pull_hbm;
+  uint32_t pull_hubs = 0;  // a multiple of 64; 0: pull_fd holds plain ids
 
   // per-search state, sized for the resident graph
   bfsx::DevBuf levels, parents, vis, fbm[2], queue[2], ctl, mark;
@@ -191,6 +201,10 @@ void order_adjacency(bfsx_ctx *ctx, bool hub_first, uint32_t *out);
 // fd[v] = {row_idx[col_ptr[v]] (~0u without neighbours), degree} for v < n_rows
 // (degrees below 2^32).
 void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint2 *fd);
+// Names the first neighbours that are among the graph's top-degree vertices
+// by their index in ctx->pull_hub (sets pull_hub, pull_hbm, pull_hubs); needs
+// ctx->pull_fd and n_cols < 2^31.
+void index_hub_neighbours(bfsx_ctx *ctx);
 // The graph with vertices renamed by (degree descending, id ascending):
 // rank[v] = new id of v, order[i] = old id of new vertex i, new_cp / new_ri
 // the relabelled CSC (lists ascending in new ids), col_ptr of the graph's

@@ -96,6 +96,9 @@ typedef struct bfsx_graph_info {
   uint32_t flags;        /* BFSX_GRAPH_* */
   uint32_t max_degree;
   uint32_t n_isolated;   /* vertices < n_rows with no stored edge */
+  uint32_t pull_hubs;    /* top-degree vertices that bottom-up steps probe in
+                            a separate hub bitmap (BFSX_ORDER_PULL_HUB_FIRST
+                            builds it; 0: none) */
 } bfsx_graph_info;
 
 typedef struct bfsx_run_stats {

@@ -251,8 +251,12 @@ def test_pull_hub_first_copy(engine, ptr):
     engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 85, 86, ptr)
     cp, ri = engine.download_csc()
     engine.order_adjacency("pull_hub_first")
-    flags = engine.info()["flags"]
+    info = engine.info()
+    flags = info["flags"]
     assert flags & bfsx.GRAPH_PULL_COPY and not flags & bfsx.GRAPH_HUB_FIRST
+    # first neighbours among the 128 top-degree vertices are probed in the
+    # hub bitmap (one vertex in 256, whole waves)
+    assert info["pull_hubs"] == 128
     cp2, ri2 = engine.download_csc()
     assert (cp2 == cp).all() and (ri2 == ri).all()
     roots = engine.pick_roots(6, 87)
@@ -268,9 +272,11 @@ def test_pull_hub_first_copy(engine, ptr):
     assert (lev == cpu.msbfs_u8(cp, ri, many, np.arange(128), 128)).all()
     engine.order_adjacency("id")
     assert not engine.info()["flags"] & bfsx.GRAPH_PULL_COPY
+    assert engine.info()["pull_hubs"] == 0
     engine.order_adjacency("pull_hub_first")
     engine.generate_rmat(12, 8, 0.57, 0.19, 0.19, 85, 86, ptr)
     assert not engine.info()["flags"] & bfsx.GRAPH_PULL_COPY
+    assert engine.info()["pull_hubs"] == 0
     cp, ri = engine.download_csc()
     assert_levels(engine, cp, ri, engine.pick_roots(3, 89), ["pull"])
 
