@@ -178,15 +178,26 @@ __device__ __forceinline__ void discover(const KArgs &a, const PtrT *col_ptr, ui
 // Each lane takes EPT consecutive edges per round: one binary search finds
 // the owner slot of the first, the others advance it; all row_idx loads,
 // then all visited-bitmap loads are issued before any store, so every lane
-// keeps EPT independent memory chains in flight.
+// keeps EPT independent memory chains in flight.  Exact-once discoveries
+// are appended with one atomicAdd per wave and edge group (GLB), or, with
+// BLOCK_APPEND, one per work-group and round (a block scan gives every
+// thread its slots; LLB/SLB): on thin levels spread over many LLB tiles,
+// such as a grid's, per-wave atomics on the one queue counter serialise,
+// while GLB's few, larger rounds are faster without the block scan.
+// Every thread of the group calls this with the same lo and hi.
 constexpr int EPT = 4;
 
-template <typename PtrT, int WG>
+template <typename PtrT, int WG, bool BLOCK_APPEND>
 __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr,
                                              const PtrT *s_scan, const PtrT *s_start,
                                              const uint32_t *s_v, PtrT lo, PtrT hi,
                                              unsigned long long &dacc) {
-  for (PtrT base = lo + (PtrT)threadIdx.x * EPT; base < hi; base += (PtrT)WG * EPT) {
+  __shared__ uint32_t s_wsum[WG / 64 + 1];
+  __shared__ uint32_t s_qbase;
+  constexpr PtrT PER_ROUND = (PtrT)WG * EPT;
+  const PtrT rounds = hi > lo ? (hi - lo + PER_ROUND - 1) / PER_ROUND : PtrT(0);
+  for (PtrT r = 0; r < rounds; r++) {
+    const PtrT base = lo + r * PER_ROUND + (PtrT)threadIdx.x * EPT;
     uint32_t j = 0;
 #pragma unroll
     for (uint32_t step = WG / 2; step > 0; step >>= 1)
@@ -214,7 +225,7 @@ __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr
           a.mark[n[k]] = 1;
           if (a.parents) a.parents[n[k]] = from[k];
         }
-    } else {
+    } else if (!BLOCK_APPEND) {
 #pragma unroll
       for (int k = 0; k < EPT; k++) {
         const bool isnew = valid[k] && try_visit(a.vis, n[k]);
@@ -225,6 +236,30 @@ __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr
         }
         wave_append(isnew, n[k], a.q_out, a.q_out_count);
       }
+    } else {
+      bool isnew[EPT];
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int k = 0; k < EPT; k++) {
+        isnew[k] = valid[k] && try_visit(a.vis, n[k]);
+        if (isnew[k]) {
+          a.levels[n[k]] = a.next_level;
+          if (a.parents) a.parents[n[k]] = from[k];
+          if (a.q_out_edges) dacc += out_degree(col_ptr, n[k], a.n_rows);
+          cnt++;
+        }
+      }
+      uint32_t total;
+      const uint32_t ex = block_excl_scan<WG>(cnt, total, s_wsum);
+      if (threadIdx.x == 0 && total) s_qbase = atomicAdd(a.q_out_count, total);
+      __syncthreads();
+      if (cnt) {
+        uint32_t p = s_qbase + ex;
+#pragma unroll
+        for (int k = 0; k < EPT; k++)
+          if (isnew[k]) a.q_out[p++] = n[k];
+      }
+      __syncthreads();  // s_qbase and s_wsum are rewritten by the next round
     }
   }
 }
@@ -283,7 +318,7 @@ __device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh
   __syncthreads();
   // owner of edge i = last slot j with s_scan[j] <= i (empty slots tie with
   // the next one and are skipped by taking the last)
-  expand_edges<PtrT, WG>(a, col_ptr, s_scan, s_start, s_v, PtrT(0), total, dacc);
+  expand_edges<PtrT, WG, true>(a, col_ptr, s_scan, s_start, s_v, PtrT(0), total, dacc);
   if (a.hub_degree) {
     const uint32_t nh = s_hub_n;
     const uint32_t w = t >> 6, l = t & 63;
@@ -475,7 +510,7 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
       __syncthreads();
       const PtrT lo = e0 > s_win_lo ? e0 : s_win_lo;
       const PtrT hi = e1 < s_win_hi ? e1 : s_win_hi;
-      expand_edges<PtrT, WG>(a, col_ptr, s_scan, s_start, s_v, lo, hi, dacc);
+      expand_edges<PtrT, WG, false>(a, col_ptr, s_scan, s_start, s_v, lo, hi, dacc);
       __syncthreads();
     }
   }

@@ -107,7 +107,7 @@ static bfsx_params default_params() {
   p.hub_degree = 0;
   p.pull_wg = 512;
   p.graph_levels = 8;
-  p.mark_ratio_q10 = 16;
+  p.mark_ratio_q10 = 64;
   p.bucket_ratio_q10 = 512;
   p.ms_relabel = 1;
   p.ms_push_divisor = 14;
