@@ -179,64 +179,57 @@ __device__ __forceinline__ void discover(const KArgs &a, const PtrT *col_ptr, ui
 // the owner slot of the first, the others advance it; all row_idx loads,
 // then all visited-bitmap loads are issued before any store, so every lane
 // keeps EPT independent memory chains in flight.  Exact-once discoveries
-// are appended with one atomicAdd per wave and edge group (GLB), or, with
-// BLOCK_APPEND, one per work-group and round (a block scan gives every
-// thread its slots; LLB/SLB): on thin levels spread over many LLB tiles,
-// such as a grid's, per-wave atomics on the one queue counter serialise,
-// while GLB's few, larger rounds are faster without the block scan.
-// Every thread of the group calls this with the same lo and hi.
+// are appended with one atomicAdd per wave and edge group, or, with
+// BLOCK_APPEND on a range of at most BLOCK_APPEND_ROUNDS rounds, one per
+// work-group and round (a block scan gives every thread its slots): on thin
+// levels spread over many small LLB tiles, such as a grid's, per-wave
+// atomics on the one queue counter serialise; on a tile holding a hub's
+// list, hundreds of rounds, the block scan's barriers would make every
+// round wait for the group's slowest wave, so such ranges (and GLB's)
+// append per wave.  Every thread of the group calls this with the same lo
+// and hi.
 constexpr int EPT = 4;
+constexpr int BLOCK_APPEND_ROUNDS = 2;
+
+// Row_idx entries of the EPT consecutive edges from `base` (those below hi),
+// and the frontier vertices they leave from.
+template <typename PtrT, int WG>
+__device__ __forceinline__ void load_edges(const KArgs &a, const PtrT *s_scan,
+                                           const PtrT *s_start, const uint32_t *s_v,
+                                           PtrT base, PtrT hi, uint32_t (&n)[EPT],
+                                           uint32_t (&from)[EPT], bool (&valid)[EPT]) {
+  uint32_t j = 0;
+#pragma unroll
+  for (uint32_t step = WG / 2; step > 0; step >>= 1)
+    if (s_scan[j + step] <= base) j += step;
+  // (the loads stay under their branches: measured faster for LLB than
+  // clamped unconditional loads, whose extra reads and later issue cost
+  // more than the overlap gained)
+#pragma unroll
+  for (int k = 0; k < EPT; k++) {
+    const PtrT e = base + k;
+    valid[k] = e < hi;
+    while (valid[k] && j + 1 < WG && s_scan[j + 1] <= e) j++;
+    n[k] = valid[k] ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0u;
+    from[k] = s_v[j];
+  }
+}
 
 template <typename PtrT, int WG, bool BLOCK_APPEND>
 __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr,
                                              const PtrT *s_scan, const PtrT *s_start,
                                              const uint32_t *s_v, PtrT lo, PtrT hi,
                                              unsigned long long &dacc) {
-  __shared__ uint32_t s_wsum[WG / 64 + 1];
-  __shared__ uint32_t s_qbase;
   constexpr PtrT PER_ROUND = (PtrT)WG * EPT;
-  const PtrT rounds = hi > lo ? (hi - lo + PER_ROUND - 1) / PER_ROUND : PtrT(0);
-  for (PtrT r = 0; r < rounds; r++) {
-    const PtrT base = lo + r * PER_ROUND + (PtrT)threadIdx.x * EPT;
-    uint32_t j = 0;
-#pragma unroll
-    for (uint32_t step = WG / 2; step > 0; step >>= 1)
-      if (s_scan[j + step] <= base) j += step;
-    // (the loads stay under their branches: measured faster for LLB than
-    // clamped unconditional loads, whose extra reads and later issue cost
-    // more than the overlap gained)
-    uint32_t n[EPT], from[EPT];
-    bool valid[EPT];
-#pragma unroll
-    for (int k = 0; k < EPT; k++) {
-      const PtrT e = base + k;
-      valid[k] = e < hi;
-      while (valid[k] && j + 1 < WG && s_scan[j + 1] <= e) j++;
-      n[k] = valid[k] ? a.row_idx[s_start[j] + (e - s_scan[j])] : 0u;
-      from[k] = s_v[j];
-    }
-    if (a.mark) {
-      uint32_t w[EPT];
-#pragma unroll
-      for (int k = 0; k < EPT; k++) w[k] = valid[k] ? a.vis[n[k] >> 5] : ~0u;
-#pragma unroll
-      for (int k = 0; k < EPT; k++)
-        if (valid[k] && !((w[k] >> (n[k] & 31)) & 1u)) {
-          a.mark[n[k]] = 1;
-          if (a.parents) a.parents[n[k]] = from[k];
-        }
-    } else if (!BLOCK_APPEND) {
-#pragma unroll
-      for (int k = 0; k < EPT; k++) {
-        const bool isnew = valid[k] && try_visit(a.vis, n[k]);
-        if (isnew) {
-          a.levels[n[k]] = a.next_level;
-          if (a.parents) a.parents[n[k]] = from[k];
-          if (a.q_out_edges) dacc += out_degree(col_ptr, n[k], a.n_rows);
-        }
-        wave_append(isnew, n[k], a.q_out, a.q_out_count);
-      }
-    } else {
+  if (BLOCK_APPEND && !a.mark && hi > lo && hi - lo <= PER_ROUND * BLOCK_APPEND_ROUNDS) {
+    // (uniform over the group: lo, hi and the mode are)
+    __shared__ uint32_t s_wsum[WG / 64 + 1];
+    __shared__ uint32_t s_qbase;
+    for (PtrT r0 = lo; r0 < hi; r0 += PER_ROUND) {
+      uint32_t n[EPT], from[EPT];
+      bool valid[EPT];
+      load_edges<PtrT, WG>(a, s_scan, s_start, s_v, r0 + (PtrT)threadIdx.x * EPT, hi, n, from,
+                           valid);
       bool isnew[EPT];
       uint32_t cnt = 0;
 #pragma unroll
@@ -260,6 +253,34 @@ __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr
           if (isnew[k]) a.q_out[p++] = n[k];
       }
       __syncthreads();  // s_qbase and s_wsum are rewritten by the next round
+    }
+    return;
+  }
+  for (PtrT base = lo + (PtrT)threadIdx.x * EPT; base < hi; base += PER_ROUND) {
+    uint32_t n[EPT], from[EPT];
+    bool valid[EPT];
+    load_edges<PtrT, WG>(a, s_scan, s_start, s_v, base, hi, n, from, valid);
+    if (a.mark) {
+      uint32_t w[EPT];
+#pragma unroll
+      for (int k = 0; k < EPT; k++) w[k] = valid[k] ? a.vis[n[k] >> 5] : ~0u;
+#pragma unroll
+      for (int k = 0; k < EPT; k++)
+        if (valid[k] && !((w[k] >> (n[k] & 31)) & 1u)) {
+          a.mark[n[k]] = 1;
+          if (a.parents) a.parents[n[k]] = from[k];
+        }
+    } else {
+#pragma unroll
+      for (int k = 0; k < EPT; k++) {
+        const bool isnew = valid[k] && try_visit(a.vis, n[k]);
+        if (isnew) {
+          a.levels[n[k]] = a.next_level;
+          if (a.parents) a.parents[n[k]] = from[k];
+          if (a.q_out_edges) dacc += out_degree(col_ptr, n[k], a.n_rows);
+        }
+        wave_append(isnew, n[k], a.q_out, a.q_out_count);
+      }
     }
   }
 }
@@ -348,7 +369,7 @@ __global__ __launch_bounds__(WG) void k_llb(KArgs a) {
 struct DevStep {
   uint32_t cur;                  // queue buffer holding the frontier
   int fixed;                     // BFSX_LLB or BFSX_SLB, or -1: read ctl->choice
-  unsigned long long mark_min_E; // frontiers with at least this many edges mark + compact
+  MarkRule mark;                 // which frontiers mark + compact
 };
 
 template <typename PtrT, int WG>
@@ -360,7 +381,7 @@ __global__ __launch_bounds__(WG) void k_td_dev(KArgs a, LevelCtl *ctl, DevStep d
   if (choice != BFSX_LLB && choice != BFSX_SLB) return;
   a.F = F;
   a.next_level = (int32_t)ctl->level + 1;
-  if (ctl->q_edges[d.cur] < d.mark_min_E) a.mark = nullptr;
+  if (!d.mark.marks(ctl->q_edges[d.cur], F)) a.mark = nullptr;
   const uint32_t n_tiles = (F + WG - 1) / WG;
   unsigned long long dacc = 0;
   for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
@@ -1375,7 +1396,7 @@ __global__ __launch_bounds__(COMPACT_WG) void k_compact_dev(
     uint32_t chunks_per_group, uint32_t *vis, int32_t *levels, LevelCtl *ctl, DevStep d,
     uint32_t *q_out, int track, uint32_t *fbm_next) {
   const uint32_t F = ctl->q_count[d.cur];
-  if (F == 0 || ctl->q_edges[d.cur] < d.mark_min_E) return;
+  if (F == 0 || !d.mark.marks(ctl->q_edges[d.cur], F)) return;
   const uint32_t out = d.cur ^ 1;
   compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, chunks_per_group, vis, levels,
                      (int32_t)ctl->level + 1, q_out, &ctl->q_count[out],
@@ -1527,8 +1548,8 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
 // ---------------------------------------------------------------------------
 // Every work-group is resident (cooperative launch), so levels are separated
 // by grid-wide barriers instead of kernel launches and host round trips.  A
-// level runs while its frontier has fewer than max_edges edges (exact-once
-// discovery, the regime where a level costs microseconds) and its kind is
+// level runs while it does not mark (exact-once discovery, the regime where
+// a level costs microseconds) and its kind is
 // LLB or SLB (the fixed kind, or the scheduler's choice, made by work-group
 // 0 at each level end exactly as k_level_end makes it).  The loop leaves at
 // the first level that is heavy, or for which the scheduler picks GLB or
@@ -1539,7 +1560,7 @@ struct CoopArgs {
   uint32_t *queue[2];
   uint32_t cur;                  // queue buffer holding the frontier at entry
   int fixed;                     // BFSX_LLB or BFSX_SLB, or -1: ctl->choice
-  unsigned long long max_edges;  // frontiers with this many edges leave the loop
+  MarkRule mark;                 // frontiers it marks leave the loop
   uint32_t max_levels;
   const float *Wg;               // scheduler weights (learned) or null
   uint64_t n_cols, nnz, n_iso;
@@ -1559,7 +1580,7 @@ __global__ __launch_bounds__(WG) void k_coop_levels(KArgs base, LevelCtl *ctl, C
     const uint32_t F = ctl->q_count[cur];
     const unsigned long long EF = ctl->q_edges[cur];
     const int choice = c.fixed >= 0 ? c.fixed : (int)ctl->choice;
-    if (F == 0 || EF >= c.max_edges || (choice != BFSX_LLB && choice != BFSX_SLB)) break;
+    if (F == 0 || c.mark.marks(EF, F) || (choice != BFSX_LLB && choice != BFSX_SLB)) break;
     KArgs a = base;
     a.q_in = c.queue[cur];
     a.q_out = c.queue[cur ^ 1];
@@ -2010,8 +2031,7 @@ void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, 
 // One device-driven top-down level (LLB or SLB, or the choice in the control
 // block when fixed < 0) for a captured loop: step, compaction, bookkeeping.
 template <typename PtrT>
-static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
-                        unsigned long long mark_min_E) {
+static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark) {
   KArgs a{};
   a.col_ptr = ctx->col_ptr.p;
   a.row_idx = ctx->row_idx.as<uint32_t>();
@@ -2027,7 +2047,7 @@ static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
   a.parents = nullptr;
   a.hub_degree = ctx->params.hub_degree;
   a.mark = ctx->mark.as<uint8_t>();
-  const DevStep d{cur, fixed, mark_min_E};
+  const DevStep d{cur, fixed, mark};
   const uint32_t wg = ctx->params.llb_wg;
   const unsigned grid = cap_grid(((uint64_t)ctx->n_cols + wg - 1) / wg, (unsigned)ctx->cu_count * 8);
   hipStream_t s = ctx->stream;
@@ -2048,10 +2068,9 @@ static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
   BFSX_HIP(hipGetLastError());
 }
 
-void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
-               unsigned long long mark_min_E) {
-  if (ctx->ptr_bytes == 8) dev_level_t<uint64_t>(ctx, cur, fixed, track, mark_min_E);
-  else dev_level_t<uint32_t>(ctx, cur, fixed, track, mark_min_E);
+void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark) {
+  if (ctx->ptr_bytes == 8) dev_level_t<uint64_t>(ctx, cur, fixed, track, mark);
+  else dev_level_t<uint32_t>(ctx, cur, fixed, track, mark);
 }
 
 template <typename PtrT, int WG>
@@ -2083,7 +2102,7 @@ static bool coop_levels_t(bfsx_ctx *ctx, const KArgs &a, const CoopArgs &c) {
 }
 
 bool coop_levels(bfsx_ctx *ctx, uint32_t cur, int fixed, uint32_t allow, bool learned,
-                 unsigned long long max_edges, uint32_t max_levels) {
+                 MarkRule mark, uint32_t max_levels) {
   if (!ctx->coop_ok) return false;
   KArgs a{};
   a.col_ptr = ctx->col_ptr.p;
@@ -2101,7 +2120,7 @@ bool coop_levels(bfsx_ctx *ctx, uint32_t cur, int fixed, uint32_t allow, bool le
   c.queue[1] = ctx->queue[1].as<uint32_t>();
   c.cur = cur;
   c.fixed = fixed;
-  c.max_edges = max_edges;
+  c.mark = mark;
   c.max_levels = max_levels;
   c.Wg = learned ? ctx->sched_w.as<float>() : nullptr;
   c.n_cols = ctx->n_cols;

@@ -107,7 +107,7 @@ static bfsx_params default_params() {
   p.hub_degree = 0;
   p.pull_wg = 512;
   p.graph_levels = 8;
-  p.mark_ratio_q10 = 64;
+  p.mark_ratio_q10 = 4;
   p.bucket_ratio_q10 = 512;
   p.ms_relabel = 1;
   p.ms_push_divisor = 14;
@@ -198,13 +198,18 @@ void read_level_end(bfsx_ctx *ctx, uint32_t seq, uint32_t produced, uint32_t lev
   }
 }
 
-// Top-down levels whose frontier has at least this many edges mark and
-// compact; thinner ones discover exact-once with atomics.
-static unsigned long long mark_min_edges(const bfsx_ctx *ctx) {
+// Which top-down levels mark and compact (the others discover exact-once):
+// from n_cols * mark_ratio_q10 / 1024 frontier edges when the frontier
+// averages at least MARK_MIN_AVG_DEGREE edges per vertex, from n_cols / 16
+// (or the ratio's count, if larger) whatever the frontier; ratio 0: every
+// level, 0xFFFFFFFF: none.
+static MarkRule mark_rule(const bfsx_ctx *ctx) {
   const uint32_t r = ctx->params.mark_ratio_q10;
-  if (r == 0) return 0;
-  if (r == 0xFFFFFFFFu) return ~0ull;
-  return ((unsigned long long)ctx->n_cols * r + 1023) / 1024;
+  if (r == 0) return MarkRule{0, 0};
+  if (r == 0xFFFFFFFFu) return MarkRule{~0ull, ~0ull};
+  const unsigned long long lo = ((unsigned long long)ctx->n_cols * r + 1023) / 1024;
+  const unsigned long long all = ((unsigned long long)ctx->n_cols + 15) / 16;
+  return MarkRule{lo, lo > all ? lo : all};
 }
 
 static bool graph_symmetric(const bfsx_ctx *ctx) {
@@ -256,12 +261,13 @@ struct Walker {
     a.EF = EF;
     const uint32_t br = ctx->params.bucket_ratio_q10;
     const unsigned long long UE = ctx->nnz > VE ? ctx->nnz - VE : 0ull;
-    const bool use_mark = choice != BFSX_PULL && EF >= mark_min_edges(ctx);
+    const MarkRule mr = mark_rule(ctx);
+    const bool use_mark = choice != BFSX_PULL && mr.marks(EF, F);
     // bucketed GLB from n_cols/256 frontier edges on (below that its fixed
     // cost, about a pass over the visited bitmap, exceeds exact-once's
     // atomics) or wherever a marking step would run
     a.bucket_ok = br != 0xFFFFFFFFu && (double)UE * 1024.0 >= (double)ctx->nnz * br &&
-                  (use_mark || EF * 256 >= ctx->n_cols) && mark_min_edges(ctx) != ~0ull;
+                  (use_mark || EF * 256 >= ctx->n_cols) && mr.lo_E != ~0ull;
     a.mark = use_mark ? ctx->mark.as<uint8_t>() : nullptr;
     // next frontier bitmap of a top-down step (kept for bottom-up steps)
     uint32_t *td_fbm_next = pull_possible ? ctx->fbm[fbm_cur ^ 1].as<uint32_t>() : nullptr;
@@ -359,11 +365,12 @@ void drop_loop_graph(bfsx_ctx *ctx) {
 // fixed = BFSX_LLB / BFSX_SLB, or -1 for the scheduler's choice between them
 static hipGraphExec_t loop_graph(bfsx_ctx *ctx, int fixed, bool learned) {
   const uint32_t K = (ctx->params.graph_levels + 1) & ~1u;  // even: buffers return to 0
-  const unsigned long long mme = mark_min_edges(ctx);
+  const MarkRule mr = mark_rule(ctx);
   const uint64_t key = 1 + ((uint64_t)(fixed + 1) | ((uint64_t)learned << 3) |
                             ((uint64_t)K << 4) | ((uint64_t)ctx->params.llb_wg << 12) |
                             ((uint64_t)ctx->params.hub_degree << 24)) ^
-                       ((uint64_t)mme * 0x9E3779B97F4A7C15ull);
+                       ((uint64_t)mr.lo_E * 0x9E3779B97F4A7C15ull) ^
+                       ((uint64_t)mr.hi_E * 0xC2B2AE3D27D4EB4Full);
   if (ctx->loop_exec && ctx->loop_key == key) return ctx->loop_exec;
   drop_loop_graph(ctx);
   hipStream_t s = ctx->stream;
@@ -372,7 +379,7 @@ static hipGraphExec_t loop_graph(bfsx_ctx *ctx, int fixed, bool learned) {
     for (uint32_t k = 0; k < K; k++) {
       const uint32_t cur = k & 1u;
       if (learned) level_begin(ctx);
-      dev_level(ctx, cur, fixed, true, mme);
+      dev_level(ctx, cur, fixed, true, mr);
       level_end(ctx, cur, cur ^ 1, learned, fixed,
                 learned ? (1u << BFSX_LLB) | (1u << BFSX_SLB) : 0u, true);
     }
@@ -494,19 +501,19 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   uint32_t seq = ctx->h_ctl->seq;  // level ends so far (h_ctl: the block k_init wrote)
   uint32_t decisions = ctx->h_ctl->decisions;
   const bool dev_loop = (flags & BFSX_RUN_DEVICE_LOOP) && ctx->coop_ok;
-  const unsigned long long thin = mark_min_edges(ctx);
+  const MarkRule mr = mark_rule(ctx);
   while (w.F > 0) {
     // a search has at most n_cols levels; more means a broken invariant
     if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
     const int choice = learned  ? (int)ctx->view.choice
                        : beamer ? rule.next(ctx, w.pull_possible, w.F, w.EF, w.VE)
                                 : fixed;
-    if (dev_loop && (choice == BFSX_LLB || choice == BFSX_SLB) && w.queue_valid && w.EF < thin) {
+    if (dev_loop && (choice == BFSX_LLB || choice == BFSX_SLB) && w.queue_valid && !mr.marks(w.EF, w.F)) {
       // a run of thin levels on the device, up to the first heavy level, GLB
       // or PULL choice, or the end of the search
       const uint32_t level0 = w.level;
       // (at most 4096 levels per launch: the host launches again if needed)
-      if (coop_levels(ctx, w.cur, learned ? -1 : fixed, w.decide_allow, learned, thin, 4096)) {
+      if (coop_levels(ctx, w.cur, learned ? -1 : fixed, w.decide_allow, learned, mr, 4096)) {
         read_level_end(ctx, ++seq, w.cur, level0);
         const uint32_t ran = (ctx->view.level - level0) & MIRROR_LEVEL_MASK;
         if (ran > 0) {

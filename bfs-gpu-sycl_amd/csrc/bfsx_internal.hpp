@@ -71,6 +71,22 @@ struct LevelMirror {
 constexpr unsigned long long MIRROR_48 = (1ull << 48) - 1;
 constexpr uint32_t MIRROR_LEVEL_MASK = (1u << 23) - 1;
 
+// Whether a top-down level marks and compacts or discovers exact-once.  A
+// level marks from hi_E frontier edges on, and from lo_E on when its
+// frontier averages at least MARK_MIN_AVG_DEGREE edges per vertex.  Many
+// edges per frontier vertex (hub lists) send many edges to the same new
+// vertices, whose exact-once atomics then serialise; few (a grid's levels)
+// find mostly distinct new vertices, and their appends cost less than the
+// compaction's pass over all n marks.
+constexpr uint32_t MARK_MIN_AVG_DEGREE = 8;
+
+struct MarkRule {
+  unsigned long long lo_E, hi_E;
+  __host__ __device__ bool marks(unsigned long long EF, uint32_t F) const {
+    return EF >= hi_E || (EF >= lo_E && EF >= (unsigned long long)MARK_MIN_AVG_DEGREE * F);
+  }
+};
+
 // The host's copy of those values (from the mirror, or from a full copy of
 // the control block).
 struct LevelView {
@@ -258,18 +274,17 @@ void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next
 // to the one that decision was computed for.
 void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
                bool stamp, int kind, uint32_t decide_allow = 0, bool reuse = false);
-// mark_min_E: top-down levels whose frontier has at least this many edges
-// mark and compact (needs track: the edge counts are kept in the control block)
-void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track,
-               unsigned long long mark_min_E);
+// mark: which top-down levels mark and compact (needs track: the edge
+// counts are kept in the control block)
+void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark);
 void level_begin(bfsx_ctx *ctx);
-// Runs consecutive thin top-down levels (frontier edges below max_edges,
+// Runs consecutive thin top-down levels (levels `mark` does not mark,
 // kind LLB or SLB: `fixed`, or the scheduler's choice when fixed < 0) in one
 // cooperative launch, from the frontier in queue `cur`; the launch ends with
 // one set of level-end words (ctx->d_mirror) carrying the level reached.
 // False when the device cannot launch cooperatively.
 bool coop_levels(bfsx_ctx *ctx, uint32_t cur, int fixed, uint32_t allow, bool learned,
-                 unsigned long long max_edges, uint32_t max_levels);
+                 MarkRule mark, uint32_t max_levels);
 // Waits for the level end with sequence number `seq` (the tagged host-mapped
 // words) and leaves its values in ctx->view; `produced` = the queue buffer
 // the level filled (for the synchronising fallback).

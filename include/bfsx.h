@@ -147,11 +147,13 @@ typedef struct bfsx_params {
                                degree are expanded by a whole wave (0 = off) */
   uint32_t pull_wg;         /* work-group size of the bottom-up step */
   uint32_t graph_levels;    /* levels per captured hipGraph launch */
-  uint32_t mark_ratio_q10;  /* top-down steps whose frontier edges (exact
+  uint32_t mark_ratio_q10;  /* top-down steps whose frontier edges E (exact
                                out-degree sum, kept on the device) reach
-                               n_cols * mark_ratio_q10 / 1024 mark
+                               n_cols * mark_ratio_q10 / 1024 with at least
+                               8 edges per frontier vertex, or reach
+                               max(that, n_cols / 16) in any case, mark
                                neighbours with plain byte stores and compact
-                               afterwards; smaller ones discover exact-once
+                               afterwards; the others discover exact-once
                                with atomics (0 = always mark,
                                0xFFFFFFFF = never) */
   uint32_t bucket_ratio_q10; /* a GLB step with at least n_cols/256
