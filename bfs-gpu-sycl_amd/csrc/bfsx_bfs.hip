@@ -490,6 +490,62 @@ __global__ void k_glb_offsets(const PtrT *g_loc, const PtrT *tile_off, const Ptr
   }
 }
 
+// The three kernels above in one work-group, for frontiers of at most
+// GLB_SMALL_F entries (the hub-heavy first levels of R-MAT searches): each
+// thread takes 8 consecutive entries per round, a block scan with a running
+// carry gives their global offsets, and the chunk table comes from each
+// entry's [offset, offset + degree).  Saves two launches and their gaps.
+constexpr uint32_t GLB_SMALL_F = 16384;
+constexpr int GLB_SMALL_WG = 1024;
+
+template <typename PtrT>
+__global__ __launch_bounds__(GLB_SMALL_WG) void k_glb_scan_small(
+    const PtrT *col_ptr, uint32_t n_rows, const uint32_t *q_in, uint32_t F, PtrT *g_start,
+    PtrT *g_scan, uint32_t chunk, uint32_t *chunk_lo, LevelCtl *ctl) {
+  __shared__ PtrT s_wave[GLB_SMALL_WG / 64 + 1];
+  PtrT carry = 0;
+  for (uint32_t r0 = 0; r0 < F; r0 += GLB_SMALL_WG * 8) {
+    const uint32_t base = r0 + threadIdx.x * 8;
+    PtrT st[8], d[8];
+    PtrT sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t i = base + k;
+      st[k] = d[k] = 0;
+      if (i < F) {
+        const uint32_t v = q_in[i];
+        if (v < n_rows) {
+          st[k] = col_ptr[v];
+          d[k] = col_ptr[v + 1] - st[k];
+        }
+      }
+      sum += d[k];
+    }
+    PtrT total;
+    PtrT ex = block_excl_scan<GLB_SMALL_WG>(sum, total, s_wave) + carry;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint32_t i = base + k;
+      if (i < F) {
+        g_start[i] = st[k];
+        g_scan[i] = ex;
+        // chunks whose first edge falls in [ex, ex + d)
+        const unsigned long long c0 = ((unsigned long long)ex + chunk - 1) / chunk;
+        const unsigned long long c1 = ((unsigned long long)(ex + d[k]) + chunk - 1) / chunk;
+        for (unsigned long long c = c0; c < c1; c++) chunk_lo[c] = i;
+      }
+      ex += d[k];
+    }
+    carry += total;
+    __syncthreads();  // s_wave is rewritten by the next round's scan
+  }
+  if (threadIdx.x == 0) {
+    ctl->glb_total = (unsigned long long)carry;
+    ctl->glb_chunks = (uint32_t)(((unsigned long long)carry + chunk - 1) / chunk);
+    ctl->glb_cursor = 0;
+  }
+}
+
 template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
                                                    const PtrT *g_start,
@@ -784,7 +840,10 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
       const uint32_t l = t[u];
       if (l == 0xFFFFFFFFu) continue;
       const uint32_t bit = 1u << (l & 31);
-      if (!(s_vis[l >> 5] & bit)) atomicOr(&s_found[l >> 5], bit);
+      // a plain read skips the atomic for targets already found: at heavy
+      // levels many edges lead to the same few vertices (hubs), whose
+      // same-word LDS atomics would serialise
+      if (!((s_vis[l >> 5] | s_found[l >> 5]) & bit)) atomicOr(&s_found[l >> 5], bit);
     }
   }
   __syncthreads();
@@ -1801,13 +1860,20 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   PtrT *g_start = ctx->glb_start.as<PtrT>();
   PtrT *tile = ctx->glb_tile.as<PtrT>();
   PtrT *g_end = tile + ((uint64_t)ctx->n_cols / GLB_TILE + 2);
-  hipLaunchKernelGGL(k_glb_tiles<PtrT>, dim3(n_tiles), dim3(256), 0, st,
-                     static_cast<const PtrT *>(s.col_ptr), s.n_rows, s.q_in, F, g_loc, g_start,
-                     tile);
-  hipLaunchKernelGGL(k_glb_tile_offsets<PtrT>, dim3(1), dim3(1024), 0, st, tile, n_tiles, chunk,
-                     s.ctl, g_end);
-  hipLaunchKernelGGL(k_glb_offsets<PtrT>, dim3(cap_grid((F + 255) / 256, 65535)), dim3(256), 0,
-                     st, g_loc, tile, g_end, F, g_scan, chunk, ctx->glb_chunk_lo.as<uint32_t>());
+  if (F <= GLB_SMALL_F) {
+    hipLaunchKernelGGL(k_glb_scan_small<PtrT>, dim3(1), dim3(GLB_SMALL_WG), 0, st,
+                       static_cast<const PtrT *>(s.col_ptr), s.n_rows, s.q_in, F, g_start, g_scan,
+                       chunk, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl);
+  } else {
+    hipLaunchKernelGGL(k_glb_tiles<PtrT>, dim3(n_tiles), dim3(256), 0, st,
+                       static_cast<const PtrT *>(s.col_ptr), s.n_rows, s.q_in, F, g_loc, g_start,
+                       tile);
+    hipLaunchKernelGGL(k_glb_tile_offsets<PtrT>, dim3(1), dim3(1024), 0, st, tile, n_tiles,
+                       chunk, s.ctl, g_end);
+    hipLaunchKernelGGL(k_glb_offsets<PtrT>, dim3(cap_grid((F + 255) / 256, 65535)), dim3(256),
+                       0, st, g_loc, tile, g_end, F, g_scan, chunk,
+                       ctx->glb_chunk_lo.as<uint32_t>());
+  }
   const KArgs a = kargs(s, 0);
   if (bucketed) {
     // fewer, larger groups than the GLB expand grid: longer runs per
