@@ -45,7 +45,7 @@ class GraphInfo(C.Structure):
     _fields_ = [("n_rows", C.c_uint32), ("n_cols", C.c_uint32), ("nnz", C.c_uint64),
                 ("col_ptr_bytes", C.c_int32), ("flags", C.c_uint32),
                 ("max_degree", C.c_uint32), ("n_isolated", C.c_uint32),
-                ("pull_hubs", C.c_uint32)]
+                ("pull_hubs", C.c_uint32), ("hot_vertices", C.c_uint32)]
 
 
 class RunStats(C.Structure):

@@ -143,7 +143,31 @@ struct KArgs {
   uint32_t hub_degree;
   uint8_t *mark;      // non-null: mark-then-compact mode
   unsigned long long *examined;  // nullable: bottom-up adjacency entries read
+  // mark mode only: row_idx is the hot copy; a flagged entry names hot
+  // vertex hot_id[h], whose visited bit is bit h of hot_vis (null: plain ids)
+  const uint32_t *hot_vis;
+  const uint32_t *hot_id;
 };
+
+// Visited-bitmap word and bit index of a row_idx entry x: with HOT (the hot
+// copy in row_idx) a flagged x is a hot index, probed in a.hot_vis.
+template <bool HOT>
+__device__ __forceinline__ const uint32_t *probe_word(const KArgs &a, uint32_t x,
+                                                      uint32_t &idx) {
+  if (!HOT) {
+    idx = x;
+    return a.vis + (x >> 5);
+  }
+  const bool hot = (x & PULL_HUB_FLAG) != 0;
+  idx = hot ? x & ~PULL_HUB_FLAG : x;
+  return (hot ? a.hot_vis : a.vis) + (idx >> 5);
+}
+
+// The vertex id of a row_idx entry.
+template <bool HOT>
+__device__ __forceinline__ uint32_t entry_vertex(const KArgs &a, uint32_t x) {
+  return (HOT && (x & PULL_HUB_FLAG)) ? a.hot_id[x & ~PULL_HUB_FLAG] : x;
+}
 
 // Two ways to discover a neighbour n of a frontier vertex:
 //  * exact-once: atomicOr on the visited bitmap picks one winner, which
@@ -153,13 +177,15 @@ struct KArgs {
 //    writers store the same value, so no atomic is needed.  k_compact then
 //    turns marks into levels, bitmap bits and the next queue (heavy levels,
 //    where per-edge device-scope atomics would dominate).
-template <typename PtrT>
+template <typename PtrT, bool HOT>
 __device__ __forceinline__ void discover(const KArgs &a, const PtrT *col_ptr, uint32_t n,
                                          uint32_t from, unsigned long long &dacc) {
   if (a.mark) {
-    if (!((a.vis[n >> 5] >> (n & 31)) & 1u)) {
-      a.mark[n] = 1;
-      if (a.parents) a.parents[n] = from;
+    uint32_t idx;
+    if (!((*probe_word<HOT>(a, n, idx) >> (idx & 31)) & 1u)) {
+      const uint32_t v = entry_vertex<HOT>(a, n);
+      a.mark[v] = 1;
+      if (a.parents) a.parents[v] = from;
     }
     return;
   }
@@ -215,7 +241,7 @@ __device__ __forceinline__ void load_edges(const KArgs &a, const PtrT *s_scan,
   }
 }
 
-template <typename PtrT, int WG, bool BLOCK_APPEND>
+template <typename PtrT, int WG, bool BLOCK_APPEND, bool HOT>
 __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr,
                                              const PtrT *s_scan, const PtrT *s_start,
                                              const uint32_t *s_v, PtrT lo, PtrT hi,
@@ -261,14 +287,18 @@ __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr
     bool valid[EPT];
     load_edges<PtrT, WG>(a, s_scan, s_start, s_v, base, hi, n, from, valid);
     if (a.mark) {
-      uint32_t w[EPT];
+      uint32_t w[EPT], idx[EPT];
 #pragma unroll
-      for (int k = 0; k < EPT; k++) w[k] = valid[k] ? a.vis[n[k] >> 5] : ~0u;
+      for (int k = 0; k < EPT; k++) {
+        const uint32_t *p = probe_word<HOT>(a, n[k], idx[k]);
+        w[k] = valid[k] ? *p : ~0u;
+      }
 #pragma unroll
       for (int k = 0; k < EPT; k++)
-        if (valid[k] && !((w[k] >> (n[k] & 31)) & 1u)) {
-          a.mark[n[k]] = 1;
-          if (a.parents) a.parents[n[k]] = from[k];
+        if (valid[k] && !((w[k] >> (idx[k] & 31)) & 1u)) {
+          const uint32_t v = entry_vertex<HOT>(a, n[k]);
+          a.mark[v] = 1;
+          if (a.parents) a.parents[v] = from[k];
         }
     } else {
 #pragma unroll
@@ -300,7 +330,7 @@ struct LlbShared {
 
 // One LLB tile: WG frontier entries (contiguous, or strided by n_tiles for
 // SLB), block scan of their degrees, cooperative expansion of their edges.
-template <typename PtrT, int WG>
+template <typename PtrT, int WG, bool HOT = false>
 __device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh, uint32_t tile,
                                          uint32_t n_tiles, bool strided,
                                          unsigned long long &dacc) {
@@ -339,7 +369,7 @@ __device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh
   __syncthreads();
   // owner of edge i = last slot j with s_scan[j] <= i (empty slots tie with
   // the next one and are skipped by taking the last)
-  expand_edges<PtrT, WG, true>(a, col_ptr, s_scan, s_start, s_v, PtrT(0), total, dacc);
+  expand_edges<PtrT, WG, true, HOT>(a, col_ptr, s_scan, s_start, s_v, PtrT(0), total, dacc);
   if (a.hub_degree) {
     const uint32_t nh = s_hub_n;
     const uint32_t w = t >> 6, l = t & 63;
@@ -348,18 +378,18 @@ __device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh
       const PtrT b = s_start[slot];
       const PtrT e1 = col_ptr[s_v[slot] + 1];
       for (PtrT e = b + l; e < e1; e += 64)
-        discover<PtrT>(a, col_ptr, a.row_idx[e], s_v[slot], dacc);
+        discover<PtrT, HOT>(a, col_ptr, a.row_idx[e], s_v[slot], dacc);
     }
   }
 }
 
 // Host-sized launch: one tile per work-group, grid = ceil(F / WG).  The
 // degree sum of exact-once discoveries leaves with one atomic per group.
-template <typename PtrT, int WG, bool STRIDED>
+template <typename PtrT, int WG, bool STRIDED, bool HOT>
 __global__ __launch_bounds__(WG) void k_llb(KArgs a) {
   __shared__ LlbShared<PtrT, WG> sh;
   unsigned long long dacc = 0;
-  llb_tile<PtrT, WG>(a, sh, blockIdx.x, gridDim.x, STRIDED, dacc);
+  llb_tile<PtrT, WG, HOT>(a, sh, blockIdx.x, gridDim.x, STRIDED, dacc);
   if (a.q_out_edges && !a.mark) block_atomic_add<WG>(dacc, a.q_out_edges);
 }
 
@@ -587,7 +617,7 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
       __syncthreads();
       const PtrT lo = e0 > s_win_lo ? e0 : s_win_lo;
       const PtrT hi = e1 < s_win_hi ? e1 : s_win_hi;
-      expand_edges<PtrT, WG, false>(a, col_ptr, s_scan, s_start, s_v, lo, hi, dacc);
+      expand_edges<PtrT, WG, false, false>(a, col_ptr, s_scan, s_start, s_v, lo, hi, dacc);
       __syncthreads();
     }
   }
@@ -1774,34 +1804,73 @@ static KArgs kargs(const StepArgs &s, uint32_t hub) {
   a.hub_degree = hub;
   a.mark = s.mark;
   a.examined = s.count_examined ? &s.ctl->examined : nullptr;
+  a.hot_vis = nullptr;
+  a.hot_id = nullptr;
   return a;
 }
 
-template <typename PtrT, bool STRIDED>
+// hot_vis bit h = visited bit of hot_id[h] (H a multiple of 64, and so is
+// the grid-stride: every wave runs whole rounds).
+__global__ void k_hot_bits(const uint32_t *vis, const uint32_t *hot_id, uint32_t H,
+                           uint32_t *hot_vis) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H; h += stride) {
+    const uint32_t v = hot_id[h];
+    const unsigned long long m = __ballot((vis[v >> 5] >> (v & 31)) & 1u);
+    if (lane == 0) hot_vis[h >> 5] = (uint32_t)m;
+    if (lane == 32) hot_vis[h >> 5] = (uint32_t)(m >> 32);
+  }
+}
+
+template <typename PtrT, bool STRIDED, bool HOT>
 static void launch_llb_t(bfsx_ctx *ctx, const KArgs &a) {
   const uint32_t wg = ctx->params.llb_wg;
   const unsigned grid = (unsigned)((a.F + wg - 1) / wg);
   hipStream_t s = ctx->stream;
   switch (wg) {
-    case 64: hipLaunchKernelGGL((k_llb<PtrT, 64, STRIDED>), dim3(grid), dim3(64), 0, s, a); break;
-    case 128: hipLaunchKernelGGL((k_llb<PtrT, 128, STRIDED>), dim3(grid), dim3(128), 0, s, a); break;
-    case 256: hipLaunchKernelGGL((k_llb<PtrT, 256, STRIDED>), dim3(grid), dim3(256), 0, s, a); break;
-    case 512: hipLaunchKernelGGL((k_llb<PtrT, 512, STRIDED>), dim3(grid), dim3(512), 0, s, a); break;
-    case 1024: hipLaunchKernelGGL((k_llb<PtrT, 1024, STRIDED>), dim3(grid), dim3(1024), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((k_llb<PtrT, 64, STRIDED, HOT>), dim3(grid), dim3(64), 0, s, a); break;
+    case 128: hipLaunchKernelGGL((k_llb<PtrT, 128, STRIDED, HOT>), dim3(grid), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((k_llb<PtrT, 256, STRIDED, HOT>), dim3(grid), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((k_llb<PtrT, 512, STRIDED, HOT>), dim3(grid), dim3(512), 0, s, a); break;
+    case 1024: hipLaunchKernelGGL((k_llb<PtrT, 1024, STRIDED, HOT>), dim3(grid), dim3(1024), 0, s, a); break;
     default: fail(BFSX_ERR_ARG, "llb_wg must be 64, 128, 256, 512 or 1024");
   }
 }
 
 void step_llb(bfsx_ctx *ctx, const StepArgs &s, bool strided) {
   if (s.F == 0) return;
-  const KArgs a = kargs(s, ctx->params.hub_degree);
+  KArgs a = kargs(s, ctx->params.hub_degree);
+  if (a.mark && s.hot_ok && ctx->hot_n && s.row_idx == ctx->row_idx.as<uint32_t>()) {
+    // a marking level probes the hot vertices' visited bits in an
+    // L2-resident bitmap, gathered first.  (Before most of them are visited,
+    // a hot probe that finds its vertex unvisited costs a dependent load of
+    // the vertex id: hot_ok is false then.)
+    a.row_idx = ctx->row_idx_hot.as<uint32_t>();
+    a.hot_vis = ctx->hot_vis.as<uint32_t>();
+    a.hot_id = ctx->hot_id.as<uint32_t>();
+    hipLaunchKernelGGL(k_hot_bits, dim3(cap_grid(ctx->hot_n / 256 + 1, (unsigned)ctx->cu_count * 8)),
+                       dim3(256), 0, ctx->stream, s.vis, a.hot_id, ctx->hot_n,
+                       ctx->hot_vis.as<uint32_t>());
+  }
   KernelTimer kt(s, ctx->stream);
+  const bool hot = a.hot_vis != nullptr;
   if (ctx->ptr_bytes == 8) {
-    if (strided) launch_llb_t<uint64_t, true>(ctx, a);
-    else launch_llb_t<uint64_t, false>(ctx, a);
+    if (hot) {
+      if (strided) launch_llb_t<uint64_t, true, true>(ctx, a);
+      else launch_llb_t<uint64_t, false, true>(ctx, a);
+    } else {
+      if (strided) launch_llb_t<uint64_t, true, false>(ctx, a);
+      else launch_llb_t<uint64_t, false, false>(ctx, a);
+    }
   } else {
-    if (strided) launch_llb_t<uint32_t, true>(ctx, a);
-    else launch_llb_t<uint32_t, false>(ctx, a);
+    if (hot) {
+      if (strided) launch_llb_t<uint32_t, true, true>(ctx, a);
+      else launch_llb_t<uint32_t, false, true>(ctx, a);
+    } else {
+      if (strided) launch_llb_t<uint32_t, true, false>(ctx, a);
+      else launch_llb_t<uint32_t, false, false>(ctx, a);
+    }
   }
   BFSX_HIP(hipGetLastError());
 }

@@ -79,12 +79,17 @@ static void ensure_pull_state(bfsx_ctx *ctx) {
   ensure(ctx->fsum, sizeof(uint32_t) * bfsx::PULL_SUM_WORDS_MAX, ctx->stream);
 }
 
-// the {first neighbour, degree} array and its hub index
+// the {first neighbour, degree} array and its hub index, and the top-down
+// copy with hot neighbours named by index
 static void drop_pull_fd(bfsx_ctx *ctx) {
   release(ctx->pull_fd);
   release(ctx->pull_hub);
   release(ctx->pull_hbm);
   ctx->pull_hubs = 0;
+  release(ctx->row_idx_hot);
+  release(ctx->hot_id);
+  release(ctx->hot_vis);
+  ctx->hot_n = 0;
 }
 
 static void after_graph_change(bfsx_ctx *ctx) {
@@ -269,6 +274,8 @@ struct Walker {
     a.bucket_ok = br != 0xFFFFFFFFu && (double)UE * 1024.0 >= (double)ctx->nnz * br &&
                   (use_mark || EF * 256 >= ctx->n_cols) && mr.lo_E != ~0ull;
     a.mark = use_mark ? ctx->mark.as<uint8_t>() : nullptr;
+    // the hot copy pays once the visited vertices hold half the edges
+    a.hot_ok = VE * 2 >= ctx->nnz;
     // next frontier bitmap of a top-down step (kept for bottom-up steps)
     uint32_t *td_fbm_next = pull_possible ? ctx->fbm[fbm_cur ^ 1].as<uint32_t>() : nullptr;
     bool bucketed = false;
@@ -968,6 +975,7 @@ int bfsx_graph_info_get(bfsx_ctx *ctx, bfsx_graph_info *out) {
     out->max_degree = ctx->max_degree;
     out->n_isolated = ctx->n_isolated;
     out->pull_hubs = ctx->pull_hubs;
+    out->hot_vertices = ctx->hot_n;
   });
 }
 
@@ -988,6 +996,7 @@ int bfsx_order_adjacency(bfsx_ctx *ctx, int order) {
         first_neighbours(ctx, ctx->row_idx_pull.as<uint32_t>(), ctx->pull_fd.as<uint2>());
         index_hub_neighbours(ctx);
       }
+      build_hot_copy(ctx);
       ctx->gflags |= BFSX_GRAPH_PULL_COPY;
       return;
     }

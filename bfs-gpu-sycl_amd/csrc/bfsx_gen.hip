@@ -540,6 +540,52 @@ void index_hub_neighbours(bfsx_ctx *ctx) {
   ctx->pull_hubs = (uint32_t)H;
 }
 
+// out[k] = PULL_HUB_FLAG | hrank[x] for hot x = row_idx[k], else x
+__global__ void k_flag_hot(const uint32_t *row_idx, uint64_t nnz, const uint32_t *hrank,
+                           uint32_t *out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += stride) {
+    const uint32_t x = row_idx[k];
+    const uint32_t h = hrank[x];
+    out[k] = h != ~0u ? PULL_HUB_FLAG | h : x;
+  }
+}
+
+void build_hot_copy(bfsx_ctx *ctx) {
+  release(ctx->row_idx_hot);
+  release(ctx->hot_id);
+  release(ctx->hot_vis);
+  ctx->hot_n = 0;
+  const uint32_t n = ctx->n_rows;
+  // Only where the visited bitmap outgrows an XCD's L2 (4 MiB) by enough
+  // that most probes miss it: below 2^24 vertices (a 2 MiB bitmap) the
+  // whole bitmap stays in L2 and the hot path only adds work (fixed LLB on
+  // R-MAT 19 / 22 lost 22 % / 13 % with it).
+  if (ctx->n_cols >= PULL_HUB_FLAG || ctx->n_cols < HOT_MIN_VERTICES || ctx->nnz == 0) return;
+  // one vertex in 16, in whole waves of 64, at most 2^22 (a 512 KiB bitmap;
+  // at R-MAT scale 26 one in 8 was no faster)
+  uint64_t H = ((uint64_t)n / 16 + 63) & ~63ull;
+  H = std::max<uint64_t>(H, 64);
+  H = std::min<uint64_t>(H, 1ull << 22);  // (and H <= n)
+  hipStream_t s = ctx->stream;
+  ensure(ctx->hot_id, sizeof(uint32_t) * H, s);
+  ensure(ctx->hot_vis, sizeof(uint32_t) * (H / 32), s);
+  ensure(ctx->row_idx_hot, sizeof(uint32_t) * ctx->nnz, s);
+  top_degree_vertices(ctx, (uint32_t)H, ctx->hot_id.as<uint32_t>());
+  DevBuf hrank;
+  ensure(hrank, sizeof(uint32_t) * ctx->n_cols, s);
+  BFSX_HIP(hipMemsetAsync(hrank.p, 0xFF, sizeof(uint32_t) * ctx->n_cols, s));
+  hipLaunchKernelGGL(k_hub_rank, dim3(grid_for(H)), dim3(256), 0, s, ctx->hot_id.as<uint32_t>(),
+                     (uint32_t)H, hrank.as<uint32_t>());
+  hipLaunchKernelGGL(k_flag_hot, dim3(grid_for(ctx->nnz)), dim3(256), 0, s,
+                     ctx->row_idx.as<uint32_t>(), ctx->nnz, hrank.as<uint32_t>(),
+                     ctx->row_idx_hot.as<uint32_t>());
+  BFSX_HIP(hipGetLastError());
+  BFSX_HIP(hipStreamSynchronize(s));
+  release(hrank);
+  ctx->hot_n = (uint32_t)H;
+}
+
 void order_adjacency(bfsx_ctx *ctx, bool hub_first, uint32_t *out) {
   if (ctx->ptr_bytes == 8) order_adjacency_t<uint64_t>(ctx, hub_first, out);
   else order_adjacency_t<uint32_t>(ctx, hub_first, out);

@@ -145,6 +145,13 @@ struct bfsx_ctx {
   // (pull_hubs bits), a bitmap small enough to stay in L2.
   bfsx::DevBuf pull_hub, pull_hbm;
   uint32_t pull_hubs = 0;  // a multiple of 64; 0: pull_fd holds plain ids
+  // Top-down copy of row_idx in which every neighbour among the hot_n
+  // vertices of largest degree is named by its index (PULL_HUB_FLAG | h for
+  // hot_id[h]); heavy LLB/SLB levels read it and probe those neighbours in
+  // hot_vis (hot_n bits, rebuilt from the visited bitmap at each such level,
+  // small enough to stay in L2).  Built with the bottom-up copy.
+  bfsx::DevBuf row_idx_hot, hot_id, hot_vis;
+  uint32_t hot_n = 0;
 
   // per-search state, sized for the resident graph
   bfsx::DevBuf levels, parents, vis, fbm[2], queue[2], ctl, mark;
@@ -221,6 +228,10 @@ void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint2 *fd);
 // by their index in ctx->pull_hub (sets pull_hub, pull_hbm, pull_hubs); needs
 // ctx->pull_fd and n_cols < 2^31.
 void index_hub_neighbours(bfsx_ctx *ctx);
+// Builds row_idx_hot / hot_id / hot_vis / hot_n for graphs of
+// HOT_MIN_VERTICES to 2^31 vertices (otherwise leaves hot_n = 0).
+constexpr uint32_t HOT_MIN_VERTICES = 1u << 24;
+void build_hot_copy(bfsx_ctx *ctx);
 // The graph with vertices renamed by (degree descending, id ascending):
 // rank[v] = new id of v, order[i] = old id of new vertex i, new_cp / new_ri
 // the relabelled CSC (lists ascending in new ids), col_ptr of the graph's
@@ -256,6 +267,8 @@ struct StepArgs {
   hipEvent_t ev_k0, ev_k1; // non-null: recorded around the step's main kernel
   unsigned long long EF;   // frontier edges (out-degree sum of q_in)
   bool bucket_ok;          // a marking GLB step may use bucketed discovery
+  bool hot_ok;             // a marking LLB/SLB step may read the hot copy (most
+                           // hot vertices visited: their probes then stop there)
 };
 
 void init_search(bfsx_ctx *ctx, uint32_t root, bool symmetric_iso, uint32_t *parents);

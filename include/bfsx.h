@@ -99,6 +99,10 @@ typedef struct bfsx_graph_info {
   uint32_t pull_hubs;    /* top-degree vertices that bottom-up steps probe in
                             a separate hub bitmap (BFSX_ORDER_PULL_HUB_FIRST
                             builds it; 0: none) */
+  uint32_t hot_vertices; /* top-degree vertices that heavy top-down LLB/SLB
+                            levels probe in a separate visited bitmap
+                            (BFSX_ORDER_PULL_HUB_FIRST builds it on graphs of
+                            2^24 vertices or more; 0: none) */
 } bfsx_graph_info;
 
 typedef struct bfsx_run_stats {

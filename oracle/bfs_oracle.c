@@ -194,6 +194,17 @@ uint64_t oracle_check_parents(uint32_t n_rows, uint32_t n_cols,
                               const void *col_ptr, int ptr_bytes,
                               const uint32_t *row_idx, uint32_t root,
                               const int32_t *levels, const uint32_t *parent) {
+  /* whether each list is in ascending order (then a binary search finds v;
+     otherwise a scan): a hub is the parent of many vertices, and scanning
+     its list once per child would be quadratic */
+  uint8_t *sorted = calloc(n_rows ? n_rows : 1, 1);
+  if (!sorted) return UINT64_MAX;
+  for (uint32_t u = 0; u < n_rows; u++) {
+    uint64_t b = cp_at(col_ptr, ptr_bytes, u), e = cp_at(col_ptr, ptr_bytes, (uint64_t)u + 1);
+    int ok = 1;
+    for (uint64_t i = b + 1; i < e && ok; i++) ok = row_idx[i - 1] <= row_idx[i];
+    sorted[u] = (uint8_t)ok;
+  }
   uint64_t bad = 0;
   for (uint32_t v = 0; v < n_cols; v++) {
     if (levels[v] < 0) { bad += parent[v] != 0xFFFFFFFFu; continue; }
@@ -203,8 +214,17 @@ uint64_t oracle_check_parents(uint32_t n_rows, uint32_t n_cols,
     uint64_t b = cp_at(col_ptr, ptr_bytes, u);
     uint64_t e = cp_at(col_ptr, ptr_bytes, (uint64_t)u + 1);
     int found = 0;
-    for (uint64_t i = b; i < e && !found; i++) found = row_idx[i] == v;
+    if (sorted[u]) {
+      while (b < e) {  /* first entry >= v */
+        uint64_t m = b + (e - b) / 2;
+        if (row_idx[m] < v) b = m + 1; else e = m;
+      }
+      found = b < cp_at(col_ptr, ptr_bytes, (uint64_t)u + 1) && row_idx[b] == v;
+    } else {
+      for (uint64_t i = b; i < e && !found; i++) found = row_idx[i] == v;
+    }
     bad += !found;
   }
+  free(sorted);
   return bad;
 }

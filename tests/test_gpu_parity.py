@@ -294,6 +294,30 @@ def test_pull_scale24_hub_first_copy(engine):
         assert np.array_equal(lev, want) and st["edges_count"] == ec, int(r)
 
 
+def test_llb_scale24_hot_copy(engine):
+    """R-MAT scale 24 (2^24 vertices, the smallest graph that gets the hot
+    copy): fixed LLB and SLB, whose heavy levels after half the edges are
+    visited read the top-down copy with the 2^20 top-degree neighbours named
+    by index and probe them in the hot bitmap; levels bit-exact and parents
+    valid, with the default rule and with every level marking."""
+    say("rmat24 hot copy")
+    engine.generate_rmat(24, 16, 0.57, 0.19, 0.19, 95, 96, 8)
+    engine.order_adjacency("pull_hub_first")
+    assert engine.info()["hot_vertices"] == 1 << 20
+    cp, ri = engine.download_csc()
+    roots = engine.pick_roots(2, 97)
+    old = engine.params
+    try:
+        for ratio in (old["mark_ratio_q10"], 0):
+            engine.set_params(mark_ratio_q10=ratio)
+            assert_levels(engine, cp, ri, roots, ["llb", "slb"], parents=True)
+            say("rmat24 hot copy, mark ratio", ratio, "exact")
+    finally:
+        engine.set_params(**old)
+    engine.order_adjacency("id")
+    assert engine.info()["hot_vertices"] == 0
+
+
 def test_directed_graph_with_fewer_rows(engine):
     """Directed input, duplicates, self loops, n_rows < n_cols (reference quirk)."""
     rng = np.random.default_rng(9)

@@ -248,6 +248,37 @@ def test_check_parents():
     assert cpu.check_parents(cp, ri, 0, lev, bad) == 1
 
 
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_check_parents_against_brute_force(shuffle):
+    """The checker (binary search in ascending lists, a scan in others)
+    against a set of edges, on a random graph with planted wrong parents."""
+    rng = np.random.default_rng(5)
+    cp, ri = random_graph(rng, 400, 3000)
+    if shuffle:  # lists out of order: the scanning path
+        ri = ri.copy()
+        for u in range(400):
+            seg = ri[cp[u]:cp[u + 1]]
+            ri[cp[u]:cp[u + 1]] = seg[::-1]
+    root = int(np.argmax(np.diff(cp.astype(np.int64))))
+    lev, _, _ = cpu.bfs_queue(cp, ri, root)
+    edges = {(u, int(v)) for u in range(400) for v in ri[cp[u]:cp[u + 1]]}
+    par = np.full(400, 0xFFFFFFFF, np.uint32)
+    for v in range(400):
+        if lev[v] > 0:
+            par[v] = next(u for u in range(400) if lev[u] == lev[v] - 1 and (u, v) in edges)
+    par[root] = root
+    assert cpu.check_parents(cp, ri, root, lev, par) == 0
+    bad = par.copy()
+    want = 0
+    for v in rng.choice(np.flatnonzero(lev > 0), 40, replace=False):
+        bad[v] = rng.integers(0, 400)
+    for v in range(400):
+        if lev[v] > 0:
+            u = int(bad[v])
+            want += not (lev[u] == lev[v] - 1 and (u, v) in edges)
+    assert want > 0 and cpu.check_parents(cp, ri, root, lev, bad) == want
+
+
 # ---------------------------------------------------------------------------
 # graph recipe restatement
 # ---------------------------------------------------------------------------
