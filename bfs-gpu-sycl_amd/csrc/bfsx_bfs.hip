@@ -1639,8 +1639,9 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
 // by grid-wide barriers instead of kernel launches and host round trips.  A
 // level runs while it does not mark (exact-once discovery, the regime where
 // a level costs microseconds) and its kind is
-// LLB or SLB (the fixed kind, or the scheduler's choice, made by work-group
-// 0 at each level end exactly as k_level_end makes it).  The loop leaves at
+// LLB or SLB (the fixed kind, or the scheduler's choice, made at each level
+// end, inside the barrier, by the last work-group to arrive, exactly as
+// k_level_end makes it).  The loop leaves at
 // the first level that is heavy, or for which the scheduler picks GLB or
 // PULL, or when the frontier is empty, or after max_levels; work-group 0
 // then writes the tagged level-end words (with the level reached) for the
@@ -1657,10 +1658,43 @@ struct CoopArgs {
   LevelMirror *mirror;
 };
 
+// Grid-wide barrier of a cooperative launch (every work-group is resident),
+// with the level's bookkeeping inside it: one arrival atomic per
+// work-group; the last group to arrive runs `body` (all its threads) and
+// then resets the count and bumps the generation the others wait for (one
+// barrier per level instead of one before and one after the bookkeeping).
+// The fences make the writes of every group before the barrier, and of
+// `body`, visible to every group after it.
+template <class Body>
+__device__ __forceinline__ void grid_barrier_last(LevelCtl *ctl, uint32_t n_groups, Body body) {
+  __shared__ uint32_t s_last, s_gen;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_gen = __hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    s_last = atomicAdd(&ctl->bar_count, 1u) == n_groups - 1;
+  }
+  __syncthreads();
+  if (s_last) {  // (uniform over the work-group)
+    __threadfence();
+    body();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __hip_atomic_store(&ctl->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+      __hip_atomic_fetch_add(&ctl->bar_gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  } else if (threadIdx.x == 0) {
+    // relaxed polls (no cache invalidation per poll), one acquire after
+    while (__hip_atomic_load(&ctl->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == s_gen)
+      __builtin_amdgcn_s_sleep(2);
+    __threadfence();
+  }
+  __syncthreads();
+}
+
 template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_coop_levels(KArgs base, LevelCtl *ctl, CoopArgs c) {
-  namespace cg = cooperative_groups;
-  cg::grid_group grid = cg::this_grid();
   __shared__ LlbShared<PtrT, WG> sh;
   uint32_t cur = c.cur;
   for (uint32_t it = 0; it < c.max_levels; it++) {
@@ -1683,10 +1717,9 @@ __global__ __launch_bounds__(WG) void k_coop_levels(KArgs base, LevelCtl *ctl, C
     for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
       llb_tile<PtrT, WG>(a, sh, tile, n_tiles, choice == BFSX_SLB, dacc);
     block_atomic_add<WG>(dacc, a.q_out_edges);
-    grid.sync();
-    if (blockIdx.x == 0)
+    grid_barrier_last(ctl, gridDim.x, [&] {
       level_end_body(ctl, cur, cur ^ 1, 0, choice, c.Wg, c.n_cols, c.nnz, c.n_iso, c.allow, 1);
-    grid.sync();
+    });
     cur ^= 1;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) write_level_end(ctl, cur, c.mirror);

@@ -118,7 +118,7 @@ static bfsx_params default_params() {
   p.ms_push_divisor = 14;
   p.ms_pull_hub = 256;
   p.ms_graph_levels = 8;
-  p.coop_groups = 32;
+  p.coop_groups = 256;
   return p;
 }
 

@@ -44,6 +44,12 @@ struct LevelCtl {
   // batched searches: hub-list length of a top-down level, and whether the
   // previous level ran bottom-up (choice holds this level's direction)
   uint32_t ms_hub_count, ms_prev_pull;
+  // grid-wide barrier of the device loop: arrivals, and the generation the
+  // waiting work-groups watch (0 at search start; every barrier leaves the
+  // count at 0)
+  uint32_t bar_count;
+  uint32_t pad_[30];  // bar_gen on another 128-byte line: polls do not hit the arrivals' line
+  uint32_t bar_gen;
   uint32_t reserved;
   // Level ends so far on this context: only the level-end kernel changes it
   // (+1 each), so the tag it puts on the host-mapped words never repeats;

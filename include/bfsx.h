@@ -181,8 +181,9 @@ typedef struct bfsx_params {
   uint32_t ms_graph_levels; /* levels per captured hipGraph launch (2..64, even) */
   uint32_t coop_groups;     /* BFSX_RUN_DEVICE_LOOP: work-groups of the
                                cooperative launch (1..4096, capped by what
-                               fits resident; grid-wide barriers cost more
-                               the more groups take part) */
+                               fits resident; default 256; grid-wide
+                               barriers cost more the more groups take
+                               part) */
 } bfsx_params;
 
 /* ---- context ------------------------------------------------------------ */
