@@ -120,6 +120,16 @@ __device__ __forceinline__ void block_atomic_add(unsigned long long v, unsigned 
   }
 }
 
+// Out-degree of vertex v (v below the padded bitmap length) from the 16-bit
+// array, col_ptr only when it saturated.
+template <typename PtrT>
+__device__ __forceinline__ unsigned long long degree16(const uint16_t *deg16, const PtrT *col_ptr,
+                                                       uint32_t v) {
+  const uint32_t d = deg16[v];
+  return d != DEG16_ESCAPE ? (unsigned long long)d
+                           : (unsigned long long)(col_ptr[v + 1] - col_ptr[v]);
+}
+
 // Out-degree of a discovered vertex (for the next frontier's edge count).
 template <typename PtrT>
 __device__ __forceinline__ unsigned long long out_degree(const PtrT *col_ptr, uint32_t n,
@@ -833,7 +843,8 @@ __global__ __launch_bounds__(1024) void k_bucket_base(const uint64_t *tot, uint3
 
 // A work-group per slice: new vertices of the slice from its targets.
 template <typename PtrT, int WG>
-__global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32_t n_rows,
+__global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr,
+                                                     const uint16_t *deg16, uint32_t n_rows,
                                                      uint32_t n_cols, const uint16_t *bkt,
                                                      const uint64_t *base, uint64_t cap,
                                                      uint32_t *vis,
@@ -911,47 +922,49 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr, uint32
     s_dense = run * 8u >= WPS * 32u;
   }
   __syncthreads();
-  // a slice with at least 1/8 of its vertices new loads col_ptr for every
-  // lane (no branch around the loads: the rounds' loads are in flight
-  // together); a sparse one only for its new vertices
+  // The next frontier's degree sum from the 16-bit degrees (2 bytes per
+  // vertex, col_ptr only for saturated ones): a slice with at least 1/8 of
+  // its vertices new loads them for every lane (no branch around the loads:
+  // the rounds' loads are in flight together); a sparse one only for its new
+  // vertices.  (Two col_ptr entries per vertex made the pass over a dense
+  // slice read the whole col_ptr array: 0.5 GB at scale 26.)
   const bool dense = s_dense;
   uint32_t pos = s_qbase + s_wcnt[wv];
   unsigned long long dsum = 0;
-  // eight rounds at a time, so their col_ptr loads are in flight together
+  // eight rounds at a time, so their loads are in flight together
   constexpr uint32_t R = 8;
   for (uint32_t i0 = wv * WPW; i0 < (wv + 1) * WPW; i0 += 2 * R) {  // two words = 64 vertices
-    PtrT lo_[R], hi_[R];
+    uint32_t d_[R];
     bool take[R];
 #pragma unroll
     for (uint32_t r = 0; r < R; r++) {
       const uint32_t i = i0 + 2 * r;
-      lo_[r] = hi_[r] = 0;
+      d_[r] = 0;
       take[r] = false;
       if (w0 + i >= n_words) continue;
       const uint32_t word = s_found[i + (lane >> 5)];
       const bool isnew = (word >> (lane & 31)) & 1u;
       const unsigned long long m = __ballot(isnew);
-      const uint32_t v = (w0 + i) * 32 + lane;
+      const uint32_t v = (w0 + i) * 32 + lane;  // below the padded length
       if (isnew) {
         levels[v] = next_level;
         q_out[pos + lanes_below(m)] = v;
       }
       if (q_edges) {
         take[r] = isnew && v < n_rows;
-        if (dense) {
-          const uint32_t vv = v < n_rows ? v : (n_rows ? n_rows - 1 : 0u);
-          lo_[r] = col_ptr[vv];
-          hi_[r] = col_ptr[vv + 1];
-        } else if (take[r]) {
-          lo_[r] = col_ptr[v];
-          hi_[r] = col_ptr[v + 1];
-        }
+        if (dense) d_[r] = deg16[v];
+        else if (take[r]) d_[r] = deg16[v];
       }
       pos += (uint32_t)__popcll(m);
     }
 #pragma unroll
-    for (uint32_t r = 0; r < R; r++)
-      if (take[r]) dsum += (unsigned long long)(hi_[r] - lo_[r]);
+    for (uint32_t r = 0; r < R; r++) {
+      if (take[r]) {
+        const uint32_t v = (w0 + i0 + 2 * r) * 32 + lane;
+        dsum += d_[r] != DEG16_ESCAPE ? (unsigned long long)d_[r]
+                                      : (unsigned long long)(col_ptr[v + 1] - col_ptr[v]);
+      }
+    }
   }
   if (q_edges) block_atomic_add<WG>(dsum, q_edges);
 }
@@ -1398,7 +1411,8 @@ __device__ __forceinline__ uint32_t mark_bits(uint4 m) {
 }
 
 template <typename PtrT>
-__device__ __forceinline__ void compact_body(const PtrT *col_ptr, uint32_t n_rows, uint8_t *mark,
+__device__ __forceinline__ void compact_body(const PtrT *col_ptr, const uint16_t *deg16,
+                                             uint32_t n_rows, uint8_t *mark,
                                              uint32_t n_chunks, uint32_t chunks_per_group,
                                              uint32_t *vis, int32_t *levels, int32_t next_level,
                                              uint32_t *q_out, uint32_t *q_count,
@@ -1442,7 +1456,7 @@ __device__ __forceinline__ void compact_body(const PtrT *col_ptr, uint32_t n_row
         const uint32_t v = v0 + (__ffs(b) - 1);
         levels[v] = next_level;
         q_out[pos++] = v;
-        if (q_edges && v < n_rows) dsum += (unsigned long long)(col_ptr[v + 1] - col_ptr[v]);
+        if (q_edges && v < n_rows) dsum += degree16(deg16, col_ptr, v);
       }
     }
     const uint32_t partner = __shfl_xor(bits, 1, 64);
@@ -1466,28 +1480,29 @@ __device__ __forceinline__ void compact_body(const PtrT *col_ptr, uint32_t n_row
 }
 
 template <typename PtrT>
-__global__ __launch_bounds__(COMPACT_WG) void k_compact(const PtrT *col_ptr, uint32_t n_rows,
+__global__ __launch_bounds__(COMPACT_WG) void k_compact(const PtrT *col_ptr,
+                                                        const uint16_t *deg16, uint32_t n_rows,
                                                         uint8_t *mark, uint32_t n_chunks,
                                                         uint32_t chunks_per_group, uint32_t *vis,
                                                         int32_t *levels, int32_t next_level,
                                                         uint32_t *q_out, uint32_t *q_count,
                                                         unsigned long long *q_edges,
                                                         uint32_t *fbm_next) {
-  compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, chunks_per_group, vis, levels, next_level,
-                     q_out, q_count, q_edges, fbm_next);
+  compact_body<PtrT>(col_ptr, deg16, n_rows, mark, n_chunks, chunks_per_group, vis, levels,
+                     next_level, q_out, q_count, q_edges, fbm_next);
 }
 
 // Compaction inside a captured level loop: runs only when the step of this
 // level marked (same rule as k_td_dev), level from the control block.
 template <typename PtrT>
 __global__ __launch_bounds__(COMPACT_WG) void k_compact_dev(
-    const PtrT *col_ptr, uint32_t n_rows, uint8_t *mark, uint32_t n_chunks,
+    const PtrT *col_ptr, const uint16_t *deg16, uint32_t n_rows, uint8_t *mark, uint32_t n_chunks,
     uint32_t chunks_per_group, uint32_t *vis, int32_t *levels, LevelCtl *ctl, DevStep d,
     uint32_t *q_out, int track, uint32_t *fbm_next) {
   const uint32_t F = ctl->q_count[d.cur];
   if (F == 0 || !d.mark.marks(ctl->q_edges[d.cur], F)) return;
   const uint32_t out = d.cur ^ 1;
-  compact_body<PtrT>(col_ptr, n_rows, mark, n_chunks, chunks_per_group, vis, levels,
+  compact_body<PtrT>(col_ptr, deg16, n_rows, mark, n_chunks, chunks_per_group, vis, levels,
                      (int32_t)ctl->level + 1, q_out, &ctl->q_count[out],
                      track ? &ctl->q_edges[out] : nullptr, fbm_next);
 }
@@ -1934,7 +1949,8 @@ static void glb_bucketed(bfsx_ctx *ctx, const StepArgs &s, const KArgs &a, const
   hipLaunchKernelGGL((k_glb_bucket<PtrT, BKT_WG, 2>), dim3(grid), dim3(BKT_WG), 0, st, a, g_scan,
                      g_start, clo, s.ctl, chunk, n_slices, cnt, base, bkt, cap);
   hipLaunchKernelGGL((k_bucket_apply<PtrT, 512>), dim3(n_slices), dim3(512), 0, st,
-                     static_cast<const PtrT *>(s.col_ptr), s.n_rows, s.n_cols, bkt, base, cap, s.vis,
+                     static_cast<const PtrT *>(s.col_ptr), ctx->deg16.as<uint16_t>(), s.n_rows,
+                     s.n_cols, bkt, base, cap, s.vis,
                      s.levels, s.next_level, s.q_out, &s.ctl->q_count[s.q_out_idx],
                      s.track_degrees ? &s.ctl->q_edges[s.q_out_idx] : nullptr, s.fbm_next);
   BFSX_HIP(hipGetLastError());
@@ -2126,11 +2142,13 @@ void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next
   uint32_t *qc = &s.ctl->q_count[s.q_out_idx];
   if (ctx->ptr_bytes == 8)
     hipLaunchKernelGGL(k_compact<uint64_t>, dim3(c.grid), dim3(COMPACT_WG), 0, ctx->stream,
-                       static_cast<const uint64_t *>(s.col_ptr), s.n_rows, mark, c.n_chunks,
+                       static_cast<const uint64_t *>(s.col_ptr), ctx->deg16.as<uint16_t>(),
+                       s.n_rows, mark, c.n_chunks,
                        c.per_group, s.vis, s.levels, s.next_level, s.q_out, qc, qe, fbm_next);
   else
     hipLaunchKernelGGL(k_compact<uint32_t>, dim3(c.grid), dim3(COMPACT_WG), 0, ctx->stream,
-                       static_cast<const uint32_t *>(s.col_ptr), s.n_rows, mark, c.n_chunks,
+                       static_cast<const uint32_t *>(s.col_ptr), ctx->deg16.as<uint16_t>(),
+                       s.n_rows, mark, c.n_chunks,
                        c.per_group, s.vis, s.levels, s.next_level, s.q_out, qc, qe, fbm_next);
   BFSX_HIP(hipGetLastError());
 }
@@ -2229,7 +2247,8 @@ static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, Mark
   }
   const CompactShape c = compact_shape(ctx, ctx->n_cols);
   hipLaunchKernelGGL(k_compact_dev<PtrT>, dim3(c.grid), dim3(COMPACT_WG), 0, s,
-                     static_cast<const PtrT *>(ctx->col_ptr.p), ctx->n_rows,
+                     static_cast<const PtrT *>(ctx->col_ptr.p), ctx->deg16.as<uint16_t>(),
+                     ctx->n_rows,
                      ctx->mark.as<uint8_t>(), c.n_chunks, c.per_group, ctx->vis.as<uint32_t>(),
                      ctx->levels.as<int32_t>(), ctl, d, ctx->queue[cur ^ 1].as<uint32_t>(),
                      track ? 1 : 0, (uint32_t *)nullptr);

@@ -835,6 +835,7 @@ void bfsx_destroy(bfsx_ctx *ctx) {
   release(ctx->row_idx_pull);
   drop_pull_fd(ctx);
   release(ctx->iso_bm);
+  release(ctx->deg16);
   release(ctx->ctl);
   release(ctx->scratch);
   for (DevBuf *b : {&ctx->sched_w, &ctx->snap_levels, &ctx->snap_vis, &ctx->snap_ctl})

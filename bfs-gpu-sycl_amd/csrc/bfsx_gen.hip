@@ -335,9 +335,26 @@ __global__ void k_summary(const PtrT *col_ptr, uint32_t n_rows, uint32_t n_cols,
   }
 }
 
+template <typename PtrT>
+__global__ void k_deg16(const PtrT *col_ptr, uint32_t n_rows, uint64_t n_pad, uint16_t *deg16) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < n_pad;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t d = v < n_rows ? (uint64_t)(col_ptr[v + 1] - col_ptr[v]) : 0ull;
+    deg16[v] = (uint16_t)(d < DEG16_ESCAPE ? d : DEG16_ESCAPE);
+  }
+}
+
 void graph_summary(bfsx_ctx *ctx) {
   hipStream_t s = ctx->stream;
   const uint32_t nwords = (uint32_t)bitmap_words(ctx->n_cols);
+  const uint64_t n_pad = (uint64_t)(nwords ? nwords : 1) * 32;
+  ensure(ctx->deg16, sizeof(uint16_t) * n_pad, s);
+  if (ctx->ptr_bytes == 8)
+    hipLaunchKernelGGL(k_deg16<uint64_t>, dim3(grid_for(n_pad)), dim3(256), 0, s,
+                       ctx->col_ptr.as<uint64_t>(), ctx->n_rows, n_pad, ctx->deg16.as<uint16_t>());
+  else
+    hipLaunchKernelGGL(k_deg16<uint32_t>, dim3(grid_for(n_pad)), dim3(256), 0, s,
+                       ctx->col_ptr.as<uint32_t>(), ctx->n_rows, n_pad, ctx->deg16.as<uint16_t>());
   ensure(ctx->iso_bm, sizeof(uint32_t) * (nwords ? nwords : 2), s);
   DevBuf out;
   ensure(out, 2 * sizeof(unsigned int), s);

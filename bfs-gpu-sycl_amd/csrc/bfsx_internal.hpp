@@ -108,6 +108,7 @@ __host__ __device__ inline uint64_t bitmap_words(uint64_t n) { return ((n + 1023
 
 // Largest frontier summary of the bottom-up step (words; 32 KiB).
 constexpr uint32_t PULL_SUM_WORDS_MAX = 8192;
+constexpr uint32_t DEG16_ESCAPE = 0xFFFFu;
 // fd[v].x with this bit set names a hub by its index (graphs below 2^31
 // vertices); at most PULL_HUBS_MAX hubs (a 32 KiB bitmap).
 constexpr uint32_t PULL_HUB_FLAG = 0x80000000u;
@@ -143,6 +144,10 @@ struct bfsx_ctx {
   uint32_t gflags = 0;
   uint32_t max_degree = 0, n_isolated = 0;
   bfsx::DevBuf col_ptr, row_idx, iso_bm;
+  // min(out-degree, DEG16_ESCAPE) per vertex, padded with zeros to whole
+  // visited-bitmap words: the degree sums of new frontiers read 2 bytes per
+  // vertex instead of two col_ptr entries (col_ptr only for the escapes)
+  bfsx::DevBuf deg16;
   bfsx::DevBuf row_idx_pull;  // hub-first copy read by bottom-up steps (optional)
   bfsx::DevBuf pull_fd;       // with it: {first (highest-degree) neighbour, degree} per vertex
   // The pull_hubs vertices of largest degree ("hubs") are named in pull_fd by
