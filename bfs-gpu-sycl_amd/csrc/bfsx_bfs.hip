@@ -251,11 +251,15 @@ __device__ __forceinline__ void load_edges(const KArgs &a, const PtrT *s_scan,
   }
 }
 
-template <typename PtrT, int WG, bool BLOCK_APPEND, bool HOT>
+// STAGE (GLB): exact-once discoveries go to the work-group's LDS buffer
+// s_buf (one LDS atomic per wave and edge group, on *s_n), which the caller
+// flushes to the queue with one global atomic per chunk.
+template <typename PtrT, int WG, bool BLOCK_APPEND, bool HOT, bool STAGE = false>
 __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr,
                                              const PtrT *s_scan, const PtrT *s_start,
                                              const uint32_t *s_v, PtrT lo, PtrT hi,
-                                             unsigned long long &dacc) {
+                                             unsigned long long &dacc,
+                                             uint32_t *s_buf = nullptr, uint32_t *s_n = nullptr) {
   constexpr PtrT PER_ROUND = (PtrT)WG * EPT;
   if (BLOCK_APPEND && !a.mark && hi > lo && hi - lo <= PER_ROUND * BLOCK_APPEND_ROUNDS) {
     // (uniform over the group: lo, hi and the mode are)
@@ -319,7 +323,8 @@ __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr
           if (a.parents) a.parents[n[k]] = from[k];
           if (a.q_out_edges) dacc += out_degree(col_ptr, n[k], a.n_rows);
         }
-        wave_append(isnew, n[k], a.q_out, a.q_out_count);
+        if (STAGE) wave_append(isnew, n[k], s_buf, s_n);  // (LDS)
+        else wave_append(isnew, n[k], a.q_out, a.q_out_count);
       }
     }
   }
@@ -586,7 +591,13 @@ __global__ __launch_bounds__(GLB_SMALL_WG) void k_glb_scan_small(
   }
 }
 
-template <typename PtrT, int WG>
+// STAGE: exact-once discoveries of a chunk (at most chunk <= GLB_STAGE_CAP)
+// are staged in LDS and appended with one global atomic per chunk: on a
+// level whose edges mostly find new vertices (the first level of an R-MAT
+// search), per-wave appends serialise on the one queue counter.
+constexpr uint32_t GLB_STAGE_CAP = 2048;
+
+template <typename PtrT, int WG, bool STAGE>
 __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
                                                    const PtrT *g_start,
                                                    const uint32_t *chunk_lo, LevelCtl *ctl,
@@ -596,13 +607,19 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
   __shared__ uint32_t s_v[WG];
   __shared__ PtrT s_win_lo, s_win_hi;
   __shared__ uint32_t s_c;
+  __shared__ uint32_t s_buf[STAGE ? GLB_STAGE_CAP : 1];
+  __shared__ uint32_t s_n, s_base;
   const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
   const uint32_t t = threadIdx.x;
   const uint32_t n_chunks = ctl->glb_chunks;
   const PtrT total = (PtrT)ctl->glb_total;
   unsigned long long dacc = 0;
-  for (;;) {
-    if (t == 0) s_c = atomicAdd(&ctl->glb_cursor, 1u);
+  if (STAGE && t == 0) s_n = 0;
+  // the first chunk of each work-group is its own index; later ones come
+  // from the cursor (chunks gridDim.x, gridDim.x + 1, ...): a level with no
+  // more chunks than work-groups takes no atomic on the one cursor address
+  for (bool first = true;; first = false) {
+    if (t == 0) s_c = first ? blockIdx.x : gridDim.x + atomicAdd(&ctl->glb_cursor, 1u);
     __syncthreads();
     const uint32_t c = s_c;
     __syncthreads();
@@ -627,8 +644,17 @@ __global__ __launch_bounds__(WG) void k_glb_expand(KArgs a, const PtrT *g_scan,
       __syncthreads();
       const PtrT lo = e0 > s_win_lo ? e0 : s_win_lo;
       const PtrT hi = e1 < s_win_hi ? e1 : s_win_hi;
-      expand_edges<PtrT, WG, false, false>(a, col_ptr, s_scan, s_start, s_v, lo, hi, dacc);
+      expand_edges<PtrT, WG, false, false, STAGE>(a, col_ptr, s_scan, s_start, s_v, lo, hi,
+                                                   dacc, s_buf, &s_n);
       __syncthreads();
+    }
+    if (STAGE) {
+      // the chunk's discoveries (at most `chunk` of them) to the queue
+      if (t == 0) s_base = s_n ? atomicAdd(a.q_out_count, s_n) : 0u;
+      __syncthreads();
+      for (uint32_t i = t; i < s_n; i += WG) a.q_out[s_base + i] = s_buf[i];
+      __syncthreads();
+      if (t == 0) s_n = 0;
     }
   }
   if (a.q_out_edges && !a.mark) block_atomic_add<WG>(dacc, a.q_out_edges);
@@ -2000,22 +2026,39 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
     return true;
   }
   KernelTimer kt(s, st);
+  const bool stage = !a.mark && chunk <= GLB_STAGE_CAP;
   switch (ctx->params.glb_wg) {
     case 128:
-      hipLaunchKernelGGL((k_glb_expand<PtrT, 128>), dim3(grid), dim3(128), 0, st, a, g_scan,
-                         g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      if (stage)
+        hipLaunchKernelGGL((k_glb_expand<PtrT, 128, true>), dim3(grid), dim3(128), 0, st, a,
+                           g_scan, g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      else
+        hipLaunchKernelGGL((k_glb_expand<PtrT, 128, false>), dim3(grid), dim3(128), 0, st, a,
+                           g_scan, g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
       break;
     case 256:
-      hipLaunchKernelGGL((k_glb_expand<PtrT, 256>), dim3(grid), dim3(256), 0, st, a, g_scan,
-                         g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      if (stage)
+        hipLaunchKernelGGL((k_glb_expand<PtrT, 256, true>), dim3(grid), dim3(256), 0, st, a,
+                           g_scan, g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      else
+        hipLaunchKernelGGL((k_glb_expand<PtrT, 256, false>), dim3(grid), dim3(256), 0, st, a,
+                           g_scan, g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
       break;
     case 512:
-      hipLaunchKernelGGL((k_glb_expand<PtrT, 512>), dim3(grid), dim3(512), 0, st, a, g_scan,
-                         g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      if (stage)
+        hipLaunchKernelGGL((k_glb_expand<PtrT, 512, true>), dim3(grid), dim3(512), 0, st, a,
+                           g_scan, g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      else
+        hipLaunchKernelGGL((k_glb_expand<PtrT, 512, false>), dim3(grid), dim3(512), 0, st, a,
+                           g_scan, g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
       break;
     case 1024:
-      hipLaunchKernelGGL((k_glb_expand<PtrT, 1024>), dim3(grid), dim3(1024), 0, st, a, g_scan,
-                         g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      if (stage)
+        hipLaunchKernelGGL((k_glb_expand<PtrT, 1024, true>), dim3(grid), dim3(1024), 0, st, a,
+                           g_scan, g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
+      else
+        hipLaunchKernelGGL((k_glb_expand<PtrT, 1024, false>), dim3(grid), dim3(1024), 0, st, a,
+                           g_scan, g_start, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl, chunk);
       break;
     default: fail(BFSX_ERR_ARG, "glb_wg must be 128, 256, 512 or 1024");
   }
