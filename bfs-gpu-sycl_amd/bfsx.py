@@ -76,7 +76,8 @@ class Params(C.Structure):
                 ("graph_levels", C.c_uint32), ("mark_ratio_q10", C.c_uint32),
                 ("bucket_ratio_q10", C.c_uint32), ("ms_relabel", C.c_uint32),
                 ("ms_push_divisor", C.c_uint32), ("ms_pull_hub", C.c_uint32),
-                ("ms_graph_levels", C.c_uint32), ("coop_groups", C.c_uint32)]
+                ("ms_graph_levels", C.c_uint32), ("coop_groups", C.c_uint32),
+                ("speculate", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -1075,9 +1075,23 @@ __device__ __forceinline__ bool in_frontier_fd(const uint32_t *fbm, const uint32
   return may && ((word >> (w & 31)) & 1u);
 }
 
+// A step launched ahead of the decision for its level (speculative) runs
+// only if that decision, made by the previous level's end, is a bottom-up
+// step and the frontier is not empty; otherwise its kernels, including its
+// level end, exit without a write.
+struct SpecGuard {
+  const LevelCtl *ctl;  // null: not speculative
+  uint32_t consumed;    // the queue counter of the level's frontier
+  __device__ __forceinline__ bool skip() const {
+    return ctl && (ctl->choice != BFSX_PULL || ctl->q_count[consumed] == 0);
+  }
+};
+
 // hbm bit h = frontier bit of hub[h] (H a multiple of 64, and so is the
 // grid-stride: every wave runs whole rounds).
-__global__ void k_hub_bits(const uint32_t *fbm, const uint32_t *hub, uint32_t H, uint32_t *hbm) {
+__global__ void k_hub_bits(const uint32_t *fbm, const uint32_t *hub, uint32_t H, uint32_t *hbm,
+                           SpecGuard guard) {
+  if (guard.skip()) return;
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t lane = threadIdx.x & 63;
   for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H; h += stride) {
@@ -1136,7 +1150,8 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
                                              uint32_t *fbm_next, const uint32_t *g_sum,
                                              uint32_t sum_words, uint32_t shift,
                                              const uint2 *fd, const uint32_t *hbm,
-                                             const uint32_t *hub) {
+                                             const uint32_t *hub, SpecGuard guard) {
+  if (guard.skip()) return;
   constexpr int NW = WG / 64;
   extern __shared__ uint32_t s_sum_mem[];  // sum_words entries when g_sum is set
   __shared__ uint32_t s_found[NW][64];
@@ -1656,6 +1671,7 @@ __device__ __forceinline__ void write_level_end(LevelCtl *ctl, uint32_t produced
   ctl->seq = seq;
   const unsigned long long t32 = (unsigned long long)seq << 32;
   const unsigned long long t16 = (unsigned long long)(seq & 0xFFFFu) << 48;
+  mirror += seq & 1u;  // the set for this parity
   mirror->w[0] = t32 | ctl->q_count[produced];
   mirror->w[1] = t16 | (ctl->q_edges[produced] & MIRROR_48);
   mirror->w[2] = t16 | (ctl->visited_edges & MIRROR_48);
@@ -1668,7 +1684,8 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
                                                   uint32_t produced, int stamp, int kind,
                                                   const float *Wg, uint64_t n_cols, uint64_t nnz,
                                                   uint64_t n_iso, uint32_t allow, int reuse,
-                                                  LevelMirror *mirror) {
+                                                  LevelMirror *mirror, SpecGuard guard) {
+  if (guard.skip()) return;
   level_end_body(ctl, consumed, produced, stamp, kind, Wg, n_cols, nnz, n_iso, allow, reuse);
   if (mirror && threadIdx.x == 0) write_level_end(ctl, produced, mirror);
 }
@@ -2088,7 +2105,7 @@ static uint32_t pull_summary_shift(uint64_t n, uint32_t max_words) {
 }
 
 template <typename PtrT>
-static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
+static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
   const KArgs a = kargs(s, 0);
   const uint32_t wg = ctx->params.pull_wg;
   const uint32_t n_chunks = (s.n_rows + PULL_CHUNK - 1) / PULL_CHUNK;
@@ -2102,7 +2119,10 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
   const uint32_t shift = pull_summary_shift(s.n_cols, max_words ? max_words : 1);
   const uint32_t sum_bits = (uint32_t)(((uint64_t)s.n_cols + (1ull << shift) - 1) >> shift);
   const uint32_t sum_words = (sum_bits + 31) / 32;
-  const bool use_sum = max_words >= 64 && ((uint64_t)s.F << shift) * 4 <= (uint64_t)s.n_cols;
+  // (a speculative step does not know its frontier's size: no summary)
+  const bool use_sum = !speculative && max_words >= 64 &&
+                       ((uint64_t)s.F << shift) * 4 <= (uint64_t)s.n_cols;
+  const SpecGuard guard{speculative ? s.ctl : nullptr, s.q_in_idx};
   const uint32_t *g_sum = nullptr;
   if (use_sum) {
     ensure(ctx->fsum, sizeof(uint32_t) * PULL_SUM_WORDS_MAX, st);
@@ -2130,33 +2150,34 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s) {
     hub = ctx->pull_hub.as<uint32_t>();
     hbm = ctx->pull_hbm.as<uint32_t>();
     hipLaunchKernelGGL(k_hub_bits, dim3(cap_grid(ctx->pull_hubs / 256 + 1, (unsigned)ctx->cu_count * 4)),
-                       dim3(256), 0, st, s.fbm_cur, hub, ctx->pull_hubs, ctx->pull_hbm.as<uint32_t>());
+                       dim3(256), 0, st, s.fbm_cur, hub, ctx->pull_hubs, ctx->pull_hbm.as<uint32_t>(),
+                       guard);
   }
   KernelTimer kt(s, st);
   switch (wg) {
     case 64:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 64, true>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
-      else hipLaunchKernelGGL((k_pull<PtrT, 64, false>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 64, true>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
+      else hipLaunchKernelGGL((k_pull<PtrT, 64, false>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
       break;
     case 128:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 128, true>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
-      else hipLaunchKernelGGL((k_pull<PtrT, 128, false>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 128, true>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
+      else hipLaunchKernelGGL((k_pull<PtrT, 128, false>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
       break;
     case 256:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 256, true>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
-      else hipLaunchKernelGGL((k_pull<PtrT, 256, false>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 256, true>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
+      else hipLaunchKernelGGL((k_pull<PtrT, 256, false>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
       break;
     case 512:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 512, true>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
-      else hipLaunchKernelGGL((k_pull<PtrT, 512, false>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 512, true>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
+      else hipLaunchKernelGGL((k_pull<PtrT, 512, false>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
       break;
     default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
   }
 }
 
-void step_pull(bfsx_ctx *ctx, const StepArgs &s) {
-  if (ctx->ptr_bytes == 8) step_pull_t<uint64_t>(ctx, s);
-  else step_pull_t<uint32_t>(ctx, s);
+void step_pull(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
+  if (ctx->ptr_bytes == 8) step_pull_t<uint64_t>(ctx, s, speculative);
+  else step_pull_t<uint32_t>(ctx, s, speculative);
   BFSX_HIP(hipGetLastError());
 }
 
@@ -2245,7 +2266,7 @@ void level_begin(bfsx_ctx *ctx) {
 }
 
 void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, int kind,
-               uint32_t decide_allow, bool reuse) {
+               uint32_t decide_allow, bool reuse, bool speculative) {
   const uint64_t n_iso = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols
                              ? ctx->n_isolated
                              : 0;
@@ -2253,7 +2274,8 @@ void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, 
                      consumed, produced, stamp ? 1 : 0, kind,
                      decide_allow ? ctx->sched_w.as<float>() : (const float *)nullptr,
                      (uint64_t)ctx->n_cols, (uint64_t)ctx->nnz, n_iso, decide_allow,
-                     reuse ? 1 : 0, ctx->d_mirror);
+                     reuse ? 1 : 0, ctx->d_mirror,
+                     SpecGuard{speculative ? ctx->ctl.as<LevelCtl>() : nullptr, consumed});
   BFSX_HIP(hipGetLastError());
 }
 

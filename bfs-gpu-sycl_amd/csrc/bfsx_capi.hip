@@ -119,6 +119,7 @@ static bfsx_params default_params() {
   p.ms_pull_hub = 256;
   p.ms_graph_levels = 8;
   p.coop_groups = 256;
+  p.speculate = 1;
   return p;
 }
 
@@ -135,6 +136,7 @@ static void validate_params(const bfsx_params &p) {
   if (p.ms_relabel > 1) fail(BFSX_ERR_ARG, "ms_relabel must be 0 or 1");
   if (p.coop_groups < 1 || p.coop_groups > 4096)
     fail(BFSX_ERR_ARG, "coop_groups must be in [1,4096]");
+  if (p.speculate > 1) fail(BFSX_ERR_ARG, "speculate must be 0 or 1");
   if (p.ms_push_divisor < 1 || p.ms_push_divisor > 1024) fail(BFSX_ERR_ARG, "ms_push_divisor must be in [1,1024]");
   if (p.ms_pull_hub < 32 || p.ms_pull_hub > 65536) fail(BFSX_ERR_ARG, "ms_pull_hub must be in [32,65536]");
   if (p.ms_graph_levels < 2 || p.ms_graph_levels > 64 || (p.ms_graph_levels & 1))
@@ -177,23 +179,30 @@ static void view_from_ctl(bfsx_ctx *ctx, uint32_t produced) {
 // After a second without them, the stream is synchronised, which reports any
 // device error, and the values come from a full copy of the control block.
 void read_level_end(bfsx_ctx *ctx, uint32_t seq, uint32_t produced, uint32_t level0) {
-  const volatile unsigned long long *w = ctx->h_mirror->w;
+  const volatile unsigned long long *w = ctx->h_mirror[seq & 1u].w;
   const unsigned long long t32 = seq, t16 = seq & 0xFFFFu;
   const auto t0 = std::chrono::steady_clock::now();
-  for (uint32_t spin = 0;; spin++) {
+  auto take = [&]() {
     const unsigned long long w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
-    if ((w0 >> 32) == t32 && (w1 >> 48) == t16 && (w2 >> 48) == t16 && (w3 >> 32) == t32) {
-      ctx->view.F = (uint32_t)w0;
-      ctx->view.EF = w1 & MIRROR_48;
-      ctx->view.VE = w2 & MIRROR_48;
-      ctx->view.choice = (uint32_t)(w3 & 0xFFu);
-      ctx->view.done = (uint32_t)((w3 >> 8) & 1u);
-      ctx->view.level = (uint32_t)((w3 >> 9) & MIRROR_LEVEL_MASK);
-      return;
-    }
+    if (!((w0 >> 32) == t32 && (w1 >> 48) == t16 && (w2 >> 48) == t16 && (w3 >> 32) == t32))
+      return false;
+    ctx->view.F = (uint32_t)w0;
+    ctx->view.EF = w1 & MIRROR_48;
+    ctx->view.VE = w2 & MIRROR_48;
+    ctx->view.choice = (uint32_t)(w3 & 0xFFu);
+    ctx->view.done = (uint32_t)((w3 >> 8) & 1u);
+    ctx->view.level = (uint32_t)((w3 >> 9) & MIRROR_LEVEL_MASK);
+    return true;
+  };
+  for (uint32_t spin = 0;; spin++) {
+    if (take()) return;
     if ((spin & 1023u) == 1023u &&
         std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
       read_ctl(ctx);  // synchronises the stream (reports device errors)
+      // every write has landed now; with a speculative level end after
+      // this one, the control block is already past it, but this level's
+      // words are in their own set
+      if (take()) return;
       if (ctx->h_ctl->seq != seq) fail(BFSX_ERR_DEVICE, "level end did not arrive");
       // after a device loop: the frontier's buffer follows the levels run
       if (level0 != ~0u) produced ^= (ctx->h_ctl->level - level0) & 1u;
@@ -313,6 +322,36 @@ struct Walker {
     level_end(ctx, cur, cur ^ 1, stamp, choice, decide_allow, decide_reuse);
     last = choice;
     return new_fbm;
+  }
+
+  // Launches, right after this level's kernels and before its end has been
+  // read, the bottom-up step of the next level, guarded on the device: it
+  // runs only if this level's end picks PULL for the next level and the
+  // frontier is not empty, so that a run of bottom-up levels does not wait
+  // for a host round trip per level.  Only after a bottom-up step (the next
+  // frontier then exists as the bitmap it wrote), learned, untimed.
+  void launch_speculative_pull() {
+    StepArgs a{};
+    a.col_ptr = ctx->col_ptr.p;
+    a.row_idx = ctx->row_idx_pull.p ? ctx->row_idx_pull.as<uint32_t>() : ctx->row_idx.as<uint32_t>();
+    a.n_rows = ctx->n_rows;
+    a.n_cols = ctx->n_cols;
+    a.q_in = ctx->queue[cur ^ 1].as<uint32_t>();
+    a.q_in_idx = cur ^ 1;
+    a.F = 0;  // not known yet
+    a.q_out = ctx->queue[cur].as<uint32_t>();
+    a.q_out_idx = cur;
+    a.vis = ctx->vis.as<uint32_t>();
+    a.levels = ctx->levels.as<int32_t>();
+    a.parents = par;
+    a.ctl = ctx->ctl.as<LevelCtl>();
+    a.next_level = (int32_t)level + 2;
+    a.track_degrees = track;
+    a.count_examined = count;
+    a.fbm_cur = ctx->fbm[fbm_cur ^ 1].as<uint32_t>();
+    a.fbm_next = ctx->fbm[fbm_cur].as<uint32_t>();
+    step_pull(ctx, a, true);
+    level_end(ctx, cur ^ 1, cur, false, BFSX_PULL, decide_allow, decide_reuse, true);
   }
 
   // Host bookkeeping after the control block of the level has been read.
@@ -509,6 +548,9 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   uint32_t decisions = ctx->h_ctl->decisions;
   const bool dev_loop = (flags & BFSX_RUN_DEVICE_LOOP) && ctx->coop_ok;
   const MarkRule mr = mark_rule(ctx);
+  // speculative bottom-up steps (see Walker::launch_speculative_pull)
+  const bool can_spec = ctx->params.speculate && learned && !timed && w.pull_possible;
+  bool spec = false;  // one is in the stream for level w.level
   while (w.F > 0) {
     // a search has at most n_cols levels; more means a broken invariant
     if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
@@ -547,7 +589,17 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
       std::memcpy(r.sched_out, ctx->h_ctl->sched_out, sizeof(r.sched_out));
     }
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
-    const bool new_fbm = w.launch(choice);
+    bool new_fbm;
+    if (spec && choice == BFSX_PULL) {
+      // this level's step is the speculative one already in the stream
+      new_fbm = true;
+      w.last = BFSX_PULL;
+    } else {
+      // (a speculative step that was not picked exits without a write)
+      new_fbm = w.launch(choice);
+    }
+    spec = can_spec && choice == BFSX_PULL;
+    if (spec) w.launch_speculative_pull();
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
     read_level_end(ctx, ++seq, cur ^ 1);
     // per-level records read the whole block (queue sizes, running totals,
@@ -811,9 +863,9 @@ int bfsx_create(int device, bfsx_ctx **out) {
     BFSX_HIP(hipMemsetAsync(ctx->ctl.p, 0, sizeof(LevelCtl), ctx->stream));  // level-end counter 0
     BFSX_HIP(hipStreamSynchronize(ctx->stream));
     BFSX_HIP(hipHostMalloc((void **)&ctx->h_ctl, sizeof(LevelCtl), hipHostMallocDefault));
-    BFSX_HIP(hipHostMalloc((void **)&ctx->h_mirror, sizeof(LevelMirror),
+    BFSX_HIP(hipHostMalloc((void **)&ctx->h_mirror, sizeof(LevelMirror) * MIRROR_SLOTS,
                            hipHostMallocMapped | hipHostMallocCoherent));
-    std::memset(ctx->h_mirror, 0, sizeof(LevelMirror));
+    std::memset(ctx->h_mirror, 0, sizeof(LevelMirror) * MIRROR_SLOTS);
     BFSX_HIP(hipHostGetDevicePointer((void **)&ctx->d_mirror, ctx->h_mirror, 0));
     ctx->params = default_params();
   });

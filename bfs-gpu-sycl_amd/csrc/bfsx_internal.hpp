@@ -71,9 +71,13 @@ constexpr size_t CTL_RESET_BYTES = offsetof(LevelCtl, seq);
 //   w[2] = (seq & 0xFFFF) << 48 | edges of the visited vertices (48 bits)
 //   w[3] = seq << 32 | next step kind (8 bits) | done << 8 | level << 9
 //          (the level of the next frontier, 23 bits)
+// Two such sets alternate by the parity of seq: a level end launched ahead
+// of the host's read of the previous one (a speculative step) writes the
+// other set, so it cannot overwrite words the host has not read yet.
 struct LevelMirror {
   unsigned long long w[4];
 };
+constexpr int MIRROR_SLOTS = 2;
 constexpr unsigned long long MIRROR_48 = (1ull << 48) - 1;
 constexpr uint32_t MIRROR_LEVEL_MASK = (1u << 23) - 1;
 
@@ -171,7 +175,7 @@ struct bfsx_ctx {
   bfsx::DevBuf bkt, bkt_cnt, bkt_tot, bkt_base;  // GLB bucketed discovery
   bfsx::DevBuf scratch;  // generic scratch (reductions, root picking)
   bfsx::LevelCtl *h_ctl = nullptr;        // pinned: full copies of the control block
-  bfsx::LevelMirror *h_mirror = nullptr;  // pinned, host-mapped: tagged level-end words
+  bfsx::LevelMirror *h_mirror = nullptr;  // pinned, host-mapped: tagged level-end words (2 sets)
   bfsx::LevelMirror *d_mirror = nullptr;  // device address of h_mirror (k_level_end writes it)
   bfsx::LevelView view;                   // the values of the last level end the host read
   bool mark_dirty = false;          // a mark step did not reach its compaction
@@ -288,7 +292,9 @@ void step_llb(bfsx_ctx *ctx, const StepArgs &a, bool strided);
 // parents): the next queue, levels, visited bits and (if a.fbm_next) the
 // next frontier bitmap are then final and no compaction follows.
 bool step_glb(bfsx_ctx *ctx, const StepArgs &a);
-void step_pull(bfsx_ctx *ctx, const StepArgs &a);
+// speculative: launched before the previous level's end has decided this
+// level's kind (see SpecGuard); it runs only if that decision is PULL
+void step_pull(bfsx_ctx *ctx, const StepArgs &a, bool speculative = false);
 void bitmap_to_queue(bfsx_ctx *ctx, const uint32_t *bm, uint32_t *q);
 void queue_to_fbm(bfsx_ctx *ctx, const uint32_t *q, uint32_t F, uint32_t *fbm);
 void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next);
@@ -297,7 +303,8 @@ void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next
 // and with reuse it keeps the last decision while the frontier stays close
 // to the one that decision was computed for.
 void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
-               bool stamp, int kind, uint32_t decide_allow = 0, bool reuse = false);
+               bool stamp, int kind, uint32_t decide_allow = 0, bool reuse = false,
+               bool speculative = false);
 // mark: which top-down levels mark and compact (needs track: the edge
 // counts are kept in the control block)
 void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark);

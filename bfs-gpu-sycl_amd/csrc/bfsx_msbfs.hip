@@ -426,6 +426,7 @@ __global__ void k_ms_end(LevelCtl *ctl, uint32_t qi, LevelMirror *mirror) {
   ctl->seq = seq;
   const unsigned long long t32 = (unsigned long long)seq << 32;
   const unsigned long long t16 = (unsigned long long)(seq & 0xFFFFu) << 48;
+  mirror += seq & 1u;  // the set for this parity
   mirror->w[0] = t32 | ctl->q_count[qo];
   mirror->w[1] = t16 | (ctl->q_edges[qo] & MIRROR_48);
   mirror->w[2] = t16 | (ctl->overflow ? 1ull : 0ull);

@@ -184,6 +184,10 @@ typedef struct bfsx_params {
                                fits resident; default 256; grid-wide
                                barriers cost more the more groups take
                                part) */
+  uint32_t speculate;       /* learned scheduler: after a bottom-up level,
+                               launch the next level's bottom-up step before
+                               the decision is read back; it runs only if
+                               the decision is PULL (1, default) or not (0) */
 } bfsx_params;
 
 /* ---- context ------------------------------------------------------------ */
