@@ -415,6 +415,7 @@ struct DevStep {
   uint32_t cur;                  // queue buffer holding the frontier
   int fixed;                     // BFSX_LLB or BFSX_SLB, or -1: read ctl->choice
   MarkRule mark;                 // which frontiers mark + compact
+  int spec;                      // speculative: skip the frontiers `mark` marks
 };
 
 template <typename PtrT, int WG>
@@ -427,6 +428,7 @@ __global__ __launch_bounds__(WG) void k_td_dev(KArgs a, LevelCtl *ctl, DevStep d
   a.F = F;
   a.next_level = (int32_t)ctl->level + 1;
   if (!d.mark.marks(ctl->q_edges[d.cur], F)) a.mark = nullptr;
+  else if (d.spec) return;
   const uint32_t n_tiles = (F + WG - 1) / WG;
   unsigned long long dacc = 0;
   for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
@@ -1076,16 +1078,22 @@ __device__ __forceinline__ bool in_frontier_fd(const uint32_t *fbm, const uint32
 }
 
 // A step launched ahead of the decision for its level (speculative) runs
-// only if that decision, made by the previous level's end, is a bottom-up
-// step and the frontier is not empty; otherwise its kernels, including its
-// level end, exit without a write.
+// only if that decision, made by the previous level's end, is one of the
+// step kinds it was launched for and the frontier is not empty; otherwise
+// its kernels, including its level end, exit without a write.
 struct SpecGuard {
   const LevelCtl *ctl;  // null: not speculative
   uint32_t consumed;    // the queue counter of the level's frontier
+  uint32_t kinds;       // bit k: runs if the decision is kind k
+  MarkRule heavy;       // and if this rule does not mark the frontier
   __device__ __forceinline__ bool skip() const {
-    return ctl && (ctl->choice != BFSX_PULL || ctl->q_count[consumed] == 0);
+    if (!ctl) return false;
+    const uint32_t F = ctl->q_count[consumed];
+    return !((kinds >> (ctl->choice & 31u)) & 1u) || F == 0 ||
+           heavy.marks(ctl->q_edges[consumed], F);
   }
 };
+constexpr MarkRule NEVER_MARKS{~0ull, ~0ull};  // (frontier edges stay below 2^48)
 
 // hbm bit h = frontier bit of hub[h] (H a multiple of 64, and so is the
 // grid-stride: every wave runs whole rounds).
@@ -2122,7 +2130,7 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
   // (a speculative step does not know its frontier's size: no summary)
   const bool use_sum = !speculative && max_words >= 64 &&
                        ((uint64_t)s.F << shift) * 4 <= (uint64_t)s.n_cols;
-  const SpecGuard guard{speculative ? s.ctl : nullptr, s.q_in_idx};
+  const SpecGuard guard{speculative ? s.ctl : nullptr, s.q_in_idx, 1u << BFSX_PULL, NEVER_MARKS};
   const uint32_t *g_sum = nullptr;
   if (use_sum) {
     ensure(ctx->fsum, sizeof(uint32_t) * PULL_SUM_WORDS_MAX, st);
@@ -2266,7 +2274,8 @@ void level_begin(bfsx_ctx *ctx) {
 }
 
 void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, int kind,
-               uint32_t decide_allow, bool reuse, bool speculative) {
+               uint32_t decide_allow, bool reuse, uint32_t spec_kinds,
+               const MarkRule *spec_heavy) {
   const uint64_t n_iso = (ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols
                              ? ctx->n_isolated
                              : 0;
@@ -2275,14 +2284,16 @@ void level_end(bfsx_ctx *ctx, uint32_t consumed, uint32_t produced, bool stamp, 
                      decide_allow ? ctx->sched_w.as<float>() : (const float *)nullptr,
                      (uint64_t)ctx->n_cols, (uint64_t)ctx->nnz, n_iso, decide_allow,
                      reuse ? 1 : 0, ctx->d_mirror,
-                     SpecGuard{speculative ? ctx->ctl.as<LevelCtl>() : nullptr, consumed});
+                     SpecGuard{spec_kinds ? ctx->ctl.as<LevelCtl>() : nullptr, consumed,
+                               spec_kinds, spec_heavy ? *spec_heavy : NEVER_MARKS});
   BFSX_HIP(hipGetLastError());
 }
 
 // One device-driven top-down level (LLB or SLB, or the choice in the control
 // block when fixed < 0) for a captured loop: step, compaction, bookkeeping.
 template <typename PtrT>
-static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark) {
+static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark,
+                        bool speculative) {
   KArgs a{};
   a.col_ptr = ctx->col_ptr.p;
   a.row_idx = ctx->row_idx.as<uint32_t>();
@@ -2298,7 +2309,7 @@ static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, Mark
   a.parents = nullptr;
   a.hub_degree = ctx->params.hub_degree;
   a.mark = ctx->mark.as<uint8_t>();
-  const DevStep d{cur, fixed, mark};
+  const DevStep d{cur, fixed, mark, speculative ? 1 : 0};
   const uint32_t wg = ctx->params.llb_wg;
   const unsigned grid = cap_grid(((uint64_t)ctx->n_cols + wg - 1) / wg, (unsigned)ctx->cu_count * 8);
   hipStream_t s = ctx->stream;
@@ -2310,6 +2321,10 @@ static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, Mark
     case 1024: hipLaunchKernelGGL((k_td_dev<PtrT, 1024>), dim3(grid), dim3(1024), 0, s, a, ctl, d); break;
     default: fail(BFSX_ERR_ARG, "llb_wg must be 64, 128, 256, 512 or 1024");
   }
+  if (speculative) {  // (it never marks)
+    BFSX_HIP(hipGetLastError());
+    return;
+  }
   const CompactShape c = compact_shape(ctx, ctx->n_cols);
   hipLaunchKernelGGL(k_compact_dev<PtrT>, dim3(c.grid), dim3(COMPACT_WG), 0, s,
                      static_cast<const PtrT *>(ctx->col_ptr.p), ctx->deg16.as<uint16_t>(),
@@ -2320,9 +2335,10 @@ static void dev_level_t(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, Mark
   BFSX_HIP(hipGetLastError());
 }
 
-void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark) {
-  if (ctx->ptr_bytes == 8) dev_level_t<uint64_t>(ctx, cur, fixed, track, mark);
-  else dev_level_t<uint32_t>(ctx, cur, fixed, track, mark);
+void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark,
+               bool speculative) {
+  if (ctx->ptr_bytes == 8) dev_level_t<uint64_t>(ctx, cur, fixed, track, mark, speculative);
+  else dev_level_t<uint32_t>(ctx, cur, fixed, track, mark, speculative);
 }
 
 template <typename PtrT, int WG>

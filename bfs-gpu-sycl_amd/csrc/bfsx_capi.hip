@@ -351,7 +351,20 @@ struct Walker {
     a.fbm_cur = ctx->fbm[fbm_cur ^ 1].as<uint32_t>();
     a.fbm_next = ctx->fbm[fbm_cur].as<uint32_t>();
     step_pull(ctx, a, true);
-    level_end(ctx, cur ^ 1, cur, false, BFSX_PULL, decide_allow, decide_reuse, true);
+    level_end(ctx, cur ^ 1, cur, false, BFSX_PULL, decide_allow, decide_reuse, 1u << BFSX_PULL);
+  }
+
+  // The same after a top-down level, whose successor is often a thin
+  // top-down level too (the tail of a search): the next level's LLB or SLB
+  // step, sized on the device (the captured loop's kernel: frontier, level
+  // and kind come from the control block), runs only if the decision is LLB
+  // or SLB and the frontier is not empty and not one `mr` marks (a heavy
+  // level runs the host-sized step).  That kernel records no parents: not
+  // with a parent array.
+  void launch_speculative_td(const MarkRule &mr) {
+    dev_level(ctx, cur ^ 1, -1, true, mr, true);
+    level_end(ctx, cur ^ 1, cur, false, -1, decide_allow, decide_reuse,
+              (1u << BFSX_LLB) | (1u << BFSX_SLB), &mr);
   }
 
   // Host bookkeeping after the control block of the level has been read.
@@ -548,9 +561,9 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   uint32_t decisions = ctx->h_ctl->decisions;
   const bool dev_loop = (flags & BFSX_RUN_DEVICE_LOOP) && ctx->coop_ok;
   const MarkRule mr = mark_rule(ctx);
-  // speculative bottom-up steps (see Walker::launch_speculative_pull)
-  const bool can_spec = ctx->params.speculate && learned && !timed && w.pull_possible;
-  bool spec = false;  // one is in the stream for level w.level
+  // speculative steps (see Walker::launch_speculative_pull / _td)
+  const bool can_spec = ctx->params.speculate && learned && !timed;
+  int spec = -1;  // the kind speculated for level w.level: BFSX_PULL, BFSX_LLB (LLB or SLB) or -1
   while (w.F > 0) {
     // a search has at most n_cols levels; more means a broken invariant
     if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
@@ -590,16 +603,26 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
     }
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step0, s));
     bool new_fbm;
-    if (spec && choice == BFSX_PULL) {
+    if (spec == BFSX_PULL && choice == BFSX_PULL) {
       // this level's step is the speculative one already in the stream
       new_fbm = true;
       w.last = BFSX_PULL;
+    } else if (spec == BFSX_LLB && (choice == BFSX_LLB || choice == BFSX_SLB) &&
+               !mr.marks(w.EF, w.F)) {
+      new_fbm = false;  // (the speculative top-down step writes the queue only)
+      w.last = choice;
     } else {
       // (a speculative step that was not picked exits without a write)
       new_fbm = w.launch(choice);
     }
-    spec = can_spec && choice == BFSX_PULL;
-    if (spec) w.launch_speculative_pull();
+    spec = -1;
+    if (can_spec && choice == BFSX_PULL && w.pull_possible) {
+      w.launch_speculative_pull();
+      spec = BFSX_PULL;
+    } else if (can_spec && !dev_loop && choice != BFSX_PULL && !par) {
+      w.launch_speculative_td(mr);
+      spec = BFSX_LLB;
+    }
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
     read_level_end(ctx, ++seq, cur ^ 1);
     // per-level records read the whole block (queue sizes, running totals,

@@ -302,12 +302,18 @@ void compact(bfsx_ctx *ctx, const StepArgs &s, uint8_t *mark, uint32_t *fbm_next
 // the next level's step kind (among the allowed ones) in the same launch,
 // and with reuse it keeps the last decision while the frontier stays close
 // to the one that decision was computed for.
+// spec_kinds != 0: launched ahead of the previous level's decision; runs
+// only if that decision is one of these kinds (bit k for kind k) and the
+// frontier is not empty and (spec_heavy) not one that rule marks.
 void level_end(bfsx_ctx *ctx, uint32_t consumed_idx, uint32_t produced_idx,
                bool stamp, int kind, uint32_t decide_allow = 0, bool reuse = false,
-               bool speculative = false);
+               uint32_t spec_kinds = 0, const MarkRule *spec_heavy = nullptr);
 // mark: which top-down levels mark and compact (needs track: the edge
-// counts are kept in the control block)
-void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark);
+// counts are kept in the control block).  speculative: launched ahead of
+// the previous level's decision; the step then skips frontiers `mark` marks
+// (no compaction follows) besides kinds other than LLB and SLB.
+void dev_level(bfsx_ctx *ctx, uint32_t cur, int fixed, bool track, MarkRule mark,
+               bool speculative = false);
 void level_begin(bfsx_ctx *ctx);
 // Runs consecutive thin top-down levels (levels `mark` does not mark,
 // kind LLB or SLB: `fixed`, or the scheduler's choice when fixed < 0) in one

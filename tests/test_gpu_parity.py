@@ -648,6 +648,61 @@ def test_device_loop_parity(engine, ratio):
         engine.set_params(**old)
 
 
+@pytest.mark.parametrize("ratio", [4, 0, 0xFFFFFFFF])
+def test_speculative_steps(engine, ratio):
+    """The learned host loop launches the next level's bottom-up or thin
+    top-down step before it has read the current level's end; a step whose
+    guard fails (another kind picked, an empty or a marking frontier) exits
+    without a write.  Levels, edge counts and per-kind level counts with
+    speculation match the oracle and the same searches without it, under
+    random scheduler weights (frequent guesses that fail) and the trained
+    ones, on an R-MAT graph, a grid, a path and a directed graph; searches
+    with parents or per-level traces between them stay exact."""
+    old = engine.params
+    try:
+        engine.set_params(mark_ratio_q10=ratio)
+        cases = []
+        engine.generate_rmat(15, 16, 0.57, 0.19, 0.19, 251, 252, 4)
+        engine.order_adjacency("pull_hub_first")
+        cases.append((engine.download_csc(), engine.pick_roots(3, 253), True, None))
+        engine.generate_grid(300, 0.05, 5, 254, 8)
+        cases.append((engine.download_csc(), engine.pick_roots(2, 255, 1), True, None))
+        cases.append((path_graph(2000), np.array([7, 1000], np.uint32), True, None))
+        rng = np.random.default_rng(13)
+        dcp, dri = random_graph(rng, 5000, 30000, directed=True)
+        cases.append(((dcp, dri), rng.integers(0, 5000, 2), False, 4990))
+        for ci, ((cp, ri), roots, sym, n_rows) in enumerate(cases):
+            say("speculation", ratio, "case", ci)
+            engine.upload_csc(cp, ri, n_rows=n_rows, symmetric=sym)
+            if sym:
+                engine.order_adjacency("pull_hub_first")
+            for r in roots:
+                want, ec, _ = cpu.bfs_queue(cp, ri, int(r), n_rows=n_rows)
+                for seed in (None, 260, 261, 262):
+                    if seed is None:
+                        engine.load_scheduler()
+                    else:
+                        engine.set_scheduler_weights(mlp_ref.init_weights(seed) * 3)
+                    steps = []
+                    for spec in (1, 0, 1):
+                        engine.set_params(speculate=spec)
+                        lev, st = engine.run(int(r), "learned")
+                        assert np.array_equal(lev, want), (ratio, ci, seed, spec, int(r))
+                        assert st["edges_count"] == ec
+                        assert st["n_levels"] == int(want.max()) + 1 == sum(st["steps"].values())
+                        steps.append(st["steps"])
+                    assert steps[0] == steps[1] == steps[2], (ratio, ci, seed, int(r))
+                    engine.set_params(speculate=1)
+                    lev, st, par = engine.run(int(r), "learned", parents=True)
+                    assert np.array_equal(lev, want) and st["steps"] == steps[0]
+                    assert cpu.check_parents(cp, ri, int(r), want, par, n_rows=n_rows) == 0
+                    lev, st = engine.run(int(r), "learned", trace=True)[:2]
+                    assert np.array_equal(lev, want) and st["steps"] == steps[0]
+    finally:
+        engine.set_scheduler_weights(None)
+        engine.set_params(**old)
+
+
 def test_device_loop_block_counts_and_tiny_graphs(engine):
     old = engine.params
     try:
