@@ -1005,8 +1005,6 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr,
 constexpr uint32_t PULL_LANE_PREFIX = 256;
 // Candidates per lane per round of the bottom-up step.
 constexpr int PULL_SLOTS = 2;
-// A wave takes 2048 vertices (64 visited-bitmap words) at a time.
-constexpr uint32_t PULL_CHUNK = 2048;
 // Frontier summary: one bit per 2^shift vertices, at most
 // PULL_SUM_WORDS_MAX words, held in dynamic LDS by every work-group of the
 // bottom-up step.
@@ -1157,7 +1155,8 @@ template <typename PtrT, int WG, bool SUM>
 __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
                                              uint32_t *fbm_next, const uint32_t *g_sum,
                                              uint32_t sum_words, uint32_t shift,
-                                             const uint2 *fd, const uint32_t *hbm,
+                                             const uint2 *fd, const uint32_t *iso,
+                                             const uint32_t *crank, const uint32_t *hbm,
                                              const uint32_t *hub, SpecGuard guard) {
   if (guard.skip()) return;
   constexpr int NW = WG / 64;
@@ -1188,6 +1187,18 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
     const uint32_t incl = wave_incl_scan(cnt);
     const uint32_t total = __shfl(incl, 63, 64);
     const uint32_t excl = incl - cnt;
+    // fd holds the vertices with neighbours only: this lane's word of them
+    // and the count before it (the candidates are among them: isolated
+    // vertices start visited)
+    uint32_t lw = 0, lexcl = 0, lbase = 0;
+    if (fd && total) {
+      lw = wi < n_words ? ~iso[wi] : 0u;
+      if (vbase >= a.n_rows) lw = 0;
+      else if (a.n_rows - vbase < 32) lw &= (1u << (a.n_rows - vbase)) - 1u;
+      const uint32_t lc = __popc(lw);
+      lexcl = wave_incl_scan(lc) - lc;
+      lbase = crank[c];
+    }
     fw[lane] = 0;
     wave_sync_lds();
     for (uint32_t r = 0; r < total; r += 64 * PULL_SLOTS) {
@@ -1195,7 +1206,7 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
       // probes (phase 1) are in flight together; the rest of each one's scan
       // (phase 2) follows slot by slot, uniformly over the wave.
       bool cand_[PULL_SLOTS], found_[PULL_SLOTS];
-      uint32_t off_[PULL_SLOTS], u_[PULL_SLOTS], par_[PULL_SLOTS];
+      uint32_t off_[PULL_SLOTS], u_[PULL_SLOTS], par_[PULL_SLOTS], fi_[PULL_SLOTS];
       PtrT b_[PULL_SLOTS], e_[PULL_SLOTS], deg_[PULL_SLOTS];
 #pragma unroll
       for (int sl = 0; sl < PULL_SLOTS; sl++) {
@@ -1211,23 +1222,31 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         }
         const uint32_t om = __shfl(cm, (int)o, 64);
         const uint32_t oe = __shfl(excl, (int)o, 64);
-        off_[sl] = cand_[sl] ? o * 32 + kth_set_bit(om, idx - oe) : 0u;
+        const uint32_t bit = kth_set_bit(om, idx - oe);
+        off_[sl] = cand_[sl] ? o * 32 + bit : 0u;
         u_[sl] = c * PULL_CHUNK + off_[sl];
+        if (fd) {
+          // rank among the vertices with neighbours (not a candidate: the
+          // chunk's first such vertex, or the padding entry)
+          const uint32_t ol = __shfl(lw, (int)o, 64);
+          const uint32_t ole = __shfl(lexcl, (int)o, 64);
+          fi_[sl] = lbase + (cand_[sl] ? ole + __popc(ol & ((1u << bit) - 1u)) : 0u);
+        }
         found_[sl] = false;
         par_[sl] = 0;
         b_[sl] = e_[sl] = deg_[sl] = 0;
       }
       if (fd) {
-        // With the hub-first copy, fd[u] = {first neighbour, degree} (a
-        // dense array read in vertex order) holds u's highest-degree
-        // neighbour: on dense frontiers that probe decides most candidates,
-        // which then cost neither a col_ptr nor a row_idx line.  A first
-        // neighbour among the top-degree vertices is named by its hub index
-        // and probed in the hub bitmap.  (u is a vertex of this chunk below
-        // n_rows for every lane, so the loads need no branch.)
+        // With the hub-first copy, fd = {first neighbour, degree} of every
+        // vertex with neighbours (a dense array read in vertex order) holds
+        // u's highest-degree neighbour: on dense frontiers that probe
+        // decides most candidates, which then cost neither a col_ptr nor a
+        // row_idx line.  A first neighbour among the top-degree vertices is
+        // named by its hub index and probed in the hub bitmap.  (fi_ is an
+        // entry of the array for every lane, so the loads need no branch.)
         uint2 x[PULL_SLOTS];
 #pragma unroll
-        for (int sl = 0; sl < PULL_SLOTS; sl++) x[sl] = fd[u_[sl]];
+        for (int sl = 0; sl < PULL_SLOTS; sl++) x[sl] = fd[fi_[sl]];
         uint32_t f[PULL_SLOTS];
         bool hf[PULL_SLOTS];
 #pragma unroll
@@ -2152,6 +2171,12 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
   const uint2 *fd = ctx->pull_fd.p && s.row_idx == ctx->row_idx_pull.as<uint32_t>()
                         ? ctx->pull_fd.as<uint2>()
                         : nullptr;
+  // (its entries exist for the vertices with neighbours: the others must
+  // start visited, which a search does on a symmetric graph)
+  if (fd && !((ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols))
+    fail(BFSX_ERR_UNSUPPORTED, "bottom-up step on a non-symmetric graph");
+  const uint32_t *iso = fd ? ctx->iso_bm.as<uint32_t>() : nullptr;
+  const uint32_t *crank = fd ? ctx->pull_crank.as<uint32_t>() : nullptr;
   // hub indexing belongs to fd: the hubs' frontier bits, gathered first
   const uint32_t *hbm = nullptr, *hub = nullptr;
   if (fd && ctx->pull_hubs) {
@@ -2164,20 +2189,20 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
   KernelTimer kt(s, st);
   switch (wg) {
     case 64:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 64, true>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
-      else hipLaunchKernelGGL((k_pull<PtrT, 64, false>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 64, true>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+      else hipLaunchKernelGGL((k_pull<PtrT, 64, false>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
       break;
     case 128:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 128, true>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
-      else hipLaunchKernelGGL((k_pull<PtrT, 128, false>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 128, true>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+      else hipLaunchKernelGGL((k_pull<PtrT, 128, false>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
       break;
     case 256:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 256, true>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
-      else hipLaunchKernelGGL((k_pull<PtrT, 256, false>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 256, true>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+      else hipLaunchKernelGGL((k_pull<PtrT, 256, false>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
       break;
     case 512:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 512, true>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
-      else hipLaunchKernelGGL((k_pull<PtrT, 512, false>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, hbm, hub, guard);
+      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 512, true>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+      else hipLaunchKernelGGL((k_pull<PtrT, 512, false>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
       break;
     default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
   }

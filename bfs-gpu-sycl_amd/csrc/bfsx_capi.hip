@@ -83,6 +83,7 @@ static void ensure_pull_state(bfsx_ctx *ctx) {
 // copy with hot neighbours named by index
 static void drop_pull_fd(bfsx_ctx *ctx) {
   release(ctx->pull_fd);
+  release(ctx->pull_crank);
   release(ctx->pull_hub);
   release(ctx->pull_hbm);
   ctx->pull_hubs = 0;
@@ -1071,6 +1072,7 @@ int bfsx_order_adjacency(bfsx_ctx *ctx, int order) {
         ensure(ctx->pull_fd, sizeof(uint2) * std::max<uint64_t>(ctx->n_rows, 1), ctx->stream);
         first_neighbours(ctx, ctx->row_idx_pull.as<uint32_t>(), ctx->pull_fd.as<uint2>());
         index_hub_neighbours(ctx);
+        compact_first_neighbours(ctx);
       }
       build_hot_copy(ctx);
       ctx->gflags |= BFSX_GRAPH_PULL_COPY;

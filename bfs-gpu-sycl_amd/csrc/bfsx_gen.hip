@@ -18,6 +18,7 @@
 //   tag 3  grid edge g keep/drop:          ctr = (lo g, hi g, 0, 3)
 //   tag 4  grid long-range edge k:         ctr = (k, 0, 0, 4)
 #include <cstring>
+#include <utility>
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -555,6 +556,102 @@ void index_hub_neighbours(bfsx_ctx *ctx) {
   BFSX_HIP(hipStreamSynchronize(s));
   release(hrank);
   ctx->pull_hubs = (uint32_t)H;
+}
+
+// Vertices with neighbours (below n_rows) in word w of the isolated bitmap.
+__device__ __forceinline__ uint32_t linked_word(const uint32_t *iso, uint32_t w, uint32_t n_rows) {
+  const uint64_t v0 = (uint64_t)w * 32;
+  if (v0 >= n_rows) return 0u;
+  const uint32_t m = n_rows - v0 < 32 ? (1u << (n_rows - v0)) - 1u : ~0u;
+  return ~iso[w] & m;
+}
+
+// cnt[c] = vertices with neighbours in chunk c (a thread per chunk)
+__global__ void k_chunk_linked(const uint32_t *iso, uint32_t n_rows, uint32_t n_chunks,
+                               uint32_t *cnt) {
+  constexpr uint32_t WPC = PULL_CHUNK / 32;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n_chunks;
+       c += gridDim.x * blockDim.x) {
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < WPC; i++) n += __popc(linked_word(iso, c * WPC + i, n_rows));
+    cnt[c] = n;
+  }
+}
+
+// In-place exclusive scan of cnt[0..n) (one work-group of 1024 threads,
+// each owning a contiguous run); the total lands in cnt[n].
+__global__ __launch_bounds__(1024) void k_scan_runs(uint32_t *cnt, uint32_t n) {
+  __shared__ uint64_t s_tot[1024];
+  const uint32_t t = threadIdx.x;
+  const uint32_t run = (n + 1023) / 1024;
+  const uint32_t i0 = min(t * run, n), i1 = min(i0 + run, n);
+  uint64_t sum = 0;
+  for (uint32_t i = i0; i < i1; i++) sum += cnt[i];
+  s_tot[t] = sum;
+  __syncthreads();
+  if (t == 0) {  // (one-off, 1024 entries)
+    uint64_t acc = 0;
+    for (uint32_t k = 0; k < 1024; k++) {
+      const uint64_t x = s_tot[k];
+      s_tot[k] = acc;
+      acc += x;
+    }
+    cnt[n] = (uint32_t)acc;
+  }
+  __syncthreads();
+  uint64_t pos = s_tot[t];
+  for (uint32_t i = i0; i < i1; i++) {
+    const uint32_t x = cnt[i];
+    cnt[i] = (uint32_t)pos;
+    pos += x;
+  }
+}
+
+// out[crank[c] + k] = fd[v] for the k-th vertex v with neighbours of chunk c
+__global__ void k_fd_compact(const uint2 *fd, const uint32_t *iso, uint32_t n_rows,
+                             uint32_t n_chunks, const uint32_t *crank, uint2 *out) {
+  constexpr uint32_t WPC = PULL_CHUNK / 32;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n_chunks;
+       c += gridDim.x * blockDim.x) {
+    uint32_t pos = crank[c];
+    for (uint32_t i = 0; i < WPC; i++) {
+      uint32_t m = linked_word(iso, c * WPC + i, n_rows);
+      while (m) {
+        const uint32_t b = __ffs(m) - 1;
+        m &= m - 1;
+        out[pos++] = fd[(c * WPC + i) * 32 + b];
+      }
+    }
+  }
+}
+
+void compact_first_neighbours(bfsx_ctx *ctx) {
+  release(ctx->pull_crank);
+  if (!ctx->pull_fd.p) return;
+  hipStream_t s = ctx->stream;
+  const uint32_t n = ctx->n_rows;
+  const uint32_t n_chunks = (uint32_t)(((uint64_t)n + PULL_CHUNK - 1) / PULL_CHUNK);
+  ensure(ctx->pull_crank, sizeof(uint32_t) * ((uint64_t)n_chunks + 1), s);
+  uint32_t *crank = ctx->pull_crank.as<uint32_t>();
+  const uint32_t *iso = ctx->iso_bm.as<uint32_t>();
+  hipLaunchKernelGGL(k_chunk_linked, dim3(grid_for(n_chunks)), dim3(256), 0, s, iso, n,
+                     n_chunks, crank);
+  hipLaunchKernelGGL(k_scan_runs, dim3(1), dim3(1024), 0, s, crank, n_chunks);
+  BFSX_HIP(hipGetLastError());
+  uint32_t linked = 0;
+  BFSX_HIP(hipMemcpyAsync(&linked, crank + n_chunks, sizeof(linked), hipMemcpyDeviceToHost, s));
+  BFSX_HIP(hipStreamSynchronize(s));
+  if (linked != n - ctx->n_isolated) fail(BFSX_ERR_DEVICE, "isolated-vertex count mismatch");
+  DevBuf out;
+  ensure(out, sizeof(uint2) * ((uint64_t)linked + 1), s);
+  // the padding entry: a vertex with no neighbours (read, never used)
+  BFSX_HIP(hipMemsetAsync(out.as<uint2>() + linked, 0, sizeof(uint2), s));
+  hipLaunchKernelGGL(k_fd_compact, dim3(grid_for(n_chunks)), dim3(256), 0, s,
+                     ctx->pull_fd.as<uint2>(), iso, n, n_chunks, crank, out.as<uint2>());
+  BFSX_HIP(hipGetLastError());
+  BFSX_HIP(hipStreamSynchronize(s));
+  std::swap(ctx->pull_fd, out);
+  release(out);
 }
 
 // out[k] = PULL_HUB_FLAG | hrank[x] for hot x = row_idx[k], else x

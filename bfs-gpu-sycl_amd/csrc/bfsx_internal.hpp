@@ -110,6 +110,9 @@ struct LevelView {
 // chunk; mark arrays hold one byte per vertex, padded the same way.
 __host__ __device__ inline uint64_t bitmap_words(uint64_t n) { return ((n + 1023) / 1024) * 32; }
 
+// A wave of the bottom-up step takes 2048 vertices (64 visited-bitmap words)
+// at a time.
+constexpr uint32_t PULL_CHUNK = 2048;
 // Largest frontier summary of the bottom-up step (words; 32 KiB).
 constexpr uint32_t PULL_SUM_WORDS_MAX = 8192;
 constexpr uint32_t DEG16_ESCAPE = 0xFFFFu;
@@ -153,7 +156,13 @@ struct bfsx_ctx {
   // vertex instead of two col_ptr entries (col_ptr only for the escapes)
   bfsx::DevBuf deg16;
   bfsx::DevBuf row_idx_pull;  // hub-first copy read by bottom-up steps (optional)
-  bfsx::DevBuf pull_fd;       // with it: {first (highest-degree) neighbour, degree} per vertex
+  // With it: {first (highest-degree) neighbour, degree} for every vertex
+  // with neighbours, in vertex order and nothing for the isolated ones (a
+  // bottom-up step reaches only vertices with neighbours: the isolated ones
+  // start visited).  The entry of v sits at pull_crank[v / PULL_CHUNK] plus
+  // the vertices with neighbours before v in its PULL_CHUNK-vertex chunk
+  // (from the isolated-vertex bitmap); one padding entry follows.
+  bfsx::DevBuf pull_fd, pull_crank;
   // The pull_hubs vertices of largest degree ("hubs") are named in pull_fd by
   // their index: fd[v].x = PULL_HUB_FLAG | h for the hub pull_hub[h].  Before
   // every bottom-up step their frontier bits are gathered into pull_hbm
@@ -241,8 +250,11 @@ void order_adjacency(bfsx_ctx *ctx, bool hub_first, uint32_t *out);
 void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint2 *fd);
 // Names the first neighbours that are among the graph's top-degree vertices
 // by their index in ctx->pull_hub (sets pull_hub, pull_hbm, pull_hubs); needs
-// ctx->pull_fd and n_cols < 2^31.
+// ctx->pull_fd (one entry per vertex) and n_cols < 2^31.
 void index_hub_neighbours(bfsx_ctx *ctx);
+// Drops the isolated vertices' entries from ctx->pull_fd (one entry per
+// vertex before) and builds ctx->pull_crank (needs the isolated bitmap).
+void compact_first_neighbours(bfsx_ctx *ctx);
 // Builds row_idx_hot / hot_id / hot_vis / hot_n for graphs of
 // HOT_MIN_VERTICES to 2^31 vertices (otherwise leaves hot_n = 0).
 constexpr uint32_t HOT_MIN_VERTICES = 1u << 24;
