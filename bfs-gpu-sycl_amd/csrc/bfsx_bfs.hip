@@ -869,6 +869,17 @@ __global__ __launch_bounds__(1024) void k_bucket_base(const uint64_t *tot, uint3
   if (t == 0) base[n_slices] = total;
 }
 
+// Bit q = word 2q or 2q + 1 of m is set (64 words -> 32 pairs).
+__device__ __forceinline__ uint32_t pair_bits(unsigned long long m) {
+  unsigned long long x = (m | (m >> 1)) & 0x5555555555555555ull;
+  x = (x | (x >> 1)) & 0x3333333333333333ull;
+  x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+  return (uint32_t)x;
+}
+
 // A work-group per slice: new vertices of the slice from its targets.
 template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr,
@@ -924,17 +935,23 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr,
   constexpr uint32_t NW = WG / 64;
   constexpr uint32_t WPW = WPS / NW;  // words per wave
   __shared__ uint32_t s_wcnt[NW + 1];
-  __shared__ bool s_dense;
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr uint32_t NJ = WPW / 64;
+  unsigned long long wm[NJ];  // the wave's words holding new vertices
   uint32_t cnt = 0;
-  for (uint32_t i = wv * WPW + lane; i < (wv + 1) * WPW; i += 64) {
+#pragma unroll
+  for (uint32_t j = 0; j < NJ; j++) {
+    const uint32_t i = wv * WPW + 64 * j + lane;
     const uint32_t w = w0 + i;
-    if (w >= n_words) continue;
-    const uint32_t nw = s_found[i] & ~s_vis[i];
-    s_found[i] = nw;
-    if (nw) vis[w] = s_vis[i] | nw;
-    if (fbm_next) fbm_next[w] = nw;
-    cnt += __popc(nw);
+    uint32_t nw = 0;
+    if (w < n_words) {
+      nw = s_found[i] & ~s_vis[i];
+      s_found[i] = nw;
+      if (nw) vis[w] = s_vis[i] | nw;
+      if (fbm_next) fbm_next[w] = nw;
+      cnt += __popc(nw);
+    }
+    wm[j] = __ballot(nw != 0);
   }
   cnt = wave_sum(cnt);
   if (lane == 0) s_wcnt[wv] = cnt;
@@ -947,29 +964,43 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr,
       run += c;
     }
     s_qbase = run ? atomicAdd(q_count, run) : 0u;
-    s_dense = run * 8u >= WPS * 32u;
   }
   __syncthreads();
   // The next frontier's degree sum from the 16-bit degrees (2 bytes per
-  // vertex, col_ptr only for saturated ones): a slice with at least 1/8 of
-  // its vertices new loads them for every lane (no branch around the loads:
-  // the rounds' loads are in flight together); a sparse one only for its new
-  // vertices.  (Two col_ptr entries per vertex made the pass over a dense
-  // slice read the whole col_ptr array: 0.5 GB at scale 26.)
-  const bool dense = s_dense;
+  // vertex, col_ptr only for saturated ones; two col_ptr entries per vertex
+  // made the pass over a dense slice read the whole col_ptr array: 0.5 GB at
+  // scale 26).
   uint32_t pos = s_qbase + s_wcnt[wv];
   unsigned long long dsum = 0;
-  // eight rounds at a time, so their loads are in flight together
+  // Pair P of the wave (bit P of pm) = words 2P and 2P + 1 of its range, set
+  // when either holds a new vertex: only those pairs get a round (a round per
+  // pair of all 2048 words cost about 40 us of VALU time per launch on levels
+  // that find few vertices).  Rounds go eight at a time, their 16-bit degree
+  // loads (every lane of the round: one line) in flight together.
+  static_assert(NJ <= 4, "two 64-bit pair masks per wave");
+  unsigned long long pm[2] = {0ull, 0ull};
+#pragma unroll
+  for (uint32_t j = 0; j < NJ; j++) pm[j >> 1] |= (unsigned long long)pair_bits(wm[j]) << (32 * (j & 1));
   constexpr uint32_t R = 8;
-  for (uint32_t i0 = wv * WPW; i0 < (wv + 1) * WPW; i0 += 2 * R) {  // two words = 64 vertices
+  while (pm[0] | pm[1]) {
+    uint32_t i_[R];
+    bool ok[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; r++) {
+      const int h = pm[0] ? 0 : 1;
+      ok[r] = (pm[0] | pm[1]) != 0;
+      const uint32_t P = ok[r] ? (uint32_t)(64 * h + __ffsll(pm[h]) - 1) : 0u;
+      if (ok[r]) pm[h] &= pm[h] - 1;
+      i_[r] = wv * WPW + 2 * P;  // (both words below n_words)
+    }
     uint32_t d_[R];
     bool take[R];
 #pragma unroll
     for (uint32_t r = 0; r < R; r++) {
-      const uint32_t i = i0 + 2 * r;
       d_[r] = 0;
       take[r] = false;
-      if (w0 + i >= n_words) continue;
+      if (!ok[r]) continue;
+      const uint32_t i = i_[r];
       const uint32_t word = s_found[i + (lane >> 5)];
       const bool isnew = (word >> (lane & 31)) & 1u;
       const unsigned long long m = __ballot(isnew);
@@ -980,15 +1011,14 @@ __global__ __launch_bounds__(WG) void k_bucket_apply(const PtrT *col_ptr,
       }
       if (q_edges) {
         take[r] = isnew && v < n_rows;
-        if (dense) d_[r] = deg16[v];
-        else if (take[r]) d_[r] = deg16[v];
+        d_[r] = deg16[v];
       }
       pos += (uint32_t)__popcll(m);
     }
 #pragma unroll
     for (uint32_t r = 0; r < R; r++) {
       if (take[r]) {
-        const uint32_t v = (w0 + i0 + 2 * r) * 32 + lane;
+        const uint32_t v = (w0 + i_[r]) * 32 + lane;
         dsum += d_[r] != DEG16_ESCAPE ? (unsigned long long)d_[r]
                                       : (unsigned long long)(col_ptr[v + 1] - col_ptr[v]);
       }
