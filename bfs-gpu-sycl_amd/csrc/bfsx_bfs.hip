@@ -1169,6 +1169,111 @@ __device__ __forceinline__ uint32_t kth_set_bit(uint32_t m, uint32_t k) {
   return p;
 }
 
+// The rest of a candidate's list, from entry b to e (wave-collective: every
+// lane calls it; cand says whether the lane holds a candidate still to be
+// decided).  Every candidate first scans up to PULL_LANE_PREFIX neighbours in
+// its own lane, and the rest of a longer list goes to a quarter wave.
+template <typename PtrT, bool SUM>
+__device__ __forceinline__ void scan_rest(const KArgs &a, const uint32_t *fbm_cur,
+                                          const uint32_t *s_sum, uint32_t shift, bool cand,
+                                          bool &found, uint32_t &par, PtrT b, const PtrT e,
+                                          unsigned long long &scanned) {
+  const uint32_t lane = threadIdx.x & 63;
+  const bool longer = e - b > PULL_LANE_PREFIX;  // the rest of the list
+  const PtrT lim = longer ? b + PULL_LANE_PREFIX : e;
+  // Every candidate first scans up to PULL_LANE_PREFIX neighbours in its
+  // own lane, four independent probes per round (indices past the end of
+  // the prefix repeat its last neighbour, which is harmless): all 64
+  // candidates of the wave progress together, and with hub-first lists
+  // most stop in the first round.  Longer lists not decided by their
+  // prefix continue four at a time, each by a quarter wave (16 lanes x 4
+  // loads = 64 consecutive neighbours per round, all loads in flight).
+  if (cand && !found) {
+    PtrT k = b;
+    for (; k < lim; k += 4) {
+      const PtrT last = lim - 1;
+      const uint32_t w[4] = {a.row_idx[k], a.row_idx[k + 1 < lim ? k + 1 : last],
+                             a.row_idx[k + 2 < lim ? k + 2 : last],
+                             a.row_idx[k + 3 < lim ? k + 3 : last]};
+      bool h[4];
+      in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
+      if (h[0] | h[1] | h[2] | h[3]) {
+        found = true;
+        par = h[0] ? w[0] : h[1] ? w[1] : h[2] ? w[2] : w[3];
+        break;
+      }
+    }
+    if (a.examined) scanned += (unsigned long long)((k + 4 < lim ? k + 4 : lim) - b);
+    b = lim;  // where a long list's quarter-wave scan resumes
+  }
+  unsigned long long big = __ballot(cand && longer && !found);
+  const uint32_t q = lane >> 4, ql = lane & 15;
+  const unsigned long long qmask = 0xFFFFull << (16 * q);
+  while (big) {
+    int my_src = -1;  // the candidate lane my quarter serves
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+      if (big) {
+        if (i == q) my_src = __ffsll(big) - 1;
+        big &= big - 1;
+      }
+    }
+    const PtrT sb = __shfl(b, my_src < 0 ? 0 : my_src, 64);
+    const PtrT se = __shfl(e, my_src < 0 ? 0 : my_src, 64);
+    PtrT base = my_src < 0 ? se : sb;
+    bool done = base >= se, hit = false;
+    uint32_t hp = 0;
+    while (__ballot(!done)) {
+      uint32_t w[4];
+      bool h[4];
+#pragma unroll
+      for (uint32_t j = 0; j < 4; j++) {
+        const PtrT k = base + j * 16 + ql;
+        // (entry 0 exists: some candidate has a long list)
+        w[j] = a.row_idx[(!done && k < se) ? k : PtrT(0)];
+      }
+      in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
+#pragma unroll
+      for (uint32_t j = 0; j < 4; j++) {
+        const PtrT k = base + j * 16 + ql;
+        h[j] = h[j] && !done && k < se;
+      }
+      bool got = false;
+      uint32_t gw = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < 4; j++) {
+        const unsigned long long hb = __ballot(h[j]) & qmask;
+        const uint32_t pw = __shfl(w[j], hb ? __ffsll(hb) - 1 : (int)lane, 64);
+        if (!got && hb) {
+          got = true;
+          gw = pw;
+        }
+      }
+      if (!done) {
+        if (a.examined && ql == 0) scanned += (unsigned long long)(se - base < 64 ? se - base : 64);
+        if (got) {
+          hit = true;
+          hp = gw;
+          done = true;
+        }
+        base += 64;
+        if (base >= se) done = true;
+      }
+    }
+    // hand each result to its candidate lane
+#pragma unroll
+    for (uint32_t qq = 0; qq < 4; qq++) {
+      const int s_q = __shfl(my_src, (int)(16 * qq), 64);
+      const bool h_q = __shfl((int)hit, (int)(16 * qq), 64) != 0;
+      const uint32_t p_q = __shfl(hp, (int)(16 * qq), 64);
+      if (s_q == (int)lane) {
+        found = h_q;
+        par = p_q;
+      }
+    }
+  }
+}
+
 // Bottom-up step.  A wave owns 2048-vertex chunks (grid-stride): lane l
 // reads visited word l of the chunk (one coalesced load) and holds its
 // unvisited bits.  The unvisited vertices are then checked 64 at a time,
@@ -1321,99 +1426,7 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
         uint32_t par = par_[sl];
         PtrT b = b_[sl];
         const PtrT e = e_[sl], deg = deg_[sl];
-        const bool longer = e - b > PULL_LANE_PREFIX;  // the rest of the list
-        const PtrT lim = longer ? b + PULL_LANE_PREFIX : e;
-        // Every candidate first scans up to PULL_LANE_PREFIX neighbours in its
-        // own lane, four independent probes per round (indices past the end of
-        // the prefix repeat its last neighbour, which is harmless): all 64
-        // candidates of the wave progress together, and with hub-first lists
-        // most stop in the first round.  Longer lists not decided by their
-        // prefix continue four at a time, each by a quarter wave (16 lanes x 4
-        // loads = 64 consecutive neighbours per round, all loads in flight).
-        if (cand && !found) {
-          PtrT k = b;
-          for (; k < lim; k += 4) {
-            const PtrT last = lim - 1;
-            const uint32_t w[4] = {a.row_idx[k], a.row_idx[k + 1 < lim ? k + 1 : last],
-                                   a.row_idx[k + 2 < lim ? k + 2 : last],
-                                   a.row_idx[k + 3 < lim ? k + 3 : last]};
-            bool h[4];
-            in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
-            if (h[0] | h[1] | h[2] | h[3]) {
-              found = true;
-              par = h[0] ? w[0] : h[1] ? w[1] : h[2] ? w[2] : w[3];
-              break;
-            }
-          }
-          if (a.examined) scanned += (unsigned long long)((k + 4 < lim ? k + 4 : lim) - b);
-          b = lim;  // where a long list's quarter-wave scan resumes
-        }
-        unsigned long long big = __ballot(cand && longer && !found);
-        const uint32_t q = lane >> 4, ql = lane & 15;
-        const unsigned long long qmask = 0xFFFFull << (16 * q);
-        while (big) {
-          int my_src = -1;  // the candidate lane my quarter serves
-  #pragma unroll
-          for (uint32_t i = 0; i < 4; i++) {
-            if (big) {
-              if (i == q) my_src = __ffsll(big) - 1;
-              big &= big - 1;
-            }
-          }
-          const PtrT sb = __shfl(b, my_src < 0 ? 0 : my_src, 64);
-          const PtrT se = __shfl(e, my_src < 0 ? 0 : my_src, 64);
-          PtrT base = my_src < 0 ? se : sb;
-          bool done = base >= se, hit = false;
-          uint32_t hp = 0;
-          while (__ballot(!done)) {
-            uint32_t w[4];
-            bool h[4];
-  #pragma unroll
-            for (uint32_t j = 0; j < 4; j++) {
-              const PtrT k = base + j * 16 + ql;
-              // (entry 0 exists: some candidate has a long list)
-              w[j] = a.row_idx[(!done && k < se) ? k : PtrT(0)];
-            }
-            in_frontier4<SUM>(fbm_cur, s_sum, shift, w, h);
-  #pragma unroll
-            for (uint32_t j = 0; j < 4; j++) {
-              const PtrT k = base + j * 16 + ql;
-              h[j] = h[j] && !done && k < se;
-            }
-            bool got = false;
-            uint32_t gw = 0;
-  #pragma unroll
-            for (uint32_t j = 0; j < 4; j++) {
-              const unsigned long long hb = __ballot(h[j]) & qmask;
-              const uint32_t pw = __shfl(w[j], hb ? __ffsll(hb) - 1 : (int)lane, 64);
-              if (!got && hb) {
-                got = true;
-                gw = pw;
-              }
-            }
-            if (!done) {
-              if (a.examined && ql == 0) scanned += (unsigned long long)(se - base < 64 ? se - base : 64);
-              if (got) {
-                hit = true;
-                hp = gw;
-                done = true;
-              }
-              base += 64;
-              if (base >= se) done = true;
-            }
-          }
-          // hand each result to its candidate lane
-  #pragma unroll
-          for (uint32_t qq = 0; qq < 4; qq++) {
-            const int s_q = __shfl(my_src, (int)(16 * qq), 64);
-            const bool h_q = __shfl((int)hit, (int)(16 * qq), 64) != 0;
-            const uint32_t p_q = __shfl(hp, (int)(16 * qq), 64);
-            if (s_q == (int)lane) {
-              found = h_q;
-              par = p_q;
-            }
-          }
-        }
+        scan_rest<PtrT, SUM>(a, fbm_cur, s_sum, shift, cand, found, par, b, e, scanned);
         if (found) {
           a.levels[u] = a.next_level;
           if (a.parents) a.parents[u] = par;
@@ -1430,6 +1443,218 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
     }
     n_found += __popc(f);
     wave_sync_lds();  // fw is cleared by the next chunk
+  }
+  // one atomic per work-group for each total
+  __shared__ uint32_t s_n[NW];
+  __shared__ unsigned long long s_d[NW], s_x[NW];
+  const uint32_t nf = wave_sum(n_found);
+  const unsigned long long dg = wave_sum(found_deg);
+  const unsigned long long sx = a.examined ? wave_sum(scanned) : 0ull;
+  if (lane == 0) {
+    s_n[wv] = nf;
+    s_d[wv] = dg;
+    s_x[wv] = sx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t n = 0;
+    unsigned long long d = 0, x = 0;
+    for (int w = 0; w < NW; w++) {
+      n += s_n[w];
+      d += s_d[w];
+      x += s_x[w];
+    }
+    if (n) atomicAdd(a.q_out_count, n);
+    if (a.q_out_edges && d) atomicAdd(a.q_out_edges, d);
+    if (a.examined && x) atomicAdd(a.examined, x);
+  }
+}
+
+// Owner word and bit of the idx-th set bit of a wave's 64 words (lane l
+// holds word l as m, with excl = set bits in the words before it): the
+// last lane whose exclusive count is <= idx (it has a non-zero count, since a
+// zero-count lane ties with the next one), by a shuffle binary search.
+__device__ __forceinline__ void wave_kth_bit(uint32_t m, uint32_t excl, uint32_t idx,
+                                             uint32_t &word, uint32_t &bit) {
+  uint32_t o = 0;
+#pragma unroll
+  for (uint32_t st = 32; st > 0; st >>= 1) {
+    const uint32_t ex = __shfl(excl, (int)(o + st), 64);
+    if (ex <= idx) o += st;
+  }
+  const uint32_t om = __shfl(m, (int)o, 64);
+  const uint32_t oe = __shfl(excl, (int)o, 64);
+  word = o;
+  bit = kth_set_bit(om, idx - oe);
+}
+
+// Bottom-up step with the hub-first copy and the first-neighbour array, in
+// two phases per chunk.  Phase 1 decides every candidate it can from its
+// first (highest-degree) neighbour alone: fd entry, one probe (of the hub
+// bitmap for a hub); candidates left undecided with more neighbours get a
+// bit in an LDS mask.  Phase 2 takes those 64 * PULL_SLOTS at a time: fd
+// entry and col_ptr together, then the rest of the list (scan_rest).
+// Separating them keeps the short rounds of phase 1 from waiting for the
+// long chain (col_ptr, list line, probes) of a few undecided lanes.
+// Otherwise as k_pull: found vertices set bits in 64 LDS words the wave then
+// writes as the next frontier bitmap and ORs into the visited bitmap; sizes
+// and degree sums leave with one atomic per work-group.
+template <typename PtrT, int WG, bool SUM>
+__global__ __launch_bounds__(WG) void k_pull2(KArgs a, const uint32_t *fbm_cur,
+                                              uint32_t *fbm_next, const uint32_t *g_sum,
+                                              uint32_t sum_words, uint32_t shift,
+                                              const uint2 *fd, const uint32_t *iso,
+                                              const uint32_t *crank, const uint32_t *hbm,
+                                              const uint32_t *hub, SpecGuard guard) {
+  if (guard.skip()) return;
+  constexpr int NW = WG / 64;
+  constexpr int S1 = 2 * PULL_SLOTS;  // phase-1 candidates per lane per round
+  extern __shared__ uint32_t s_sum_mem[];  // sum_words entries when g_sum is set
+  __shared__ uint32_t s_found[NW][64];
+  __shared__ uint32_t s_undec[NW][64];
+  const PtrT *col_ptr = static_cast<const PtrT *>(a.col_ptr);
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t *s_sum = nullptr;
+  if (g_sum) {
+    for (uint32_t i = threadIdx.x; i < sum_words; i += WG) s_sum_mem[i] = g_sum[i];
+    __syncthreads();
+    s_sum = s_sum_mem;
+  }
+  const uint32_t n_words = (uint32_t)bitmap_words(a.n_cols);
+  const uint32_t n_chunks = (a.n_rows + PULL_CHUNK - 1) / PULL_CHUNK;
+  uint32_t n_found = 0;
+  unsigned long long found_deg = 0, scanned = 0;
+  uint32_t *fw = s_found[wv];
+  uint32_t *uw = s_undec[wv];
+  for (uint32_t c = blockIdx.x * NW + wv; c < n_chunks; c += gridDim.x * NW) {
+    const uint32_t wi = c * 64 + lane;
+    const uint32_t vbase = wi * 32;
+    const uint32_t vword = wi < n_words ? a.vis[wi] : ~0u;
+    uint32_t cm = ~vword;
+    uint32_t lw = wi < n_words ? ~iso[wi] : 0u;  // vertices with neighbours
+    if (vbase >= a.n_rows) {
+      cm = 0;
+      lw = 0;
+    } else if (a.n_rows - vbase < 32) {
+      cm &= (1u << (a.n_rows - vbase)) - 1u;
+      lw &= (1u << (a.n_rows - vbase)) - 1u;
+    }
+    const uint32_t cnt = __popc(cm);
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t excl = incl - cnt;
+    fw[lane] = 0;
+    uw[lane] = 0;
+    wave_sync_lds();
+    if (total) {
+      // fd entry of a vertex of this chunk: its rank among the vertices with
+      // neighbours (the candidates are among them: isolated ones start
+      // visited)
+      const uint32_t lc = __popc(lw);
+      const uint32_t lexcl = wave_incl_scan(lc) - lc;
+      const uint32_t lbase = crank[c];
+      // (its shuffles run on every lane: a shuffle from a lane that is not
+      // active returns 0, so callers select afterwards)
+      auto rank_of = [&](uint32_t o, uint32_t bit) {
+        const uint32_t ol = __shfl(lw, (int)o, 64);
+        const uint32_t ole = __shfl(lexcl, (int)o, 64);
+        return lbase + ole + __popc(ol & ((1u << bit) - 1u));
+      };
+      // phase 1: first neighbours
+      for (uint32_t r = 0; r < total; r += 64 * S1) {
+        bool cand[S1];
+        uint32_t off[S1], fi[S1];
+#pragma unroll
+        for (int sl = 0; sl < S1; sl++) {
+          const uint32_t idx = r + 64 * sl + lane;
+          cand[sl] = idx < total;
+          uint32_t o, bit;
+          wave_kth_bit(cm, excl, idx, o, bit);
+          off[sl] = cand[sl] ? o * 32 + bit : 0u;
+          // (not a candidate: the chunk's first vertex with neighbours)
+          const uint32_t rk = rank_of(o, bit);
+          fi[sl] = cand[sl] ? rk : lbase;
+        }
+        uint2 x[S1];
+#pragma unroll
+        for (int sl = 0; sl < S1; sl++) x[sl] = fd[fi[sl]];
+        bool hf[S1];
+#pragma unroll
+        for (int sl = 0; sl < S1; sl++)
+          hf[sl] = in_frontier_fd<SUM>(fbm_cur, hbm, s_sum, shift, cand[sl] ? x[sl].x : 0u);
+#pragma unroll
+        for (int sl = 0; sl < S1; sl++) {
+          if (!cand[sl]) continue;
+          const uint32_t u = c * PULL_CHUNK + off[sl];
+          if (a.examined) scanned += 1;
+          if (hf[sl]) {
+            a.levels[u] = a.next_level;
+            if (a.parents) {
+              const uint32_t f = x[sl].x;
+              a.parents[u] = (hbm && (f & PULL_HUB_FLAG)) ? hub[f & ~PULL_HUB_FLAG] : f;
+            }
+            atomicOr(&fw[off[sl] >> 5], 1u << (off[sl] & 31));
+            found_deg += (unsigned long long)x[sl].y;
+          } else if (x[sl].y > 1) {
+            atomicOr(&uw[off[sl] >> 5], 1u << (off[sl] & 31));
+          }
+        }
+      }
+      wave_sync_lds();
+      // phase 2: the rest of the undecided candidates' lists
+      const uint32_t um = uw[lane];
+      const uint32_t cnt2 = __popc(um);
+      const uint32_t incl2 = wave_incl_scan(cnt2);
+      const uint32_t total2 = __shfl(incl2, 63, 64);
+      const uint32_t excl2 = incl2 - cnt2;
+      for (uint32_t r = 0; r < total2; r += 64 * PULL_SLOTS) {
+        bool cand[PULL_SLOTS];
+        uint32_t off[PULL_SLOTS], fi[PULL_SLOTS];
+#pragma unroll
+        for (int sl = 0; sl < PULL_SLOTS; sl++) {
+          const uint32_t idx = r + 64 * sl + lane;
+          cand[sl] = idx < total2;
+          uint32_t o, bit;
+          wave_kth_bit(um, excl2, idx, o, bit);
+          off[sl] = cand[sl] ? o * 32 + bit : 0u;
+          const uint32_t rk = rank_of(o, bit);
+          fi[sl] = cand[sl] ? rk : lbase;
+        }
+        // (u is a vertex of this chunk below n_rows for every lane)
+        uint2 x[PULL_SLOTS];
+        PtrT cb[PULL_SLOTS];
+#pragma unroll
+        for (int sl = 0; sl < PULL_SLOTS; sl++) {
+          x[sl] = fd[fi[sl]];
+          cb[sl] = col_ptr[c * PULL_CHUNK + off[sl]];
+        }
+#pragma unroll
+        for (int sl = 0; sl < PULL_SLOTS; sl++) {
+          const PtrT deg = cand[sl] ? (PtrT)x[sl].y : PtrT(0);
+          bool found = false;
+          uint32_t par = 0;
+          // (the first neighbour is done; not a candidate: an empty range)
+          scan_rest<PtrT, SUM>(a, fbm_cur, s_sum, shift, cand[sl], found, par, cb[sl] + 1,
+                               cand[sl] ? cb[sl] + deg : cb[sl] + 1, scanned);
+          if (found) {
+            const uint32_t u = c * PULL_CHUNK + off[sl];
+            a.levels[u] = a.next_level;
+            if (a.parents) a.parents[u] = par;
+            atomicOr(&fw[off[sl] >> 5], 1u << (off[sl] & 31));
+            found_deg += (unsigned long long)deg;
+          }
+        }
+      }
+    }
+    wave_sync_lds();
+    const uint32_t f = fw[lane];
+    if (wi < n_words) {
+      fbm_next[wi] = f;
+      if (f) a.vis[wi] = vword | f;
+    }
+    n_found += __popc(f);
+    wave_sync_lds();  // fw and uw are cleared by the next chunk
   }
   // one atomic per work-group for each total
   __shared__ uint32_t s_n[NW];
@@ -2161,6 +2386,30 @@ static uint32_t pull_summary_shift(uint64_t n, uint32_t max_words) {
   return shift;
 }
 
+// The two-phase kernel wherever the first-neighbour array exists.
+template <typename PtrT, int WG>
+static void launch_pull(unsigned grid, size_t lds, hipStream_t st, const KArgs &a,
+                        const StepArgs &s, const uint32_t *g_sum, uint32_t sum_words,
+                        uint32_t shift, const uint2 *fd, const uint32_t *iso,
+                        const uint32_t *crank, const uint32_t *hbm, const uint32_t *hub,
+                        SpecGuard guard) {
+  if (fd) {
+    if (g_sum)
+      hipLaunchKernelGGL((k_pull2<PtrT, WG, true>), dim3(grid), dim3(WG), lds, st, a, s.fbm_cur,
+                         s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+    else
+      hipLaunchKernelGGL((k_pull2<PtrT, WG, false>), dim3(grid), dim3(WG), lds, st, a, s.fbm_cur,
+                         s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+  } else {
+    if (g_sum)
+      hipLaunchKernelGGL((k_pull<PtrT, WG, true>), dim3(grid), dim3(WG), lds, st, a, s.fbm_cur,
+                         s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+    else
+      hipLaunchKernelGGL((k_pull<PtrT, WG, false>), dim3(grid), dim3(WG), lds, st, a, s.fbm_cur,
+                         s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+  }
+}
+
 template <typename PtrT>
 static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
   const KArgs a = kargs(s, 0);
@@ -2170,7 +2419,8 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
   // The summary fills the LDS that the kernel's own arrays leave free (at
   // most PULL_SUM_WORDS_MAX words), and pays when at most a quarter of its
   // bits can be set.
-  const uint32_t static_lds = (wg / 64) * 64 * 4 + 1024;
+  // (static LDS: the found and undecided words of every wave, and the rest)
+  const uint32_t static_lds = 2 * (wg / 64) * 64 * 4 + 1024;
   uint32_t max_words = ctx->lds_per_block > static_lds ? (ctx->lds_per_block - static_lds) / 4 : 0;
   max_words = max_words > PULL_SUM_WORDS_MAX ? PULL_SUM_WORDS_MAX : max_words;
   const uint32_t shift = pull_summary_shift(s.n_cols, max_words ? max_words : 1);
@@ -2218,22 +2468,10 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
   }
   KernelTimer kt(s, st);
   switch (wg) {
-    case 64:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 64, true>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
-      else hipLaunchKernelGGL((k_pull<PtrT, 64, false>), dim3(grid), dim3(64), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
-      break;
-    case 128:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 128, true>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
-      else hipLaunchKernelGGL((k_pull<PtrT, 128, false>), dim3(grid), dim3(128), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
-      break;
-    case 256:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 256, true>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
-      else hipLaunchKernelGGL((k_pull<PtrT, 256, false>), dim3(grid), dim3(256), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
-      break;
-    case 512:
-      if (g_sum) hipLaunchKernelGGL((k_pull<PtrT, 512, true>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
-      else hipLaunchKernelGGL((k_pull<PtrT, 512, false>), dim3(grid), dim3(512), lds, st, a, s.fbm_cur, s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
-      break;
+    case 64: launch_pull<PtrT, 64>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard); break;
+    case 128: launch_pull<PtrT, 128>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard); break;
+    case 256: launch_pull<PtrT, 256>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard); break;
+    case 512: launch_pull<PtrT, 512>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard); break;
     default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
   }
 }
