@@ -1169,6 +1169,24 @@ __device__ __forceinline__ uint32_t kth_set_bit(uint32_t m, uint32_t k) {
   return p;
 }
 
+// Owner word and bit of the idx-th set bit of a wave's 64 words (lane l
+// holds word l as m, with excl = set bits in the words before it): the
+// last lane whose exclusive count is <= idx (it has a non-zero count, since a
+// zero-count lane ties with the next one), by a shuffle binary search.
+__device__ __forceinline__ void wave_kth_bit(uint32_t m, uint32_t excl, uint32_t idx,
+                                             uint32_t &word, uint32_t &bit) {
+  uint32_t o = 0;
+#pragma unroll
+  for (uint32_t st = 32; st > 0; st >>= 1) {
+    const uint32_t ex = __shfl(excl, (int)(o + st), 64);
+    if (ex <= idx) o += st;
+  }
+  const uint32_t om = __shfl(m, (int)o, 64);
+  const uint32_t oe = __shfl(excl, (int)o, 64);
+  word = o;
+  bit = kth_set_bit(om, idx - oe);
+}
+
 // The rest of a candidate's list, from entry b to e (wave-collective: every
 // lane calls it; cand says whether the lane holds a candidate still to be
 // decided).  Every candidate first scans up to PULL_LANE_PREFIX neighbours in
@@ -1274,7 +1292,9 @@ __device__ __forceinline__ void scan_rest(const KArgs &a, const uint32_t *fbm_cu
   }
 }
 
-// Bottom-up step.  A wave owns 2048-vertex chunks (grid-stride): lane l
+// Bottom-up step over plain lists (without the hub-first copy and its
+// first-neighbour array; k_pull2 below is the step with them).  A wave owns
+// 2048-vertex chunks (grid-stride): lane l
 // reads visited word l of the chunk (one coalesced load) and holds its
 // unvisited bits.  The unvisited vertices are then checked 64 at a time,
 // one per lane: a wave prefix sum of the per-word counts and a shuffle
@@ -1290,9 +1310,7 @@ template <typename PtrT, int WG, bool SUM>
 __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
                                              uint32_t *fbm_next, const uint32_t *g_sum,
                                              uint32_t sum_words, uint32_t shift,
-                                             const uint2 *fd, const uint32_t *iso,
-                                             const uint32_t *crank, const uint32_t *hbm,
-                                             const uint32_t *hub, SpecGuard guard) {
+                                             SpecGuard guard) {
   if (guard.skip()) return;
   constexpr int NW = WG / 64;
   extern __shared__ uint32_t s_sum_mem[];  // sum_words entries when g_sum is set
@@ -1322,92 +1340,25 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
     const uint32_t incl = wave_incl_scan(cnt);
     const uint32_t total = __shfl(incl, 63, 64);
     const uint32_t excl = incl - cnt;
-    // fd holds the vertices with neighbours only: this lane's word of them
-    // and the count before it (the candidates are among them: isolated
-    // vertices start visited)
-    uint32_t lw = 0, lexcl = 0, lbase = 0;
-    if (fd && total) {
-      lw = wi < n_words ? ~iso[wi] : 0u;
-      if (vbase >= a.n_rows) lw = 0;
-      else if (a.n_rows - vbase < 32) lw &= (1u << (a.n_rows - vbase)) - 1u;
-      const uint32_t lc = __popc(lw);
-      lexcl = wave_incl_scan(lc) - lc;
-      lbase = crank[c];
-    }
     fw[lane] = 0;
     wave_sync_lds();
     for (uint32_t r = 0; r < total; r += 64 * PULL_SLOTS) {
-      // PULL_SLOTS candidates per lane per round: their first loads and
-      // probes (phase 1) are in flight together; the rest of each one's scan
-      // (phase 2) follows slot by slot, uniformly over the wave.
-      bool cand_[PULL_SLOTS], found_[PULL_SLOTS];
-      uint32_t off_[PULL_SLOTS], u_[PULL_SLOTS], par_[PULL_SLOTS], fi_[PULL_SLOTS];
+      // PULL_SLOTS candidates per lane per round: their col_ptr loads are in
+      // flight together; the scans follow slot by slot, uniformly over the
+      // wave.
+      bool cand_[PULL_SLOTS];
+      uint32_t off_[PULL_SLOTS], u_[PULL_SLOTS];
       PtrT b_[PULL_SLOTS], e_[PULL_SLOTS], deg_[PULL_SLOTS];
 #pragma unroll
       for (int sl = 0; sl < PULL_SLOTS; sl++) {
         const uint32_t idx = r + 64 * sl + lane;
         cand_[sl] = idx < total;
-        // owner word: the last lane whose exclusive count is <= idx (it has
-        // a non-zero count, since a zero-count lane ties with the next one)
-        uint32_t o = 0;
-#pragma unroll
-        for (uint32_t st = 32; st > 0; st >>= 1) {
-          const uint32_t ex = __shfl(excl, (int)(o + st), 64);
-          if (ex <= idx) o += st;
-        }
-        const uint32_t om = __shfl(cm, (int)o, 64);
-        const uint32_t oe = __shfl(excl, (int)o, 64);
-        const uint32_t bit = kth_set_bit(om, idx - oe);
+        uint32_t o, bit;
+        wave_kth_bit(cm, excl, idx, o, bit);
         off_[sl] = cand_[sl] ? o * 32 + bit : 0u;
         u_[sl] = c * PULL_CHUNK + off_[sl];
-        if (fd) {
-          // rank among the vertices with neighbours (not a candidate: the
-          // chunk's first such vertex, or the padding entry)
-          const uint32_t ol = __shfl(lw, (int)o, 64);
-          const uint32_t ole = __shfl(lexcl, (int)o, 64);
-          fi_[sl] = lbase + (cand_[sl] ? ole + __popc(ol & ((1u << bit) - 1u)) : 0u);
-        }
-        found_[sl] = false;
-        par_[sl] = 0;
         b_[sl] = e_[sl] = deg_[sl] = 0;
       }
-      if (fd) {
-        // With the hub-first copy, fd = {first neighbour, degree} of every
-        // vertex with neighbours (a dense array read in vertex order) holds
-        // u's highest-degree neighbour: on dense frontiers that probe
-        // decides most candidates, which then cost neither a col_ptr nor a
-        // row_idx line.  A first neighbour among the top-degree vertices is
-        // named by its hub index and probed in the hub bitmap.  (fi_ is an
-        // entry of the array for every lane, so the loads need no branch.)
-        uint2 x[PULL_SLOTS];
-#pragma unroll
-        for (int sl = 0; sl < PULL_SLOTS; sl++) x[sl] = fd[fi_[sl]];
-        uint32_t f[PULL_SLOTS];
-        bool hf[PULL_SLOTS];
-#pragma unroll
-        for (int sl = 0; sl < PULL_SLOTS; sl++) {
-          deg_[sl] = cand_[sl] ? (PtrT)x[sl].y : PtrT(0);
-          f[sl] = deg_[sl] > 0 ? x[sl].x : 0u;
-        }
-#pragma unroll
-        for (int sl = 0; sl < PULL_SLOTS; sl++)
-          hf[sl] = in_frontier_fd<SUM>(fbm_cur, hbm, s_sum, shift, f[sl]);
-#pragma unroll
-        for (int sl = 0; sl < PULL_SLOTS; sl++) {
-          if (deg_[sl] > 0) {
-            if (hf[sl]) {
-              found_[sl] = true;
-              const bool by_hub = hbm && (f[sl] & PULL_HUB_FLAG);
-              par_[sl] = !by_hub ? f[sl] : a.parents ? hub[f[sl] & ~PULL_HUB_FLAG] : 0u;
-            }
-            if (a.examined) scanned += 1;
-          }
-          if (cand_[sl] && !found_[sl] && deg_[sl] > 1) {
-            b_[sl] = col_ptr[u_[sl]] + 1;  // the first neighbour is done
-            e_[sl] = b_[sl] + (deg_[sl] - 1);
-          }
-        }
-      } else {
 #pragma unroll
         for (int sl = 0; sl < PULL_SLOTS; sl++) {
           const PtrT cb = col_ptr[u_[sl]], ce = col_ptr[u_[sl] + 1];
@@ -1417,15 +1368,13 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
             deg_[sl] = ce - cb;
           }
         }
-      }
 #pragma unroll
       for (int sl = 0; sl < PULL_SLOTS; sl++) {
         const bool cand = cand_[sl];
         const uint32_t off = off_[sl], u = u_[sl];
-        bool found = found_[sl];
-        uint32_t par = par_[sl];
-        PtrT b = b_[sl];
-        const PtrT e = e_[sl], deg = deg_[sl];
+        bool found = false;
+        uint32_t par = 0;
+        const PtrT b = b_[sl], e = e_[sl], deg = deg_[sl];
         scan_rest<PtrT, SUM>(a, fbm_cur, s_sum, shift, cand, found, par, b, e, scanned);
         if (found) {
           a.levels[u] = a.next_level;
@@ -1470,23 +1419,6 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
   }
 }
 
-// Owner word and bit of the idx-th set bit of a wave's 64 words (lane l
-// holds word l as m, with excl = set bits in the words before it): the
-// last lane whose exclusive count is <= idx (it has a non-zero count, since a
-// zero-count lane ties with the next one), by a shuffle binary search.
-__device__ __forceinline__ void wave_kth_bit(uint32_t m, uint32_t excl, uint32_t idx,
-                                             uint32_t &word, uint32_t &bit) {
-  uint32_t o = 0;
-#pragma unroll
-  for (uint32_t st = 32; st > 0; st >>= 1) {
-    const uint32_t ex = __shfl(excl, (int)(o + st), 64);
-    if (ex <= idx) o += st;
-  }
-  const uint32_t om = __shfl(m, (int)o, 64);
-  const uint32_t oe = __shfl(excl, (int)o, 64);
-  word = o;
-  bit = kth_set_bit(om, idx - oe);
-}
 
 // Bottom-up step with the hub-first copy and the first-neighbour array, in
 // two phases per chunk.  Phase 1 decides every candidate it can from its
@@ -1569,11 +1501,13 @@ __global__ __launch_bounds__(WG) void k_pull2(KArgs a, const uint32_t *fbm_cur,
         for (int sl = 0; sl < S1; sl++) {
           const uint32_t idx = r + 64 * sl + lane;
           cand[sl] = idx < total;
-          uint32_t o, bit;
-          wave_kth_bit(cm, excl, idx, o, bit);
+          uint32_t o = 0, bit = 0, rk = lbase;
+          if (r + 64 * sl < total) {  // (uniform: slots past the chunk's last do nothing)
+            wave_kth_bit(cm, excl, idx, o, bit);
+            rk = rank_of(o, bit);
+          }
           off[sl] = cand[sl] ? o * 32 + bit : 0u;
           // (not a candidate: the chunk's first vertex with neighbours)
-          const uint32_t rk = rank_of(o, bit);
           fi[sl] = cand[sl] ? rk : lbase;
         }
         uint2 x[S1];
@@ -1615,10 +1549,12 @@ __global__ __launch_bounds__(WG) void k_pull2(KArgs a, const uint32_t *fbm_cur,
         for (int sl = 0; sl < PULL_SLOTS; sl++) {
           const uint32_t idx = r + 64 * sl + lane;
           cand[sl] = idx < total2;
-          uint32_t o, bit;
-          wave_kth_bit(um, excl2, idx, o, bit);
+          uint32_t o = 0, bit = 0, rk = lbase;
+          if (r + 64 * sl < total2) {
+            wave_kth_bit(um, excl2, idx, o, bit);
+            rk = rank_of(o, bit);
+          }
           off[sl] = cand[sl] ? o * 32 + bit : 0u;
-          const uint32_t rk = rank_of(o, bit);
           fi[sl] = cand[sl] ? rk : lbase;
         }
         // (u is a vertex of this chunk below n_rows for every lane)
@@ -2403,10 +2339,10 @@ static void launch_pull(unsigned grid, size_t lds, hipStream_t st, const KArgs &
   } else {
     if (g_sum)
       hipLaunchKernelGGL((k_pull<PtrT, WG, true>), dim3(grid), dim3(WG), lds, st, a, s.fbm_cur,
-                         s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+                         s.fbm_next, g_sum, sum_words, shift, guard);
     else
       hipLaunchKernelGGL((k_pull<PtrT, WG, false>), dim3(grid), dim3(WG), lds, st, a, s.fbm_cur,
-                         s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+                         s.fbm_next, g_sum, sum_words, shift, guard);
   }
 }
 
