@@ -68,7 +68,7 @@ def pull_bytes(n, UV, S, D, w_ptr, L_b=4):
 
 
 KERNEL_NAMES = {"llb": "k_llb", "slb": "k_llb (strided)", "glb": "k_glb_expand + scan kernels",
-                "pull": "k_pull"}
+                "pull": "k_pull2"}
 
 
 def roofline_leg(eng, roots, strategy, info):
