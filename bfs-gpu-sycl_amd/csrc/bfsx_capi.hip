@@ -329,8 +329,9 @@ struct Walker {
   // read, the bottom-up step of the next level, guarded on the device: it
   // runs only if this level's end picks PULL for the next level and the
   // frontier is not empty, so that a run of bottom-up levels does not wait
-  // for a host round trip per level.  Only after a bottom-up step (the next
-  // frontier then exists as the bitmap it wrote), learned, untimed.
+  // for a host round trip per level.  Only after a step that wrote the next
+  // frontier as a bitmap (bottom-up, or bucketed or marking top-down),
+  // learned, untimed.
   void launch_speculative_pull() {
     StepArgs a{};
     a.col_ptr = ctx->col_ptr.p;
@@ -565,6 +566,7 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
   // speculative steps (see Walker::launch_speculative_pull / _td)
   const bool can_spec = ctx->params.speculate && learned && !timed;
   int spec = -1;  // the kind speculated for level w.level: BFSX_PULL, BFSX_LLB (LLB or SLB) or -1
+  uint64_t prev_F = 1;  // the previous level's frontier size
   while (w.F > 0) {
     // a search has at most n_cols levels; more means a broken invariant
     if (w.level > ctx->n_cols) fail(BFSX_ERR_DEVICE, "level loop did not terminate");
@@ -616,14 +618,22 @@ static RunOut search(bfsx_ctx *ctx, uint32_t root, int strategy, uint32_t flags,
       // (a speculative step that was not picked exits without a write)
       new_fbm = w.launch(choice);
     }
+    // The guess for the next level: bottom-up after a bottom-up step, and
+    // after a top-down step that wrote the next frontier's bitmap (bucketed
+    // or marking: heavy, and usually followed by bottom-up levels); a thin
+    // top-down step after another top-down step whose frontier did not grow
+    // more than twofold (the tail of a search, a grid), not while the
+    // frontier grows fast (a guess that fails costs its empty launches).
     spec = -1;
-    if (can_spec && choice == BFSX_PULL && w.pull_possible) {
+    if (can_spec && w.pull_possible && (choice == BFSX_PULL || new_fbm)) {
       w.launch_speculative_pull();
       spec = BFSX_PULL;
-    } else if (can_spec && !dev_loop && choice != BFSX_PULL && !par) {
+    } else if (can_spec && !dev_loop && choice != BFSX_PULL && !par && level > 0 &&
+               (uint64_t)F <= 2ull * prev_F) {
       w.launch_speculative_td(mr);
       spec = BFSX_LLB;
     }
+    prev_F = F;
     if (timed) BFSX_HIP(hipEventRecord(ctx->ev_step1, s));
     read_level_end(ctx, ++seq, cur ^ 1);
     // per-level records read the whole block (queue sizes, running totals,
