@@ -517,23 +517,53 @@ __global__ __launch_bounds__(1024) void k_glb_tile_offsets(PtrT *tile_sum, uint3
   }
 }
 
+// Chunk-table entries one thread writes itself; longer ranges (hubs) are
+// written by a whole wave or group.
+constexpr uint32_t GLB_SMALL_SERIAL = 4;
+
+// ceil(x / chunk): a shift when chunk is a power of two (shift = its log2,
+// else 0), which it is for the bucketed passes and the tuned sizes (a 64-bit
+// division is a long VALU sequence, and the small scan runs on one CU).
+__device__ __forceinline__ unsigned long long chunk_div_up(unsigned long long x, uint32_t chunk,
+                                                           uint32_t shift) {
+  return shift ? (x + chunk - 1) >> shift : (x + chunk - 1) / chunk;
+}
+
 // Global exclusive offsets, and chunk_lo[c] = frontier entry holding edge c*chunk.
 template <typename PtrT>
 __global__ void k_glb_offsets(const PtrT *g_loc, const PtrT *tile_off, const PtrT *g_scan_end,
-                              uint32_t F, PtrT *g_scan, uint32_t chunk, uint32_t *chunk_lo) {
+                              uint32_t F, PtrT *g_scan, uint32_t chunk, uint32_t shift,
+                              uint32_t *chunk_lo) {
   const PtrT total = *g_scan_end;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F; i += gridDim.x * blockDim.x) {
-    const PtrT ex = g_loc[i] + tile_off[i / GLB_TILE];
-    PtrT in;
-    if (i + 1 < F)
-      in = g_loc[i + 1] + tile_off[(i + 1) / GLB_TILE];
-    else
-      in = total;
-    g_scan[i] = ex;
-    // chunks whose first edge falls in [ex, in)
-    const unsigned long long c0 = ((unsigned long long)ex + chunk - 1) / chunk;
-    const unsigned long long c1 = ((unsigned long long)in + chunk - 1) / chunk;
-    for (unsigned long long c = c0; c < c1; c++) chunk_lo[c] = i;
+  const uint32_t lane = threadIdx.x & 63;
+  // (every lane runs every round, so that the wave can write long ranges)
+  for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < F; i0 += gridDim.x * blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    unsigned long long c0 = 0, c1 = 0;
+    if (i < F) {
+      const PtrT ex = g_loc[i] + tile_off[i / GLB_TILE];
+      PtrT in;
+      if (i + 1 < F)
+        in = g_loc[i + 1] + tile_off[(i + 1) / GLB_TILE];
+      else
+        in = total;
+      g_scan[i] = ex;
+      // chunks whose first edge falls in [ex, in)
+      c0 = chunk_div_up((unsigned long long)ex, chunk, shift);
+      c1 = chunk_div_up((unsigned long long)in, chunk, shift);
+    }
+    const bool longr = c1 - c0 > GLB_SMALL_SERIAL;
+    if (!longr)
+      for (unsigned long long c = c0; c < c1; c++) chunk_lo[c] = i;
+    // a hub's range (hundreds of entries) by the whole wave
+    unsigned long long m = __ballot(longr);
+    while (m) {
+      const int l = __ffsll(m) - 1;
+      m &= m - 1;
+      const unsigned long long a0 = __shfl(c0, l, 64), a1 = __shfl(c1, l, 64);
+      const uint32_t owner = __shfl(i, l, 64);
+      for (unsigned long long c = a0 + lane; c < a1; c += 64) chunk_lo[c] = owner;
+    }
   }
 }
 
@@ -544,12 +574,21 @@ __global__ void k_glb_offsets(const PtrT *g_loc, const PtrT *tile_off, const Ptr
 // entry's [offset, offset + degree).  Saves two launches and their gaps.
 constexpr uint32_t GLB_SMALL_F = 16384;
 constexpr int GLB_SMALL_WG = 1024;
+constexpr uint32_t GLB_SMALL_BIG = 1024;   // longer ranges listed per round
 
 template <typename PtrT>
 __global__ __launch_bounds__(GLB_SMALL_WG) void k_glb_scan_small(
     const PtrT *col_ptr, uint32_t n_rows, const uint32_t *q_in, uint32_t F, PtrT *g_start,
-    PtrT *g_scan, uint32_t chunk, uint32_t *chunk_lo, LevelCtl *ctl) {
+    PtrT *g_scan, uint32_t chunk, uint32_t shift, uint32_t *chunk_lo, LevelCtl *ctl) {
   __shared__ PtrT s_wave[GLB_SMALL_WG / 64 + 1];
+  // Entries whose edges span more than GLB_SMALL_SERIAL chunks (hubs) are
+  // listed here and their chunk-table ranges written by the whole group: one
+  // thread writing a hub's hundreds of entries serialised the kernel.
+  __shared__ uint32_t s_nbig;
+  __shared__ uint32_t s_big_i[GLB_SMALL_BIG];
+  __shared__ unsigned long long s_big_c0[GLB_SMALL_BIG], s_big_c1[GLB_SMALL_BIG];
+  if (threadIdx.x == 0) s_nbig = 0;
+  __syncthreads();
   PtrT carry = 0;
   for (uint32_t r0 = 0; r0 < F; r0 += GLB_SMALL_WG * 8) {
     const uint32_t base = r0 + threadIdx.x * 8;
@@ -577,14 +616,30 @@ __global__ __launch_bounds__(GLB_SMALL_WG) void k_glb_scan_small(
         g_start[i] = st[k];
         g_scan[i] = ex;
         // chunks whose first edge falls in [ex, ex + d)
-        const unsigned long long c0 = ((unsigned long long)ex + chunk - 1) / chunk;
-        const unsigned long long c1 = ((unsigned long long)(ex + d[k]) + chunk - 1) / chunk;
-        for (unsigned long long c = c0; c < c1; c++) chunk_lo[c] = i;
+        const unsigned long long c0 = chunk_div_up((unsigned long long)ex, chunk, shift);
+        const unsigned long long c1 = chunk_div_up((unsigned long long)(ex + d[k]), chunk, shift);
+        uint32_t slot = GLB_SMALL_BIG;
+        if (c1 - c0 > GLB_SMALL_SERIAL) slot = atomicAdd(&s_nbig, 1u);
+        if (slot < GLB_SMALL_BIG) {
+          s_big_i[slot] = i;
+          s_big_c0[slot] = c0;
+          s_big_c1[slot] = c1;
+        } else {
+          for (unsigned long long c = c0; c < c1; c++) chunk_lo[c] = i;
+        }
       }
       ex += d[k];
     }
     carry += total;
-    __syncthreads();  // s_wave is rewritten by the next round's scan
+    __syncthreads();  // s_wave is rewritten by the next round's scan; the list is complete
+    // (a wave per listed entry, its lanes over the entry's range)
+    const uint32_t nb = s_nbig < GLB_SMALL_BIG ? s_nbig : GLB_SMALL_BIG;
+    for (uint32_t j = threadIdx.x >> 6; j < nb; j += GLB_SMALL_WG / 64)
+      for (unsigned long long c = s_big_c0[j] + (threadIdx.x & 63); c < s_big_c1[j]; c += 64)
+        chunk_lo[c] = s_big_i[j];
+    __syncthreads();
+    if (threadIdx.x == 0) s_nbig = 0;
+    __syncthreads();
   }
   if (threadIdx.x == 0) {
     ctl->glb_total = (unsigned long long)carry;
@@ -2230,8 +2285,11 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   // frontier of two hubs otherwise runs on a handful of work-groups).
   uint32_t chunk = bucketed ? BKT_CHUNK : ctx->params.glb_chunk;
   if (!bucketed && s.EF > 0 && s.EF < (unsigned long long)chunk * grid) {
+    // (a power of two: the chunk table's divisions become shifts)
     const unsigned long long want = (s.EF + grid - 1) / grid;
-    chunk = (uint32_t)std::max<unsigned long long>(GLB_MIN_CHUNK, (want + 255) & ~255ull);
+    uint32_t c = GLB_MIN_CHUNK;
+    while (c < want && c < chunk) c <<= 1;
+    chunk = std::min(c, chunk);
   }
   const uint32_t n_tiles = (F + GLB_TILE - 1) / GLB_TILE;
   PtrT *g_loc = ctx->glb_scan.as<PtrT>();             // F entries, then F+1 global
@@ -2239,10 +2297,13 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
   PtrT *g_start = ctx->glb_start.as<PtrT>();
   PtrT *tile = ctx->glb_tile.as<PtrT>();
   PtrT *g_end = tile + ((uint64_t)ctx->n_cols / GLB_TILE + 2);
+  uint32_t shift = 0;  // log2(chunk) when chunk is a power of two
+  if ((chunk & (chunk - 1)) == 0)
+    while ((1u << shift) < chunk) shift++;
   if (F <= GLB_SMALL_F) {
     hipLaunchKernelGGL(k_glb_scan_small<PtrT>, dim3(1), dim3(GLB_SMALL_WG), 0, st,
                        static_cast<const PtrT *>(s.col_ptr), s.n_rows, s.q_in, F, g_start, g_scan,
-                       chunk, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl);
+                       chunk, shift, ctx->glb_chunk_lo.as<uint32_t>(), s.ctl);
   } else {
     hipLaunchKernelGGL(k_glb_tiles<PtrT>, dim3(n_tiles), dim3(256), 0, st,
                        static_cast<const PtrT *>(s.col_ptr), s.n_rows, s.q_in, F, g_loc, g_start,
@@ -2250,7 +2311,7 @@ static bool step_glb_t(bfsx_ctx *ctx, const StepArgs &s) {
     hipLaunchKernelGGL(k_glb_tile_offsets<PtrT>, dim3(1), dim3(1024), 0, st, tile, n_tiles,
                        chunk, s.ctl, g_end);
     hipLaunchKernelGGL(k_glb_offsets<PtrT>, dim3(cap_grid((F + 255) / 256, 65535)), dim3(256),
-                       0, st, g_loc, tile, g_end, F, g_scan, chunk,
+                       0, st, g_loc, tile, g_end, F, g_scan, chunk, shift,
                        ctx->glb_chunk_lo.as<uint32_t>());
   }
   const KArgs a = kargs(s, 0);
