@@ -184,10 +184,13 @@ typedef struct bfsx_params {
                                fits resident; default 256; grid-wide
                                barriers cost more the more groups take
                                part) */
-  uint32_t speculate;       /* learned scheduler: after a bottom-up level,
-                               launch the next level's bottom-up step before
-                               the decision is read back; it runs only if
-                               the decision is PULL (1, default) or not (0) */
+  uint32_t speculate;       /* learned scheduler, untimed runs: launch the
+                               next level's step before the current level's
+                               decision is read back (a bottom-up step after
+                               a level that wrote the next frontier's bitmap,
+                               a thin LLB/SLB step after a top-down level);
+                               it runs only if the decision matches, else
+                               exits without a write (1, default) or not (0) */
 } bfsx_params;
 
 /* ---- context ------------------------------------------------------------ */
