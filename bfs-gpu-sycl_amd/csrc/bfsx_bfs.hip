@@ -1544,8 +1544,8 @@ __global__ __launch_bounds__(WG) void k_pull(KArgs a, const uint32_t *fbm_cur,
 // two phases per chunk.  Phase 1 decides every candidate it can from its
 // first (highest-degree) neighbour alone: fd entry, one probe (of the hub
 // bitmap for a hub); candidates left undecided with more neighbours get a
-// bit in an LDS mask.  Phase 2 takes those 64 * PULL_SLOTS at a time: fd
-// entry and col_ptr together, then the rest of the list (scan_rest).
+// bit in an LDS mask.  Phase 2 takes those 64 * PULL_SLOTS at a time: their
+// col_ptr pair, then the rest of the list (scan_rest).
 // Separating them keeps the short rounds of phase 1 from waiting for the
 // long chain (col_ptr, list line, probes) of a few undecided lanes.
 // Otherwise as k_pull: found vertices set bits in 64 LDS words the wave then
@@ -1555,7 +1555,8 @@ template <typename PtrT, int WG, bool SUM>
 __global__ __launch_bounds__(WG) void k_pull2(KArgs a, const uint32_t *fbm_cur,
                                               uint32_t *fbm_next, const uint32_t *g_sum,
                                               uint32_t sum_words, uint32_t shift,
-                                              const uint2 *fd, const uint32_t *iso,
+                                              const uint32_t *fd, const uint16_t *fdd,
+                                              const uint32_t *iso,
                                               const uint32_t *crank, const uint32_t *hbm,
                                               const uint32_t *hub, SpecGuard guard) {
   if (guard.skip()) return;
@@ -1630,13 +1631,16 @@ __global__ __launch_bounds__(WG) void k_pull2(KArgs a, const uint32_t *fbm_cur,
           // (not a candidate: the chunk's first vertex with neighbours)
           fi[sl] = cand[sl] ? rk : lbase;
         }
-        uint2 x[S1];
+        uint32_t f[S1], d16[S1];
 #pragma unroll
-        for (int sl = 0; sl < S1; sl++) x[sl] = fd[fi[sl]];
+        for (int sl = 0; sl < S1; sl++) {
+          f[sl] = fd[fi[sl]];
+          d16[sl] = fdd[fi[sl]];
+        }
         bool hf[S1];
 #pragma unroll
         for (int sl = 0; sl < S1; sl++)
-          hf[sl] = in_frontier_fd<SUM>(fbm_cur, hbm, s_sum, shift, cand[sl] ? x[sl].x : 0u);
+          hf[sl] = in_frontier_fd<SUM>(fbm_cur, hbm, s_sum, shift, cand[sl] ? f[sl] : 0u);
 #pragma unroll
         for (int sl = 0; sl < S1; sl++) {
           if (!cand[sl]) continue;
@@ -1644,13 +1648,12 @@ __global__ __launch_bounds__(WG) void k_pull2(KArgs a, const uint32_t *fbm_cur,
           if (a.examined) scanned += 1;
           if (hf[sl]) {
             a.levels[u] = a.next_level;
-            if (a.parents) {
-              const uint32_t f = x[sl].x;
-              a.parents[u] = (hbm && (f & PULL_HUB_FLAG)) ? hub[f & ~PULL_HUB_FLAG] : f;
-            }
+            if (a.parents)
+              a.parents[u] = (hbm && (f[sl] & PULL_HUB_FLAG)) ? hub[f[sl] & ~PULL_HUB_FLAG] : f[sl];
             atomicOr(&fw[off[sl] >> 5], 1u << (off[sl] & 31));
-            found_deg += (unsigned long long)x[sl].y;
-          } else if (x[sl].y > 1) {
+            found_deg += d16[sl] != DEG16_ESCAPE ? (unsigned long long)d16[sl]
+                                                 : (unsigned long long)(col_ptr[u + 1] - col_ptr[u]);
+          } else if (d16[sl] > 1) {
             atomicOr(&uw[off[sl] >> 5], 1u << (off[sl] & 31));
           }
         }
@@ -1664,30 +1667,25 @@ __global__ __launch_bounds__(WG) void k_pull2(KArgs a, const uint32_t *fbm_cur,
       const uint32_t excl2 = incl2 - cnt2;
       for (uint32_t r = 0; r < total2; r += 64 * PULL_SLOTS) {
         bool cand[PULL_SLOTS];
-        uint32_t off[PULL_SLOTS], fi[PULL_SLOTS];
+        uint32_t off[PULL_SLOTS];
 #pragma unroll
         for (int sl = 0; sl < PULL_SLOTS; sl++) {
           const uint32_t idx = r + 64 * sl + lane;
           cand[sl] = idx < total2;
-          uint32_t o = 0, bit = 0, rk = lbase;
-          if (r + 64 * sl < total2) {
-            wave_kth_bit(um, excl2, idx, o, bit);
-            rk = rank_of(o, bit);
-          }
+          uint32_t o = 0, bit = 0;
+          if (r + 64 * sl < total2) wave_kth_bit(um, excl2, idx, o, bit);
           off[sl] = cand[sl] ? o * 32 + bit : 0u;
-          fi[sl] = cand[sl] ? rk : lbase;
         }
         // (u is a vertex of this chunk below n_rows for every lane)
-        uint2 x[PULL_SLOTS];
-        PtrT cb[PULL_SLOTS];
+        PtrT cb[PULL_SLOTS], ce[PULL_SLOTS];
 #pragma unroll
         for (int sl = 0; sl < PULL_SLOTS; sl++) {
-          x[sl] = fd[fi[sl]];
           cb[sl] = col_ptr[c * PULL_CHUNK + off[sl]];
+          ce[sl] = col_ptr[c * PULL_CHUNK + off[sl] + 1];
         }
 #pragma unroll
         for (int sl = 0; sl < PULL_SLOTS; sl++) {
-          const PtrT deg = cand[sl] ? (PtrT)x[sl].y : PtrT(0);
+          const PtrT deg = cand[sl] ? ce[sl] - cb[sl] : PtrT(0);
           bool found = false;
           uint32_t par = 0;
           // (the first neighbour is done; not a candidate: an empty range)
@@ -2456,16 +2454,17 @@ static uint32_t pull_summary_shift(uint64_t n, uint32_t max_words) {
 template <typename PtrT, int WG>
 static void launch_pull(unsigned grid, size_t lds, hipStream_t st, const KArgs &a,
                         const StepArgs &s, const uint32_t *g_sum, uint32_t sum_words,
-                        uint32_t shift, const uint2 *fd, const uint32_t *iso,
+                        uint32_t shift, const uint32_t *fd, const uint16_t *fdd,
+                        const uint32_t *iso,
                         const uint32_t *crank, const uint32_t *hbm, const uint32_t *hub,
                         SpecGuard guard) {
   if (fd) {
     if (g_sum)
       hipLaunchKernelGGL((k_pull2<PtrT, WG, true>), dim3(grid), dim3(WG), lds, st, a, s.fbm_cur,
-                         s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+                         s.fbm_next, g_sum, sum_words, shift, fd, fdd, iso, crank, hbm, hub, guard);
     else
       hipLaunchKernelGGL((k_pull2<PtrT, WG, false>), dim3(grid), dim3(WG), lds, st, a, s.fbm_cur,
-                         s.fbm_next, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard);
+                         s.fbm_next, g_sum, sum_words, shift, fd, fdd, iso, crank, hbm, hub, guard);
   } else {
     if (g_sum)
       hipLaunchKernelGGL((k_pull<PtrT, WG, true>), dim3(grid), dim3(WG), lds, st, a, s.fbm_cur,
@@ -2514,9 +2513,10 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
   const unsigned grid = cap_grid(((uint64_t)n_chunks + nw - 1) / nw,
                                  (unsigned)ctx->cu_count * per_cu);
   // the {first neighbour, degree} array belongs to the hub-first copy
-  const uint2 *fd = ctx->pull_fd.p && s.row_idx == ctx->row_idx_pull.as<uint32_t>()
-                        ? ctx->pull_fd.as<uint2>()
-                        : nullptr;
+  const uint32_t *fd = ctx->pull_fd.p && s.row_idx == ctx->row_idx_pull.as<uint32_t>()
+                           ? ctx->pull_fd.as<uint32_t>()
+                           : nullptr;
+  const uint16_t *fdd = fd ? ctx->pull_fdd.as<uint16_t>() : nullptr;
   // (its entries exist for the vertices with neighbours: the others must
   // start visited, which a search does on a symmetric graph)
   if (fd && !((ctx->gflags & BFSX_GRAPH_SYMMETRIC) && ctx->n_rows == ctx->n_cols))
@@ -2534,10 +2534,10 @@ static void step_pull_t(bfsx_ctx *ctx, const StepArgs &s, bool speculative) {
   }
   KernelTimer kt(s, st);
   switch (wg) {
-    case 64: launch_pull<PtrT, 64>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard); break;
-    case 128: launch_pull<PtrT, 128>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard); break;
-    case 256: launch_pull<PtrT, 256>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard); break;
-    case 512: launch_pull<PtrT, 512>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, iso, crank, hbm, hub, guard); break;
+    case 64: launch_pull<PtrT, 64>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, fdd, iso, crank, hbm, hub, guard); break;
+    case 128: launch_pull<PtrT, 128>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, fdd, iso, crank, hbm, hub, guard); break;
+    case 256: launch_pull<PtrT, 256>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, fdd, iso, crank, hbm, hub, guard); break;
+    case 512: launch_pull<PtrT, 512>(grid, lds, st, a, s, g_sum, sum_words, shift, fd, fdd, iso, crank, hbm, hub, guard); break;
     default: fail(BFSX_ERR_ARG, "pull_wg must be 64, 128, 256 or 512");
   }
 }

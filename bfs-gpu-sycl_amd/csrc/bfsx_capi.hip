@@ -83,6 +83,7 @@ static void ensure_pull_state(bfsx_ctx *ctx) {
 // copy with hot neighbours named by index
 static void drop_pull_fd(bfsx_ctx *ctx) {
   release(ctx->pull_fd);
+  release(ctx->pull_fdd);
   release(ctx->pull_crank);
   release(ctx->pull_hub);
   release(ctx->pull_hbm);

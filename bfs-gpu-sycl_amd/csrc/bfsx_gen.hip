@@ -607,9 +607,11 @@ __global__ __launch_bounds__(1024) void k_scan_runs(uint32_t *cnt, uint32_t n) {
   }
 }
 
-// out[crank[c] + k] = fd[v] for the k-th vertex v with neighbours of chunk c
+// out[crank[c] + k] = fd[v].x and outd[...] = min(fd[v].y, DEG16_ESCAPE) for
+// the k-th vertex v with neighbours of chunk c
 __global__ void k_fd_compact(const uint2 *fd, const uint32_t *iso, uint32_t n_rows,
-                             uint32_t n_chunks, const uint32_t *crank, uint2 *out) {
+                             uint32_t n_chunks, const uint32_t *crank, uint32_t *out,
+                             uint16_t *outd) {
   constexpr uint32_t WPC = PULL_CHUNK / 32;
   for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n_chunks;
        c += gridDim.x * blockDim.x) {
@@ -619,7 +621,10 @@ __global__ void k_fd_compact(const uint2 *fd, const uint32_t *iso, uint32_t n_ro
       while (m) {
         const uint32_t b = __ffs(m) - 1;
         m &= m - 1;
-        out[pos++] = fd[(c * WPC + i) * 32 + b];
+        const uint2 x = fd[(c * WPC + i) * 32 + b];
+        out[pos] = x.x;
+        outd[pos] = (uint16_t)(x.y < DEG16_ESCAPE ? x.y : DEG16_ESCAPE);
+        pos++;
       }
     }
   }
@@ -627,6 +632,7 @@ __global__ void k_fd_compact(const uint2 *fd, const uint32_t *iso, uint32_t n_ro
 
 void compact_first_neighbours(bfsx_ctx *ctx) {
   release(ctx->pull_crank);
+  release(ctx->pull_fdd);
   if (!ctx->pull_fd.p) return;
   hipStream_t s = ctx->stream;
   const uint32_t n = ctx->n_rows;
@@ -643,11 +649,14 @@ void compact_first_neighbours(bfsx_ctx *ctx) {
   BFSX_HIP(hipStreamSynchronize(s));
   if (linked != n - ctx->n_isolated) fail(BFSX_ERR_DEVICE, "isolated-vertex count mismatch");
   DevBuf out;
-  ensure(out, sizeof(uint2) * ((uint64_t)linked + 1), s);
+  ensure(out, sizeof(uint32_t) * ((uint64_t)linked + 1), s);
+  ensure(ctx->pull_fdd, sizeof(uint16_t) * ((uint64_t)linked + 1), s);
   // the padding entry: a vertex with no neighbours (read, never used)
-  BFSX_HIP(hipMemsetAsync(out.as<uint2>() + linked, 0, sizeof(uint2), s));
+  BFSX_HIP(hipMemsetAsync(out.as<uint32_t>() + linked, 0, sizeof(uint32_t), s));
+  BFSX_HIP(hipMemsetAsync(ctx->pull_fdd.as<uint16_t>() + linked, 0, sizeof(uint16_t), s));
   hipLaunchKernelGGL(k_fd_compact, dim3(grid_for(n_chunks)), dim3(256), 0, s,
-                     ctx->pull_fd.as<uint2>(), iso, n, n_chunks, crank, out.as<uint2>());
+                     ctx->pull_fd.as<uint2>(), iso, n, n_chunks, crank, out.as<uint32_t>(),
+                     ctx->pull_fdd.as<uint16_t>());
   BFSX_HIP(hipGetLastError());
   BFSX_HIP(hipStreamSynchronize(s));
   std::swap(ctx->pull_fd, out);

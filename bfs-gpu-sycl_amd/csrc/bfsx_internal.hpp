@@ -156,13 +156,15 @@ struct bfsx_ctx {
   // vertex instead of two col_ptr entries (col_ptr only for the escapes)
   bfsx::DevBuf deg16;
   bfsx::DevBuf row_idx_pull;  // hub-first copy read by bottom-up steps (optional)
-  // With it: {first (highest-degree) neighbour, degree} for every vertex
-  // with neighbours, in vertex order and nothing for the isolated ones (a
+  // With it: the first (highest-degree) neighbour (uint32, pull_fd) and the
+  // degree (uint16, min(degree, DEG16_ESCAPE), pull_fdd) of every vertex with
+  // neighbours, in vertex order and nothing for the isolated ones (a
   // bottom-up step reaches only vertices with neighbours: the isolated ones
   // start visited).  The entry of v sits at pull_crank[v / PULL_CHUNK] plus
   // the vertices with neighbours before v in its PULL_CHUNK-vertex chunk
-  // (from the isolated-vertex bitmap); one padding entry follows.
-  bfsx::DevBuf pull_fd, pull_crank;
+  // (from the isolated-vertex bitmap); one padding entry follows.  While the
+  // arrays are built, pull_fd holds uint2 {first neighbour, degree} per vertex.
+  bfsx::DevBuf pull_fd, pull_fdd, pull_crank;
   // The pull_hubs vertices of largest degree ("hubs") are named in pull_fd by
   // their index: fd[v].x = PULL_HUB_FLAG | h for the hub pull_hub[h].  Before
   // every bottom-up step their frontier bits are gathered into pull_hbm
@@ -252,8 +254,10 @@ void first_neighbours(bfsx_ctx *ctx, const uint32_t *row_idx, uint2 *fd);
 // by their index in ctx->pull_hub (sets pull_hub, pull_hbm, pull_hubs); needs
 // ctx->pull_fd (one entry per vertex) and n_cols < 2^31.
 void index_hub_neighbours(bfsx_ctx *ctx);
-// Drops the isolated vertices' entries from ctx->pull_fd (one entry per
-// vertex before) and builds ctx->pull_crank (needs the isolated bitmap).
+// Drops the isolated vertices' entries from ctx->pull_fd (one uint2 entry per
+// vertex before), splits it into ctx->pull_fd (first neighbours) and
+// ctx->pull_fdd (16-bit degrees), and builds ctx->pull_crank (needs the
+// isolated bitmap).
 void compact_first_neighbours(bfsx_ctx *ctx);
 // Builds row_idx_hot / hot_id / hot_vis / hot_n for graphs of
 // HOT_MIN_VERTICES to 2^31 vertices (otherwise leaves hot_n = 0).
