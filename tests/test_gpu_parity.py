@@ -393,6 +393,38 @@ def test_trace_is_consistent(engine):
             assert rec["us"] > 0
 
 
+def test_frontier_edges_with_saturated_degrees(engine):
+    """Vertices of 65535 and more neighbours (their 16-bit degree saturates,
+    and the degree sums of new frontiers then read col_ptr) discovered by
+    every step kind, with the hub-first copy (first-neighbour data of the
+    two-phase bottom-up step) and without: the per-level frontier edge
+    counts of the trace equal the oracle's degree sums.  A root joined to
+    300 middle vertices, each joined to a hub of 70000 leaves, and an
+    isolated vertex."""
+    n_leaves, n_mid = 70000, 300
+    hub, root = 0, 1
+    mids = np.arange(2, 2 + n_mid)
+    leaves = np.arange(2 + n_mid, 2 + n_mid + n_leaves)
+    n = 2 + n_mid + n_leaves + 1
+    src = np.concatenate([np.full(n_mid, root), mids, np.full(n_leaves, hub)])
+    dst = np.concatenate([mids, np.full(n_mid, hub), leaves])
+    cp, ri = csc_from_edges(n, *symmetrize(src, dst))
+    deg = np.diff(cp.astype(np.int64))
+    assert deg[hub] >= 65535
+    engine.upload_csc(cp, ri, symmetric=True)
+    for order in ("id", "pull_hub_first"):
+        engine.order_adjacency(order)
+        for r in (root, int(leaves[5]), int(mids[7])):
+            want, ec, _ = cpu.bfs_queue(cp, ri, r)
+            hist = np.bincount(want[want >= 0])
+            for s in STEPS + ["learned"]:
+                lev, st = engine.run(r, s, trace=True)
+                assert np.array_equal(lev, want) and st["edges_count"] == ec, (order, r, s)
+                tr = engine.last_trace()
+                assert [t["frontier_edges"] for t in tr] == [
+                    int(deg[want == k].sum()) for k in range(len(hist))], (order, r, s)
+
+
 @pytest.mark.parametrize("drop", [0.05, 0.3])
 def test_largest_component_roots(engine, drop):
     """Seeded roots of the largest connected component equal the oracle's
