@@ -1247,11 +1247,13 @@ constexpr MarkRule NEVER_MARKS{~0ull, ~0ull};  // (frontier edges stay below 2^4
 // grid-stride: every wave runs whole rounds).
 __global__ void k_hub_bits(const uint32_t *fbm, const uint32_t *hub, uint32_t H, uint32_t *hbm,
                            SpecGuard guard) {
-  if (guard.skip()) return;
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t lane = threadIdx.x & 63;
-  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H; h += stride) {
-    const uint32_t v = hub[h];
+  const uint32_t h0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t v0 = h0 < H ? hub[h0] : 0u;  // in flight with the guard's reads
+  if (guard.skip()) return;
+  for (uint32_t h = h0; h < H; h += stride) {
+    const uint32_t v = h == h0 ? v0 : hub[h];
     const unsigned long long m = __ballot((fbm[v >> 5] >> (v & 31)) & 1u);
     if (lane == 0) hbm[h >> 5] = (uint32_t)m;
     if (lane == 32) hbm[h >> 5] = (uint32_t)(m >> 32);
@@ -1943,41 +1945,69 @@ __global__ void k_level_begin(LevelCtl *ctl) {
 // kind that ran; kind < 0: the one in ctl->choice.  A level with an empty
 // frontier (after the end, in a captured loop) changes nothing but the done
 // flag.
-__device__ __forceinline__ void level_end_body(LevelCtl *ctl, uint32_t consumed,
+__device__ __forceinline__ bool level_end_body(LevelCtl *ctl, uint32_t consumed,
                                                uint32_t produced, int stamp, int kind,
                                                const float *Wg, uint64_t n_cols, uint64_t nnz,
-                                               uint64_t n_iso, uint32_t allow, int reuse) {
-  __shared__ int s_live;
+                                               uint64_t n_iso, uint32_t allow, int reuse,
+                                               const SpecGuard &guard) {
+  __shared__ int s_live, s_skip;
+  __shared__ float4 s_W4[NPARAM / 4];
+  // the network's weights, loaded while thread 0 reads the control block
+  // (both are round trips to memory; one waits for the other otherwise)
+  constexpr int WPT = (NPARAM / 4 + 63) / 64;
+  float4 wr[WPT];
+  if (Wg && threadIdx.x < 64) {
+    const float4 *G4 = reinterpret_cast<const float4 *>(Wg);
+#pragma unroll
+    for (int i = 0; i < WPT; i++) {
+      const int j = (int)threadIdx.x + 64 * i;
+      wr[i] = G4[j < NPARAM / 4 ? j : 0];
+    }
+  }
   if (threadIdx.x == 0) {
+    // every field the bookkeeping reads, loaded together before any store
+    const uint32_t fc = ctl->q_count[consumed];
+    const uint32_t n = ctl->q_count[produced];
+    const uint32_t choice = ctl->choice;
+    const unsigned long long ef = ctl->q_edges[produced];
+    const unsigned long long vv = ctl->visited_vertices, ve = ctl->visited_edges;
+    const uint32_t level = ctl->level;
+    const uint32_t kc0 = ctl->kind_count[0], kc1 = ctl->kind_count[1],
+                   kc2 = ctl->kind_count[2], kc3 = ctl->kind_count[3];
+    const uint32_t f0 = ctl->dec_F, d_pull = ctl->dec_pull, d_valid = ctl->dec_valid;
+    const unsigned long long e0 = ctl->dec_EF;
+    const unsigned long long ec = ctl->q_edges[consumed];
+    // a speculative launch whose guess failed changes nothing
+    const bool skip = guard.ctl && (!((guard.kinds >> (choice & 31u)) & 1u) || fc == 0 ||
+                                    guard.heavy.marks(ec, fc));
+    s_skip = skip;
     s_live = 0;
-    if (ctl->q_count[consumed] == 0) {
+    if (skip) {
+      // nothing to do
+    } else if (fc == 0) {
       ctl->done = 1;
     } else {
       if (stamp) ctl->t_end = (unsigned long long)wall_clock64();
-      const uint32_t k = kind >= 0 ? (uint32_t)kind : ctl->choice;
-      if (k < 4) ctl->kind_count[k]++;
-      const uint32_t n = ctl->q_count[produced];
+      const uint32_t k = kind >= 0 ? (uint32_t)kind : choice;
+      if (k < 4) ctl->kind_count[k] = (k == 0 ? kc0 : k == 1 ? kc1 : k == 2 ? kc2 : kc3) + 1;
       ctl->discovered = n;
-      ctl->visited_vertices += n;
-      ctl->visited_edges += ctl->q_edges[produced];
+      ctl->visited_vertices = vv + n;
+      ctl->visited_edges = ve + ef;
       ctl->q_count[consumed] = 0;
       ctl->q_edges[consumed] = 0;
       ctl->glb_cursor = 0;
       ctl->conv = 0;
-      ctl->level += 1;
+      ctl->level = level + 1;
       ctl->done = n == 0;
       s_live = n != 0;
       if (s_live && Wg) {
         // keep the last decision while the frontier (vertices and edges)
         // stays within a factor 1.5 of the one it was computed for, after
         // the same kind of step
-        const unsigned long long ef = ctl->q_edges[produced];
         const uint32_t pull = k == BFSX_PULL;
-        const uint32_t f0 = ctl->dec_F;
-        const unsigned long long e0 = ctl->dec_EF;
         const bool close = (uint64_t)n * 3 >= (uint64_t)f0 * 2 && (uint64_t)n * 2 <= (uint64_t)f0 * 3 &&
                            ef * 3 >= e0 * 2 && ef * 2 <= e0 * 3;
-        if (reuse && ctl->dec_valid && ctl->dec_pull == pull && close) {
+        if (reuse && d_valid && d_pull == pull && close) {
           s_live = 0;
         } else {
           ctl->dec_F = n;
@@ -1988,19 +2018,34 @@ __device__ __forceinline__ void level_end_body(LevelCtl *ctl, uint32_t consumed,
       }
     }
   }
+  __syncthreads();
+  if (s_skip) return false;
   if (Wg) {  // uniform over the work-group
-    __syncthreads();
     // the learned scheduler picks the next level's step in the same launch
-    // (the weights are staged only when the network runs: a level that
-    // keeps the last decision costs what a level without the scheduler does)
-    if (s_live) decide_wave(ctl, Wg, produced, n_cols, nnz, n_iso, allow, 0);
+    // (a level that keeps the last decision skips the network)
+    if (s_live) {
+      if (threadIdx.x < 64) {
+#pragma unroll
+        for (int i = 0; i < WPT; i++) {
+          const int j = (int)threadIdx.x + 64 * i;
+          if (j < NPARAM / 4) s_W4[j] = wr[i];
+        }
+      }
+      __syncthreads();
+      decide_compute(ctl, s_W4, produced, n_cols, nnz, n_iso, allow, 0);
+    }
   }
+  return true;
 }
 
 // What the host needs for the next level, into the host-mapped words, each
 // tagged with a new sequence number (by the thread that made, or saw, every
 // write to the control block): the host polls them instead of synchronising
 // the stream.  `produced` = the queue buffer holding the next frontier.
+// No fence follows: the host checks every word by its own tag (device
+// writes to host memory can arrive in any order anyway), and a system-scope
+// fence here only kept the kernel, and the stream behind it, waiting for
+// the host's acknowledgement.
 __device__ __forceinline__ void write_level_end(LevelCtl *ctl, uint32_t produced,
                                                 LevelMirror *mirror) {
   const uint32_t seq = ctl->seq + 1;
@@ -2013,7 +2058,6 @@ __device__ __forceinline__ void write_level_end(LevelCtl *ctl, uint32_t produced
   mirror->w[2] = t16 | (ctl->visited_edges & MIRROR_48);
   mirror->w[3] = t32 | (ctl->choice & 0xFFu) | ((unsigned long long)(ctl->done ? 1u : 0u) << 8) |
                  ((unsigned long long)(ctl->level & MIRROR_LEVEL_MASK) << 9);
-  __threadfence_system();
 }
 
 __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consumed,
@@ -2021,8 +2065,9 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
                                                   const float *Wg, uint64_t n_cols, uint64_t nnz,
                                                   uint64_t n_iso, uint32_t allow, int reuse,
                                                   LevelMirror *mirror, SpecGuard guard) {
-  if (guard.skip()) return;
-  level_end_body(ctl, consumed, produced, stamp, kind, Wg, n_cols, nnz, n_iso, allow, reuse);
+  if (!level_end_body(ctl, consumed, produced, stamp, kind, Wg, n_cols, nnz, n_iso, allow, reuse,
+                      guard))
+    return;
   if (mirror && threadIdx.x == 0) write_level_end(ctl, produced, mirror);
 }
 
@@ -2112,7 +2157,8 @@ __global__ __launch_bounds__(WG) void k_coop_levels(KArgs base, LevelCtl *ctl, C
       llb_tile<PtrT, WG>(a, sh, tile, n_tiles, choice == BFSX_SLB, dacc);
     block_atomic_add<WG>(dacc, a.q_out_edges);
     grid_barrier_last(ctl, gridDim.x, [&] {
-      level_end_body(ctl, cur, cur ^ 1, 0, choice, c.Wg, c.n_cols, c.nnz, c.n_iso, c.allow, 1);
+      level_end_body(ctl, cur, cur ^ 1, 0, choice, c.Wg, c.n_cols, c.nnz, c.n_iso, c.allow, 1,
+                     SpecGuard{nullptr, 0, 0, NEVER_MARKS});
     });
     cur ^= 1;
   }
