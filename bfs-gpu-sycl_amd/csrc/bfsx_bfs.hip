@@ -348,7 +348,8 @@ struct LlbShared {
 template <typename PtrT, int WG, bool HOT = false>
 __device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh, uint32_t tile,
                                          uint32_t n_tiles, bool strided,
-                                         unsigned long long &dacc) {
+                                         unsigned long long &dacc, bool have_v0 = false,
+                                         uint32_t v0 = 0) {
   constexpr int NW = WG / 64;
   PtrT *s_start = sh.start;
   PtrT *s_scan = sh.scan;
@@ -364,7 +365,7 @@ __device__ __forceinline__ void llb_tile(const KArgs &a, LlbShared<PtrT, WG> &sh
   PtrT deg = 0, st = 0;
   uint32_t v = 0;
   if (idx < a.F) {
-    v = a.q_in[idx];
+    v = have_v0 ? v0 : a.q_in[idx];  // v0: this entry, read by the caller
     if (v < a.n_rows) {
       st = col_ptr[v];
       deg = col_ptr[v + 1] - st;
@@ -418,21 +419,39 @@ struct DevStep {
   int spec;                      // speculative: skip the frontiers `mark` marks
 };
 
+static_assert(offsetof(LevelCtl, q_count) == 0 && offsetof(LevelCtl, level) == 8 &&
+                  offsetof(LevelCtl, q_edges) == 16 && alignof(LevelCtl) >= 8,
+              "k_td_dev reads the control block's first 32 bytes as two uint4");
+
 template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_td_dev(KArgs a, LevelCtl *ctl, DevStep d) {
   __shared__ LlbShared<PtrT, WG> sh;
-  const uint32_t F = ctl->q_count[d.cur];
-  if (F == 0) return;
-  const int choice = d.fixed >= 0 ? d.fixed : (int)ctl->choice;
-  if (choice != BFSX_LLB && choice != BFSX_SLB) return;
+  // The control block's fields and this group's first LLB tile of the
+  // queue in one round trip (the queue holds n_cols + 1 entries, so the
+  // read is in bounds whatever F is; entries past F are not used).
+  // (the block's first 32 bytes as two 16-byte loads: counts, level, edges)
+  const uint4 h0 = reinterpret_cast<const uint4 *>(ctl)[0];
+  const uint4 h1 = reinterpret_cast<const uint4 *>(ctl)[1];
+  const uint32_t c_choice = ctl->choice;
+  const uint32_t idx0 = blockIdx.x * WG + threadIdx.x;
+  const uint32_t v0 = idx0 < a.n_cols ? a.q_in[idx0] : 0u;
+  const uint32_t F = d.cur ? h0.y : h0.x;
+  const unsigned long long c_edges =
+      d.cur ? ((unsigned long long)h1.w << 32 | h1.z) : ((unsigned long long)h1.y << 32 | h1.x);
+  const int choice = d.fixed >= 0 ? d.fixed : (int)c_choice;
+  const bool marks = d.mark.marks(c_edges, F);
+  // one branch on all of them, so that no read waits for another's test
+  const bool run = (F != 0) & ((choice == BFSX_LLB) | (choice == BFSX_SLB)) &
+                   !(marks & (d.spec != 0));
+  if (!run) return;
   a.F = F;
-  a.next_level = (int32_t)ctl->level + 1;
-  if (!d.mark.marks(ctl->q_edges[d.cur], F)) a.mark = nullptr;
-  else if (d.spec) return;
+  a.next_level = (int32_t)h0.z + 1;
+  if (!marks) a.mark = nullptr;
   const uint32_t n_tiles = (F + WG - 1) / WG;
+  const bool llb = choice == BFSX_LLB;
   unsigned long long dacc = 0;
   for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x)
-    llb_tile<PtrT, WG>(a, sh, tile, n_tiles, choice == BFSX_SLB, dacc);
+    llb_tile<PtrT, WG>(a, sh, tile, n_tiles, !llb, dacc, llb && tile == blockIdx.x, v0);
   if (a.q_out_edges && !a.mark) block_atomic_add<WG>(dacc, a.q_out_edges);
 }
 
