@@ -1964,11 +1964,20 @@ __global__ void k_level_begin(LevelCtl *ctl) {
 // kind that ran; kind < 0: the one in ctl->choice.  A level with an empty
 // frontier (after the end, in a captured loop) changes nothing but the done
 // flag.
+// The values of the host's level-end words (see write_level_end).
+struct LevelEndWords {
+  uint32_t F, choice, done, level, seq;
+  unsigned long long EF, VE;
+};
+
+// With `words`, thread 0 also leaves there what the control block holds
+// afterwards for the host's words, so that they need no second read of it.
 __device__ __forceinline__ bool level_end_body(LevelCtl *ctl, uint32_t consumed,
                                                uint32_t produced, int stamp, int kind,
                                                const float *Wg, uint64_t n_cols, uint64_t nnz,
                                                uint64_t n_iso, uint32_t allow, int reuse,
-                                               const SpecGuard &guard) {
+                                               const SpecGuard &guard,
+                                               LevelEndWords *words = nullptr) {
   __shared__ int s_live, s_skip;
   __shared__ float4 s_W4[NPARAM / 4];
   // the network's weights, loaded while thread 0 reads the control block
@@ -1996,6 +2005,7 @@ __device__ __forceinline__ bool level_end_body(LevelCtl *ctl, uint32_t consumed,
     const uint32_t f0 = ctl->dec_F, d_pull = ctl->dec_pull, d_valid = ctl->dec_valid;
     const unsigned long long e0 = ctl->dec_EF;
     const unsigned long long ec = ctl->q_edges[consumed];
+    const uint32_t seq0 = ctl->seq;
     // a speculative launch whose guess failed changes nothing
     const bool skip = guard.ctl && (!((guard.kinds >> (choice & 31u)) & 1u) || fc == 0 ||
                                     guard.heavy.marks(ec, fc));
@@ -2005,7 +2015,9 @@ __device__ __forceinline__ bool level_end_body(LevelCtl *ctl, uint32_t consumed,
       // nothing to do
     } else if (fc == 0) {
       ctl->done = 1;
+      if (words) *words = LevelEndWords{n, choice, 1u, level, seq0, ef, ve};
     } else {
+      if (words) *words = LevelEndWords{n, choice, n == 0 ? 1u : 0u, level + 1, seq0, ef, ve + ef};
       if (stamp) ctl->t_end = (unsigned long long)wall_clock64();
       const uint32_t k = kind >= 0 ? (uint32_t)kind : choice;
       if (k < 4) ctl->kind_count[k] = (k == 0 ? kc0 : k == 1 ? kc1 : k == 2 ? kc2 : kc3) + 1;
@@ -2051,7 +2063,8 @@ __device__ __forceinline__ bool level_end_body(LevelCtl *ctl, uint32_t consumed,
         }
       }
       __syncthreads();
-      decide_compute(ctl, s_W4, produced, n_cols, nnz, n_iso, allow, 0);
+      const uint32_t picked = decide_compute(ctl, s_W4, produced, n_cols, nnz, n_iso, allow, 0);
+      if (words && threadIdx.x == 0) words->choice = picked;
     }
   }
   return true;
@@ -2065,18 +2078,26 @@ __device__ __forceinline__ bool level_end_body(LevelCtl *ctl, uint32_t consumed,
 // writes to host memory can arrive in any order anyway), and a system-scope
 // fence here only kept the kernel, and the stream behind it, waiting for
 // the host's acknowledgement.
-__device__ __forceinline__ void write_level_end(LevelCtl *ctl, uint32_t produced,
-                                                LevelMirror *mirror) {
-  const uint32_t seq = ctl->seq + 1;
+__device__ __forceinline__ void put_level_end(LevelCtl *ctl, const LevelEndWords &v,
+                                              LevelMirror *mirror) {
+  const uint32_t seq = v.seq + 1;
   ctl->seq = seq;
   const unsigned long long t32 = (unsigned long long)seq << 32;
   const unsigned long long t16 = (unsigned long long)(seq & 0xFFFFu) << 48;
   mirror += seq & 1u;  // the set for this parity
-  mirror->w[0] = t32 | ctl->q_count[produced];
-  mirror->w[1] = t16 | (ctl->q_edges[produced] & MIRROR_48);
-  mirror->w[2] = t16 | (ctl->visited_edges & MIRROR_48);
-  mirror->w[3] = t32 | (ctl->choice & 0xFFu) | ((unsigned long long)(ctl->done ? 1u : 0u) << 8) |
-                 ((unsigned long long)(ctl->level & MIRROR_LEVEL_MASK) << 9);
+  mirror->w[0] = t32 | v.F;
+  mirror->w[1] = t16 | (v.EF & MIRROR_48);
+  mirror->w[2] = t16 | (v.VE & MIRROR_48);
+  mirror->w[3] = t32 | (v.choice & 0xFFu) | ((unsigned long long)(v.done ? 1u : 0u) << 8) |
+                 ((unsigned long long)(v.level & MIRROR_LEVEL_MASK) << 9);
+}
+
+__device__ __forceinline__ void write_level_end(LevelCtl *ctl, uint32_t produced,
+                                                LevelMirror *mirror) {
+  put_level_end(ctl,
+                LevelEndWords{ctl->q_count[produced], ctl->choice, ctl->done, ctl->level,
+                              ctl->seq, ctl->q_edges[produced], ctl->visited_edges},
+                mirror);
 }
 
 __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consumed,
@@ -2084,10 +2105,11 @@ __global__ __launch_bounds__(64) void k_level_end(LevelCtl *ctl, uint32_t consum
                                                   const float *Wg, uint64_t n_cols, uint64_t nnz,
                                                   uint64_t n_iso, uint32_t allow, int reuse,
                                                   LevelMirror *mirror, SpecGuard guard) {
+  LevelEndWords words;
   if (!level_end_body(ctl, consumed, produced, stamp, kind, Wg, n_cols, nnz, n_iso, allow, reuse,
-                      guard))
+                      guard, &words))
     return;
-  if (mirror && threadIdx.x == 0) write_level_end(ctl, produced, mirror);
+  if (mirror && threadIdx.x == 0) put_level_end(ctl, words, mirror);
 }
 
 // ---------------------------------------------------------------------------

@@ -23,7 +23,8 @@ __device__ __forceinline__ float flog2ratio(double a, double b) {
 // decide_compute: the inputs of
 // frontier buffer `idx` from the control block, the three layers, and
 // ctl->choice = argmin over the allowed kinds (fixed LLB when an output is
-// NaN); inputs and outputs are recorded in the control block.
+// NaN); inputs and outputs are recorded in the control block.  Returns the
+// choice in thread 0 (0 in the others).
 //   x0 = log2((1+F)/(1+n))/32          x1 = log2((1+EF)/(1+nnz))/32
 //   x2 = log2((1+UV)/(1+n))/32         x3 = log2((1+UE)/(1+nnz))/32
 //   x4 = log2(1+EF/F)/16               x5 = log2(1+nnz/n)/16
@@ -35,7 +36,7 @@ __device__ __forceinline__ void decide_stage(const float *Wg, float4 *W4) {
   for (int i = threadIdx.x; i < NPARAM / 4; i += 64) W4[i] = G4[i];
 }
 
-__device__ __forceinline__ void decide_compute(LevelCtl *ctl, const float4 *W4, uint32_t idx,
+__device__ __forceinline__ uint32_t decide_compute(LevelCtl *ctl, const float4 *W4, uint32_t idx,
                                                uint64_t n_cols, uint64_t nnz, uint64_t n_iso,
                                                uint32_t allow_mask, int first) {
   __shared__ float x[IN], h1[H1], h2[H2], y[OUT];
@@ -77,6 +78,7 @@ __device__ __forceinline__ void decide_compute(LevelCtl *ctl, const float4 *W4, 
     y[l] = s;
   }
   __syncthreads();
+  uint32_t picked = 0;
   if (l == 0) {
     uint32_t best = 0;
     float bv = INFINITY;
@@ -90,9 +92,11 @@ __device__ __forceinline__ void decide_compute(LevelCtl *ctl, const float4 *W4, 
       }
     }
     for (int k = 0; k < IN; k++) ctl->sched_in[k] = x[k];
-    ctl->choice = ok && bv < INFINITY ? best : (uint32_t)BFSX_LLB;
+    picked = ok && bv < INFINITY ? best : (uint32_t)BFSX_LLB;
+    ctl->choice = picked;
     ctl->decisions += 1;
   }
+  return picked;
 }
 
 __device__ __forceinline__ void decide_wave(LevelCtl *ctl, const float *Wg, uint32_t idx,
