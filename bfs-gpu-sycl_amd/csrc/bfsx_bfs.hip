@@ -426,10 +426,11 @@ static_assert(offsetof(LevelCtl, q_count) == 0 && offsetof(LevelCtl, level) == 8
 template <typename PtrT, int WG>
 __global__ __launch_bounds__(WG) void k_td_dev(KArgs a, LevelCtl *ctl, DevStep d) {
   __shared__ LlbShared<PtrT, WG> sh;
-  // The control block's fields and this group's first LLB tile of the
-  // queue in one round trip (the queue holds n_cols + 1 entries, so the
-  // read is in bounds whatever F is; entries past F are not used).
-  // (the block's first 32 bytes as two 16-byte loads: counts, level, edges)
+  // The control block's fields (its first 32 bytes, counts, level and
+  // edges, as two 16-byte loads, and the choice) and this group's first LLB
+  // tile of the queue in one round trip: the queue holds n_cols + 1
+  // entries, so the read is in bounds whatever F is; entries past F are not
+  // used.
   const uint4 h0 = reinterpret_cast<const uint4 *>(ctl)[0];
   const uint4 h1 = reinterpret_cast<const uint4 *>(ctl)[1];
   const uint32_t c_choice = ctl->choice;
@@ -1959,19 +1960,20 @@ __global__ void k_level_begin(LevelCtl *ctl) {
   if (threadIdx.x == 0) ctl->t_begin = (unsigned long long)wall_clock64();
 }
 
-// Bookkeeping at the end of a level, by one work-group (every thread calls
-// it; the scheduler's inference uses the first 64).  kind >= 0: the step
-// kind that ran; kind < 0: the one in ctl->choice.  A level with an empty
-// frontier (after the end, in a captured loop) changes nothing but the done
-// flag.
 // The values of the host's level-end words (see write_level_end).
 struct LevelEndWords {
   uint32_t F, choice, done, level, seq;
   unsigned long long EF, VE;
 };
 
-// With `words`, thread 0 also leaves there what the control block holds
-// afterwards for the host's words, so that they need no second read of it.
+// Bookkeeping at the end of a level, by one work-group (every thread calls
+// it; the scheduler's inference uses the first 64).  kind >= 0: the step
+// kind that ran; kind < 0: the one in ctl->choice.  A level with an empty
+// frontier (after the end, in a captured loop) changes nothing but the done
+// flag.  Returns false, having changed nothing, when `guard` says that a
+// speculative launch's guess failed.  With `words`, thread 0 also leaves
+// there what the control block holds afterwards for the host's words, so
+// that they need no second read of it.
 __device__ __forceinline__ bool level_end_body(LevelCtl *ctl, uint32_t consumed,
                                                uint32_t produced, int stamp, int kind,
                                                const float *Wg, uint64_t n_cols, uint64_t nnz,
