@@ -270,15 +270,37 @@ __device__ __forceinline__ void expand_edges(const KArgs &a, const PtrT *col_ptr
       bool valid[EPT];
       load_edges<PtrT, WG>(a, s_scan, s_start, s_v, r0 + (PtrT)threadIdx.x * EPT, hi, n, from,
                            valid);
-      bool isnew[EPT];
+      // The EPT discoveries of a lane as three waves of memory operations
+      // instead of a chain: every visited word, then an atomicOr for each
+      // target that looked unvisited, then (tracking edges) the degrees of
+      // those targets, all issued before the first result is used (each
+      // read would otherwise wait for the previous edge's atomic, which may
+      // alias it).  Two edges of one lane to one vertex stay exact-once:
+      // a thread's atomics on one address take effect in order.
+      uint32_t word[EPT], bit[EPT], old[EPT];
+      bool cand[EPT], isnew[EPT];
+#pragma unroll
+      for (int k = 0; k < EPT; k++) {
+        bit[k] = 1u << (n[k] & 31);
+        word[k] = valid[k] ? a.vis[n[k] >> 5] : ~0u;
+      }
+#pragma unroll
+      for (int k = 0; k < EPT; k++) {
+        cand[k] = valid[k] && !(word[k] & bit[k]);
+        old[k] = cand[k] ? atomicOr(&a.vis[n[k] >> 5], bit[k]) : bit[k];
+      }
+      unsigned long long deg[EPT];
+#pragma unroll
+      for (int k = 0; k < EPT; k++)
+        deg[k] = (a.q_out_edges && cand[k]) ? out_degree(col_ptr, n[k], a.n_rows) : 0ull;
       uint32_t cnt = 0;
 #pragma unroll
       for (int k = 0; k < EPT; k++) {
-        isnew[k] = valid[k] && try_visit(a.vis, n[k]);
+        isnew[k] = !(old[k] & bit[k]);
         if (isnew[k]) {
           a.levels[n[k]] = a.next_level;
           if (a.parents) a.parents[n[k]] = from[k];
-          if (a.q_out_edges) dacc += out_degree(col_ptr, n[k], a.n_rows);
+          dacc += deg[k];
           cnt++;
         }
       }
